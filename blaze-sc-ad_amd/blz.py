@@ -1,0 +1,292 @@
+"""Python binding of the MI355X BLASTP prelim engine (ctypes over include/blz_prelim.h).
+
+Host-side mirror of the reference's hook for this path: the reference calls
+``BLAZE_GPU(seq_src, threadIndex, totalThreads, queryLen, survivingSeqs, num_survivors,
+ungapped_cutoff, gapped_cutoff)`` (BLAZE_gpu/gpu_cpu_common.h:43) and then re-runs NCBI's CPU
+prelim core on the survivors (BLAZE_ncbi/blast_engine.c:2983-2996).  Here one call returns the
+per-subject HSP lists that core would write to the HSP stream (blast_engine.c:2499).
+
+The engine only runs on a gfx950 device through ``libblaze_amd.so``; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+
+import numpy as np
+
+_HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libblaze_amd.so"
+
+HSP_DTYPE = np.dtype(
+    [("score", "<i4"), ("q_off", "<i4"), ("q_end", "<i4"), ("s_off", "<i4"), ("s_end", "<i4"),
+     ("q_gs", "<i4"), ("s_gs", "<i4")]
+)
+
+STATUS = {0: "BLZ_OK", 1: "BLZ_EINVAL", 2: "BLZ_ENOMEM", 3: "BLZ_EDEVICE", 4: "BLZ_ECAPACITY",
+          5: "BLZ_EOVERFLOW"}
+
+
+class BlzError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Params(C.Structure):
+    _fields_ = [("word_threshold", C.c_int32), ("window", C.c_int32), ("x_ungapped", C.c_int32),
+                ("x_gapped", C.c_int32), ("gap_open", C.c_int32), ("gap_extend", C.c_int32),
+                ("ungapped_cutoff", C.c_int32), ("gapped_cutoff", C.c_int32)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("subjects", C.c_uint64), ("residues", C.c_uint64), ("word_hits", C.c_uint64),
+                ("ungapped_ext", C.c_uint64), ("init_hits", C.c_uint64),
+                ("gapped_subjects", C.c_uint64), ("gapped_ext", C.c_uint64),
+                ("dp_cells", C.c_uint64), ("hsps", C.c_uint64), ("overflow_subjects", C.c_uint64),
+                ("ms_seed", C.c_double), ("ms_gapped", C.c_double), ("ms_total", C.c_double),
+                ("seed_launches", C.c_int32), ("gapped_launches", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Sink(C.Structure):
+    _fields_ = [("oids", C.POINTER(C.c_int64)), ("counts", C.POINTER(C.c_int32)),
+                ("subject_cap", C.c_int64), ("hsps", C.c_void_p), ("hsp_cap", C.c_int64),
+                ("n_subjects", C.c_int64), ("n_hsps", C.c_int64)]
+
+
+class SynthSpec(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("n_subjects", C.c_int64), ("len_median", C.c_double),
+                ("len_sigma", C.c_double), ("len_min", C.c_int32), ("len_max", C.c_int32),
+                ("homolog_frac", C.c_double), ("ident_min", C.c_double), ("ident_max", C.c_double),
+                ("indel_rate", C.c_double), ("indel_mean", C.c_double), ("repeat_frac", C.c_double),
+                ("repeat_period_min", C.c_int32), ("repeat_period_max", C.c_int32),
+                ("sort_by_length", C.c_int32)]
+
+
+# every symbol include/blz_prelim.h declares
+EXPORTS = [
+    "blz_ctx_create", "blz_ctx_destroy", "blz_last_error", "blz_ctx_stream", "blz_default_params",
+    "blz_db_upload", "blz_db_free", "blz_db_num_subjects", "blz_db_num_residues",
+    "blz_query_prepare", "blz_query_free", "blz_prelim_run", "blz_prelim_run_batch",
+    "blz_synth_default", "blz_synth_query", "blz_synth_db_size", "blz_synth_db_fill",
+]
+
+_lib = None
+
+
+def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
+    """Load libblaze_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = pathlib.Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise FileNotFoundError(f"{p} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(str(p))
+    vp, i32, i64, u64 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64
+    lib.blz_ctx_create.argtypes = [i32, C.POINTER(vp)]
+    lib.blz_ctx_create.restype = C.c_int
+    lib.blz_ctx_destroy.argtypes = [vp]
+    lib.blz_last_error.argtypes = [vp]
+    lib.blz_last_error.restype = C.c_char_p
+    lib.blz_ctx_stream.argtypes = [vp]
+    lib.blz_ctx_stream.restype = vp
+    lib.blz_default_params.argtypes = [C.POINTER(Params)]
+    lib.blz_db_upload.argtypes = [vp, vp, vp, i64, C.POINTER(vp)]
+    lib.blz_db_upload.restype = C.c_int
+    lib.blz_db_free.argtypes = [vp]
+    lib.blz_db_num_subjects.argtypes = [vp]
+    lib.blz_db_num_subjects.restype = i64
+    lib.blz_db_num_residues.argtypes = [vp]
+    lib.blz_db_num_residues.restype = u64
+    lib.blz_query_prepare.argtypes = [vp, vp, i32, C.POINTER(Params), C.POINTER(vp)]
+    lib.blz_query_prepare.restype = C.c_int
+    lib.blz_query_free.argtypes = [vp]
+    lib.blz_prelim_run.argtypes = [vp, vp, vp, C.POINTER(Sink), C.POINTER(RunStats)]
+    lib.blz_prelim_run.restype = C.c_int
+    lib.blz_prelim_run_batch.argtypes = [vp, C.POINTER(vp), i32, vp, C.POINTER(Sink), C.POINTER(RunStats)]
+    lib.blz_prelim_run_batch.restype = C.c_int
+    lib.blz_synth_default.argtypes = [C.POINTER(SynthSpec)]
+    lib.blz_synth_query.argtypes = [u64, i32, vp]
+    lib.blz_synth_db_size.argtypes = [C.POINTER(SynthSpec), C.POINTER(i64), C.POINTER(u64)]
+    lib.blz_synth_db_size.restype = C.c_int
+    lib.blz_synth_db_fill.argtypes = [C.POINTER(SynthSpec), C.POINTER(vp), vp, i32, vp, vp, vp]
+    lib.blz_synth_db_fill.restype = C.c_int
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    load_library().blz_default_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+# ---------------------------------------------------------------------------------------
+# synthetic data
+# ---------------------------------------------------------------------------------------
+def synth_spec(**kw) -> SynthSpec:
+    s = SynthSpec()
+    load_library().blz_synth_default(C.byref(s))
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
+def synth_query(seed: int, length: int) -> np.ndarray:
+    out = np.empty(length, dtype=np.uint8)
+    load_library().blz_synth_query(seed, length, out.ctypes.data)
+    return out
+
+
+def synth_db(spec: SynthSpec, queries: list[np.ndarray]):
+    """Returns (residues uint8, offsets uint64[n+1], homolog_of int32[n])."""
+    lib = load_library()
+    n = C.c_int64()
+    r = C.c_uint64()
+    st = lib.blz_synth_db_size(C.byref(spec), C.byref(n), C.byref(r))
+    if st:
+        raise BlzError(st, "synth_db_size")
+    residues = np.empty(r.value + 64, dtype=np.uint8)
+    offsets = np.empty(n.value + 1, dtype=np.uint64)
+    hom = np.empty(n.value, dtype=np.int32)
+    qs = [np.ascontiguousarray(q, dtype=np.uint8) for q in queries]
+    qptr = (C.c_void_p * max(1, len(qs)))(*[q.ctypes.data for q in qs])
+    qlen = np.array([len(q) for q in qs] or [0], dtype=np.int32)
+    st = lib.blz_synth_db_fill(C.byref(spec), qptr, qlen.ctypes.data, len(qs), residues.ctypes.data,
+                               offsets.ctypes.data, hom.ctypes.data)
+    if st:
+        raise BlzError(st, "synth_db_fill")
+    return residues[: r.value], offsets, hom
+
+
+# ---------------------------------------------------------------------------------------
+# engine
+# ---------------------------------------------------------------------------------------
+class Result:
+    """Per-subject HSP lists of one query: oids (ascending), counts, hsps (subject-major)."""
+
+    def __init__(self, oids, counts, hsps):
+        self.oids = oids
+        self.counts = counts
+        self.hsps = hsps
+
+    def as_dict(self) -> dict:
+        out, o = {}, 0
+        for oid, c in zip(self.oids.tolist(), self.counts.tolist()):
+            out[oid] = self.hsps[o:o + c]
+            o += c
+        return out
+
+
+class Engine:
+    """One HIP device, one stream, a resident database and prepared queries."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        st = self.lib.blz_ctx_create(device, C.byref(h))
+        if st:
+            raise BlzError(st, "blz_ctx_create: no usable gfx950 device")
+        self.ctx = h
+        self.db = None
+        self.db_n = 0
+
+    def _err(self, st: int, what: str):
+        msg = self.lib.blz_last_error(self.ctx)
+        raise BlzError(st, f"{what}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self.db:
+            self.lib.blz_db_free(self.db)
+            self.db = None
+        if self.ctx:
+            self.lib.blz_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def stream(self) -> int:
+        return self.lib.blz_ctx_stream(self.ctx)
+
+    def upload_db(self, residues: np.ndarray, offsets: np.ndarray):
+        residues = np.ascontiguousarray(residues, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        if self.db:
+            self.lib.blz_db_free(self.db)
+            self.db = None
+        h = C.c_void_p()
+        n = len(offsets) - 1
+        st = self.lib.blz_db_upload(self.ctx, residues.ctypes.data, offsets.ctypes.data, n, C.byref(h))
+        if st:
+            self._err(st, "blz_db_upload")
+        self.db = h
+        self.db_n = n
+
+    def prepare(self, query: np.ndarray, params: Params | None = None):
+        query = np.ascontiguousarray(query, dtype=np.uint8)
+        h = C.c_void_p()
+        st = self.lib.blz_query_prepare(self.ctx, query.ctypes.data, len(query),
+                                        C.byref(params) if params is not None else None, C.byref(h))
+        if st:
+            self._err(st, "blz_query_prepare")
+        return h
+
+    def free_query(self, qh):
+        self.lib.blz_query_free(qh)
+
+    @staticmethod
+    def _sink(n_subj_cap: int, hsp_cap: int):
+        oids = np.empty(max(1, n_subj_cap), dtype=np.int64)
+        counts = np.empty(max(1, n_subj_cap), dtype=np.int32)
+        hsps = np.empty(max(1, hsp_cap), dtype=HSP_DTYPE)
+        s = Sink(oids.ctypes.data_as(C.POINTER(C.c_int64)), counts.ctypes.data_as(C.POINTER(C.c_int32)),
+                 n_subj_cap, hsps.ctypes.data, hsp_cap, 0, 0)
+        return s, (oids, counts, hsps)
+
+    def run(self, qh, allow_overflow: bool = False, hsp_cap: int = 1 << 20):
+        """Prelim stage of one prepared query over the resident database -> (Result, stats)."""
+        stats = RunStats()
+        sink, keep = self._sink(self.db_n, hsp_cap)
+        st = self.lib.blz_prelim_run(self.ctx, qh, self.db, C.byref(sink), C.byref(stats))
+        if st == 4:  # capacity: grow and run again
+            sink, keep = self._sink(self.db_n, int(sink.n_hsps))
+            stats = RunStats()
+            st = self.lib.blz_prelim_run(self.ctx, qh, self.db, C.byref(sink), C.byref(stats))
+        if st and not (st == 5 and allow_overflow):
+            self._err(st, "blz_prelim_run")
+        oids, counts, hsps = keep
+        ns, nh = int(sink.n_subjects), int(sink.n_hsps)
+        return Result(oids[:ns].copy(), counts[:ns].copy(), hsps[:nh].copy()), stats.as_dict()
+
+    def run_batch(self, qhs, allow_overflow: bool = False, hsp_cap: int = 1 << 20):
+        n = len(qhs)
+        stats = RunStats()
+        sinks = (Sink * n)()
+        keeps = []
+        for i in range(n):
+            s, k = self._sink(self.db_n, hsp_cap)
+            sinks[i] = s
+            keeps.append(k)
+        arr = (C.c_void_p * n)(*[q.value if isinstance(q, C.c_void_p) else q for q in qhs])
+        st = self.lib.blz_prelim_run_batch(self.ctx, arr, n, self.db, sinks, C.byref(stats))
+        if st and not (st == 5 and allow_overflow):
+            self._err(st, "blz_prelim_run_batch")
+        out = []
+        for i in range(n):
+            oids, counts, hsps = keeps[i]
+            ns, nh = int(sinks[i].n_subjects), int(sinks[i].n_hsps)
+            out.append(Result(oids[:ns].copy(), counts[:ns].copy(), hsps[:nh].copy()))
+        return out, stats.as_dict()

@@ -1,0 +1,50 @@
+// blz_common.h -- constants and record layouts shared by the host runtime and the kernels.
+#pragma once
+#include <stdint.h>
+
+namespace blz {
+
+constexpr int kAlpha = 28;           // NCBIstdaa letters
+constexpr int kWords = 28 * 28 * 28; // word index a*784 + b*28 + c (gpuBLAZE.cu:991)
+constexpr int kWord = 3;             // word size
+constexpr int kHspMaxWindow = 11;    // gapped-start window (gpuBLAZE.cu:64)
+constexpr int kMinIntDp = -1073741824; // INT_MIN/2, the DP's "minus infinity"
+
+// BLOSUM62 in NCBIstdaa order (- A B C D E F G H I K L M N P Q R S T V W X Y Z U * O J),
+// the reference's table (BLAZE_gpu/util.hpp:30-57); letter 0 (gap) scores -32768.
+extern const int16_t kBlosum62[28][28];
+
+// Ungapped HSP that reached the ungapped cutoff ("init hit"): the segment and its score.
+struct InitHit {
+    int32_t q_start, s_start, len, score;
+};
+
+// Subject that reached the gapped stage: its init hits are pool[off, off+n).
+struct WorkItem {
+    int64_t sid;
+    uint64_t off;
+    int32_t n;
+    int32_t pad;
+};
+
+// HSP record, same layout as blz_hsp in blz_prelim.h.
+struct Hsp {
+    int32_t score, q_off, q_end, s_off, s_end, q_gs, s_gs;
+};
+
+// Subject's HSP list on the device: hsps[off, off+n).
+struct ResultItem {
+    int64_t sid;
+    uint64_t off;
+    int32_t n;
+    int32_t pad;
+};
+
+// Device counters of one run (atomically accumulated by the kernels).
+struct DevCounters {
+    unsigned long long word_hits, ungapped_ext, init_hits, gapped_ext, dp_cells;
+    unsigned long long pool_used, hsp_used;
+    unsigned int work_count, result_count, overflow_count, next_subject, next_work, pad;
+};
+
+}  // namespace blz

@@ -1,0 +1,503 @@
+// blz_host.cpp -- host runtime behind the C ABI in include/blz_prelim.h.
+//
+// Owns the device (one context = one HIP device + one stream + reusable work buffers), the
+// resident database, the per-query lookup tables, and the two-stage launch sequence.
+// Nothing here falls back to a CPU implementation: if the device path cannot run, the call
+// returns an error status.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "blz_common.h"
+#include "blz_device.h"
+#include "blz_prelim.h"
+
+namespace blz {
+const int16_t kBlosum62[28][28] = {
+#include "blz_blosum62.inc"
+};
+}  // namespace blz
+
+using namespace blz;
+
+struct blz_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int num_cus = 0;
+    size_t lds_per_block = 0;
+    std::string err;
+    // reusable device buffers
+    DevCounters *ctr = nullptr;
+    InitHit *stage = nullptr;
+    size_t stage_records = 0;
+    InitHit *pool = nullptr;
+    size_t pool_cap = 0;
+    WorkItem *work = nullptr;
+    size_t work_cap = 0;
+    int64_t *ovf = nullptr;
+    size_t ovf_cap = 0;
+    int2 *dp = nullptr;
+    size_t dp_entries = 0;
+    Hsp *saved = nullptr;
+    size_t saved_records = 0;
+    Hsp *hsps = nullptr;
+    size_t hsp_cap = 0;
+    ResultItem *results = nullptr;
+    size_t results_cap = 0;
+};
+
+struct blz_db {
+    blz_ctx *ctx = nullptr;
+    uint8_t *res = nullptr;     // device
+    uint64_t *offs = nullptr;   // device
+    int64_t nsubj = 0;
+    uint64_t nres = 0;
+};
+
+struct blz_query {
+    blz_ctx *ctx = nullptr;
+    int32_t qlen = 0;
+    blz_params params{};
+    uint8_t *d_query = nullptr;
+    uint32_t *d_prefix = nullptr;
+    uint16_t *d_entries = nullptr;
+    uint64_t n_entries = 0;
+};
+
+namespace {
+
+constexpr int kStageCap = 1024;   // init hits per subject kept on the device
+constexpr int kDpRing = 1024;     // DP band ring per lane (power of two)
+constexpr int kSavedCap = 128;    // saved gapped HSPs per subject
+constexpr int kSeedWavesMax = 4;
+constexpr int kGappedBlocks = 1024;
+
+bool check(blz_ctx *ctx, hipError_t e, const char *what) {
+    if (e == hipSuccess) return true;
+    ctx->err = std::string(what) + ": " + hipGetErrorString(e);
+    return false;
+}
+
+template <class T>
+bool ensure(blz_ctx *ctx, T *&ptr, size_t &cap, size_t need, const char *what) {
+    if (need <= cap && ptr) return true;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    size_t n = std::max<size_t>(need, 1);
+    if (!check(ctx, hipMalloc((void **)&ptr, n * sizeof(T)), what)) return false;
+    cap = n;
+    return true;
+}
+
+// Neighbourhoods of all 28^3 words at one threshold, computed once per process.
+struct Neighbourhoods {
+    int threshold = -1;
+    std::vector<uint32_t> start;  // [kWords+1]
+    std::vector<uint16_t> words;
+};
+std::mutex g_nb_mutex;
+Neighbourhoods g_nb;
+
+// {u : score(w,u) >= T} plus w itself (prelim_stage.cpp:135-155 and :378-386).
+void build_neighbourhoods(int T, Neighbourhoods &nb) {
+    nb.threshold = T;
+    nb.start.assign(kWords + 1, 0);
+    nb.words.clear();
+    nb.words.reserve(800000);
+    int rowmax[28];
+    for (int x = 0; x < 28; x++) {
+        rowmax[x] = -1000000;
+        for (int i = 0; i < 28; i++) rowmax[x] = std::max<int>(rowmax[x], kBlosum62[x][i]);
+    }
+    for (int w = 0; w < kWords; w++) {
+        const int x = w / 784, y = (w / 28) % 28, z = w % 28;
+        nb.start[w] = (uint32_t)nb.words.size();
+        for (int a = 0; a < 28; a++) {
+            const int sa = kBlosum62[x][a];
+            if (sa + rowmax[y] + rowmax[z] < T) continue;
+            for (int b = 0; b < 28; b++) {
+                const int sb = sa + kBlosum62[y][b];
+                if (sb + rowmax[z] < T) continue;
+                for (int c = 0; c < 28; c++)
+                    if (sb + kBlosum62[z][c] >= T) nb.words.push_back((uint16_t)(a * 784 + b * 28 + c));
+            }
+        }
+        if (kBlosum62[x][x] + kBlosum62[y][y] + kBlosum62[z][z] < T) nb.words.push_back((uint16_t)w);
+    }
+    nb.start[kWords] = (uint32_t)nb.words.size();
+}
+
+const Neighbourhoods &neighbourhoods(int T) {
+    std::lock_guard<std::mutex> lk(g_nb_mutex);
+    if (g_nb.threshold != T) build_neighbourhoods(T, g_nb);
+    return g_nb;
+}
+
+uint32_t ring_words_for(int qlen, int window) {
+    uint32_t need = (uint32_t)qlen + 64u + (uint32_t)window + 64u;
+    uint32_t m = 256;
+    while (m < need) m <<= 1;
+    return m;
+}
+
+}  // namespace
+
+extern "C" {
+
+void blz_default_params(blz_params *p) {
+    p->word_threshold = 11;
+    p->window = 40;
+    p->x_ungapped = 16;
+    p->x_gapped = 38;
+    p->gap_open = 11;
+    p->gap_extend = 1;
+    p->ungapped_cutoff = 41;
+    p->gapped_cutoff = 67;
+}
+
+blz_status blz_ctx_create(int32_t device, blz_ctx **out) {
+    if (!out) return BLZ_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0 || device < 0 || device >= n) return BLZ_EDEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return BLZ_EDEVICE;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return BLZ_EDEVICE;
+    blz_ctx *ctx = new blz_ctx();
+    ctx->device = device;
+    ctx->num_cus = prop.multiProcessorCount;
+    ctx->lds_per_block = prop.sharedMemPerBlock;
+    if (!check(ctx, hipSetDevice(device), "hipSetDevice") ||
+        !check(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "stream")) {
+        delete ctx;
+        return BLZ_EDEVICE;
+    }
+    for (auto &e : ctx->ev)
+        if (!check(ctx, hipEventCreate(&e), "event")) {
+            delete ctx;
+            return BLZ_EDEVICE;
+        }
+    if (!check(ctx, hipMalloc((void **)&ctx->ctr, sizeof(DevCounters)), "counters")) {
+        delete ctx;
+        return BLZ_ENOMEM;
+    }
+    *out = ctx;
+    return BLZ_OK;
+}
+
+void blz_ctx_destroy(blz_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    void *bufs[] = {ctx->ctr, ctx->stage, ctx->pool, ctx->work, ctx->ovf, ctx->dp, ctx->saved, ctx->hsps, ctx->results};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *blz_last_error(const blz_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+void *blz_ctx_stream(blz_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+blz_status blz_db_upload(blz_ctx *ctx, const uint8_t *residues, const uint64_t *offsets, int64_t nsubj,
+                         blz_db **out) {
+    if (!ctx || !out || !offsets || nsubj < 0 || (nsubj > 0 && !residues)) return BLZ_EINVAL;
+    *out = nullptr;
+    if (offsets[0] != 0) {
+        ctx->err = "offsets[0] must be 0";
+        return BLZ_EINVAL;
+    }
+    for (int64_t i = 0; i < nsubj; i++)
+        if (offsets[i + 1] < offsets[i] || offsets[i + 1] - offsets[i] > (1u << 20)) {
+            ctx->err = "subject offsets must be non-decreasing and subjects at most 1M letters";
+            return BLZ_EINVAL;
+        }
+    const uint64_t nres = offsets[nsubj];
+    for (uint64_t k = 0; k < nres; k++)
+        if (residues[k] >= kAlpha) {
+            ctx->err = "residue outside NCBIstdaa (>= 28)";
+            return BLZ_EINVAL;
+        }
+    (void)hipSetDevice(ctx->device);
+    blz_db *db = new blz_db();
+    db->ctx = ctx;
+    db->nsubj = nsubj;
+    db->nres = nres;
+    // 64 padding bytes so word reads near the end stay inside the allocation
+    if (!check(ctx, hipMalloc((void **)&db->res, nres + 64), "db residues") ||
+        !check(ctx, hipMalloc((void **)&db->offs, sizeof(uint64_t) * (nsubj + 1)), "db offsets")) {
+        blz_db_free(db);
+        return BLZ_ENOMEM;
+    }
+    if (!check(ctx, hipMemcpy(db->res, residues, nres, hipMemcpyHostToDevice), "db copy") ||
+        !check(ctx, hipMemset(db->res + nres, 0, 64), "db pad") ||
+        !check(ctx, hipMemcpy(db->offs, offsets, sizeof(uint64_t) * (nsubj + 1), hipMemcpyHostToDevice),
+               "db copy")) {
+        blz_db_free(db);
+        return BLZ_EDEVICE;
+    }
+    *out = db;
+    return BLZ_OK;
+}
+
+void blz_db_free(blz_db *db) {
+    if (!db) return;
+    if (db->res) (void)hipFree(db->res);
+    if (db->offs) (void)hipFree(db->offs);
+    delete db;
+}
+
+int64_t blz_db_num_subjects(const blz_db *db) { return db ? db->nsubj : 0; }
+uint64_t blz_db_num_residues(const blz_db *db) { return db ? db->nres : 0; }
+
+blz_status blz_query_prepare(blz_ctx *ctx, const uint8_t *query, int32_t qlen, const blz_params *params,
+                             blz_query **out) {
+    if (!ctx || !out || !query || qlen < 1 || qlen > 16000) {
+        if (ctx) ctx->err = "query length must be 1..16000";
+        return BLZ_EINVAL;
+    }
+    *out = nullptr;
+    blz_params p;
+    if (params)
+        p = *params;
+    else
+        blz_default_params(&p);
+    if (p.window < 1 || p.x_ungapped < 1 || p.x_gapped < 0 || p.gap_open < 0 || p.gap_extend < 1) {
+        ctx->err = "invalid parameters";
+        return BLZ_EINVAL;
+    }
+    for (int i = 0; i < qlen; i++)
+        if (query[i] >= kAlpha) {
+            ctx->err = "query residue outside NCBIstdaa (>= 28)";
+            return BLZ_EINVAL;
+        }
+    const Neighbourhoods &nb = neighbourhoods(p.word_threshold);
+    // lookup table: for every word u, the query offsets q (ascending) whose word has u in its
+    // neighbourhood (prelim_stage.cpp:360-386; flattened like :534-571 but with a u32 prefix)
+    std::vector<uint32_t> prefix(kWords + 1, 0);
+    const int nw = qlen >= kWord ? qlen - kWord + 1 : 0;
+    for (int q = 0; q < nw; q++) {
+        const int w = query[q] * 784 + query[q + 1] * 28 + query[q + 2];
+        for (uint32_t k = nb.start[w]; k < nb.start[w + 1]; k++) prefix[nb.words[k] + 1]++;
+    }
+    for (int u = 0; u < kWords; u++) prefix[u + 1] += prefix[u];
+    std::vector<uint16_t> entries(std::max<uint32_t>(prefix[kWords], 1));
+    std::vector<uint32_t> cur(prefix.begin(), prefix.end() - 1);
+    for (int q = 0; q < nw; q++) {
+        const int w = query[q] * 784 + query[q + 1] * 28 + query[q + 2];
+        for (uint32_t k = nb.start[w]; k < nb.start[w + 1]; k++) entries[cur[nb.words[k]]++] = (uint16_t)q;
+    }
+    (void)hipSetDevice(ctx->device);
+    blz_query *qq = new blz_query();
+    qq->ctx = ctx;
+    qq->qlen = qlen;
+    qq->params = p;
+    qq->n_entries = prefix[kWords];
+    if (!check(ctx, hipMalloc((void **)&qq->d_query, qlen + 64), "query") ||
+        !check(ctx, hipMalloc((void **)&qq->d_prefix, sizeof(uint32_t) * (kWords + 1)), "prefix") ||
+        !check(ctx, hipMalloc((void **)&qq->d_entries, sizeof(uint16_t) * entries.size()), "entries")) {
+        blz_query_free(qq);
+        return BLZ_ENOMEM;
+    }
+    if (!check(ctx, hipMemcpy(qq->d_query, query, qlen, hipMemcpyHostToDevice), "query copy") ||
+        !check(ctx, hipMemset(qq->d_query + qlen, 0, 64), "query pad") ||
+        !check(ctx, hipMemcpy(qq->d_prefix, prefix.data(), sizeof(uint32_t) * (kWords + 1), hipMemcpyHostToDevice),
+               "prefix copy") ||
+        !check(ctx, hipMemcpy(qq->d_entries, entries.data(), sizeof(uint16_t) * entries.size(), hipMemcpyHostToDevice),
+               "entries copy")) {
+        blz_query_free(qq);
+        return BLZ_EDEVICE;
+    }
+    *out = qq;
+    return BLZ_OK;
+}
+
+void blz_query_free(blz_query *q) {
+    if (!q) return;
+    if (q->d_query) (void)hipFree(q->d_query);
+    if (q->d_prefix) (void)hipFree(q->d_prefix);
+    if (q->d_entries) (void)hipFree(q->d_entries);
+    delete q;
+}
+
+}  // extern "C"
+
+namespace {
+
+// One query over the database: seed stage, gapped stage, results copied into the sink.
+blz_status run_one(blz_ctx *ctx, const blz_query *q, const blz_db *db, blz_hsp_sink *sink,
+                   blz_run_stats *stats) {
+    const blz_params &p = q->params;
+    const uint32_t ring = ring_words_for(q->qlen, p.window);
+    int waves = kSeedWavesMax;
+    while (waves > 1 && seed_lds_bytes(ring, waves) > ctx->lds_per_block) waves--;
+    if (seed_lds_bytes(ring, waves) > ctx->lds_per_block) {
+        ctx->err = "query too long for the diagonal ring in LDS";
+        return BLZ_EINVAL;
+    }
+    const int seed_blocks = std::max(1, ctx->num_cus * 8 / waves);
+    const size_t nwaves = (size_t)seed_blocks * waves;
+    const size_t nsubj = (size_t)db->nsubj;
+    if (!ensure(ctx, ctx->stage, ctx->stage_records, nwaves * kStageCap, "stage") ||
+        !ensure(ctx, ctx->pool, ctx->pool_cap, std::max<size_t>(1 << 22, 2 * nsubj), "pool") ||
+        !ensure(ctx, ctx->work, ctx->work_cap, std::max<size_t>(nsubj, 1), "work") ||
+        !ensure(ctx, ctx->ovf, ctx->ovf_cap, 1 << 16, "ovf") ||
+        !ensure(ctx, ctx->dp, ctx->dp_entries, (size_t)kGappedBlocks * 64 * kDpRing, "dp") ||
+        !ensure(ctx, ctx->saved, ctx->saved_records, (size_t)kGappedBlocks * 64 * kSavedCap, "saved") ||
+        !ensure(ctx, ctx->hsps, ctx->hsp_cap, std::max<size_t>(1 << 22, nsubj), "hsps") ||
+        !ensure(ctx, ctx->results, ctx->results_cap, std::max<size_t>(nsubj, 1), "results"))
+        return BLZ_ENOMEM;
+
+    hipStream_t st = ctx->stream;
+    if (!check(ctx, hipMemsetAsync(ctx->ctr, 0, sizeof(DevCounters), st), "memset")) return BLZ_EDEVICE;
+
+    SeedArgs sa{};
+    sa.res = db->res;
+    sa.offs = db->offs;
+    sa.s_begin = 0;
+    sa.s_end = db->nsubj;
+    sa.query = q->d_query;
+    sa.qlen = q->qlen;
+    sa.prefix = q->d_prefix;
+    sa.entries = q->d_entries;
+    sa.window = p.window;
+    sa.x_ungapped = p.x_ungapped;
+    sa.ungapped_cutoff = p.ungapped_cutoff;
+    sa.ring_mask = ring - 1;
+    sa.stage = ctx->stage;
+    sa.stage_cap = kStageCap;
+    sa.pool = ctx->pool;
+    sa.pool_cap = ctx->pool_cap;
+    sa.work = ctx->work;
+    sa.ovf = ctx->ovf;
+    sa.ovf_cap = (uint32_t)ctx->ovf_cap;
+    sa.ctr = ctx->ctr;
+
+    GappedArgs ga{};
+    ga.res = db->res;
+    ga.offs = db->offs;
+    ga.query = q->d_query;
+    ga.qlen = q->qlen;
+    ga.x_gapped = p.x_gapped;
+    ga.gap_open = p.gap_open;
+    ga.gap_extend = p.gap_extend;
+    ga.gapped_cutoff = p.gapped_cutoff;
+    ga.pool = ctx->pool;
+    ga.work = ctx->work;
+    ga.dp_scratch = ctx->dp;
+    ga.dp_wmask = kDpRing - 1;
+    ga.hsp_scratch = ctx->saved;
+    ga.saved_cap = kSavedCap;
+    ga.hsps = ctx->hsps;
+    ga.hsp_cap = ctx->hsp_cap;
+    ga.results = ctx->results;
+    ga.ovf = ctx->ovf;
+    ga.ovf_cap = (uint32_t)ctx->ovf_cap;
+    ga.ctr = ctx->ctr;
+
+    if (!check(ctx, hipEventRecord(ctx->ev[0], st), "event") ||
+        (db->nsubj > 0 && !check(ctx, launch_seed(sa, seed_blocks, waves, st), "k_seed launch")) ||
+        !check(ctx, hipEventRecord(ctx->ev[1], st), "event") ||
+        (db->nsubj > 0 && !check(ctx, launch_gapped(ga, kGappedBlocks, st), "k_gapped launch")) ||
+        !check(ctx, hipEventRecord(ctx->ev[2], st), "event"))
+        return BLZ_EDEVICE;
+
+    DevCounters c{};
+    if (!check(ctx, hipMemcpyAsync(&c, ctx->ctr, sizeof(c), hipMemcpyDeviceToHost, st), "counters") ||
+        !check(ctx, hipStreamSynchronize(st), "sync"))
+        return BLZ_EDEVICE;
+
+    std::vector<ResultItem> res(c.result_count);
+    std::vector<Hsp> hs(c.hsp_used);
+    if ((c.result_count &&
+         !check(ctx, hipMemcpyAsync(res.data(), ctx->results, sizeof(ResultItem) * c.result_count,
+                                    hipMemcpyDeviceToHost, st), "results")) ||
+        (c.hsp_used && c.hsp_used <= ctx->hsp_cap &&
+         !check(ctx, hipMemcpyAsync(hs.data(), ctx->hsps, sizeof(Hsp) * c.hsp_used, hipMemcpyDeviceToHost, st),
+                "hsps")) ||
+        !check(ctx, hipStreamSynchronize(st), "sync"))
+        return BLZ_EDEVICE;
+
+    float ms_seed = 0, ms_gapped = 0;
+    (void)hipEventElapsedTime(&ms_seed, ctx->ev[0], ctx->ev[1]);
+    (void)hipEventElapsedTime(&ms_gapped, ctx->ev[1], ctx->ev[2]);
+
+    std::sort(res.begin(), res.end(), [](const ResultItem &a, const ResultItem &b) { return a.sid < b.sid; });
+    int64_t total = 0;
+    for (auto &r : res) total += r.n;
+    blz_status rc = BLZ_OK;
+    if (sink) {
+        sink->n_subjects = (int64_t)res.size();
+        sink->n_hsps = total;
+        if ((int64_t)res.size() > sink->subject_cap || total > sink->hsp_cap) {
+            rc = BLZ_ECAPACITY;
+        } else {
+            int64_t o = 0;
+            for (size_t i = 0; i < res.size(); i++) {
+                sink->oids[i] = res[i].sid;
+                sink->counts[i] = res[i].n;
+                std::memcpy(sink->hsps + o, hs.data() + res[i].off, sizeof(Hsp) * res[i].n);
+                o += res[i].n;
+            }
+        }
+    }
+    if (stats) {
+        stats->subjects += (uint64_t)db->nsubj;
+        stats->residues += db->nres;
+        stats->word_hits += c.word_hits;
+        stats->ungapped_ext += c.ungapped_ext;
+        stats->init_hits += c.init_hits;
+        stats->gapped_subjects += c.work_count;
+        stats->gapped_ext += c.gapped_ext;
+        stats->dp_cells += c.dp_cells;
+        stats->hsps += (uint64_t)total;
+        stats->overflow_subjects += c.overflow_count;
+        stats->ms_seed += ms_seed;
+        stats->ms_gapped += ms_gapped;
+        stats->ms_total += ms_seed + ms_gapped;
+        stats->seed_launches += 1;
+        stats->gapped_launches += 1;
+    }
+    if (c.overflow_count > 0 && rc == BLZ_OK) {
+        ctx->err = "subjects exceeded device caps: " + std::to_string(c.overflow_count);
+        rc = BLZ_EOVERFLOW;
+    }
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+blz_status blz_prelim_run(blz_ctx *ctx, const blz_query *q, const blz_db *db, blz_hsp_sink *sink,
+                          blz_run_stats *stats) {
+    if (!ctx || !q || !db || q->ctx != ctx || db->ctx != ctx) return BLZ_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    return run_one(ctx, q, db, sink, stats);
+}
+
+blz_status blz_prelim_run_batch(blz_ctx *ctx, const blz_query *const *queries, int32_t nq, const blz_db *db,
+                                blz_hsp_sink *sinks, blz_run_stats *stats) {
+    if (!ctx || !queries || nq < 0 || !db) return BLZ_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    blz_status worst = BLZ_OK;
+    for (int32_t i = 0; i < nq; i++) {
+        blz_status rc = run_one(ctx, queries[i], db, sinks ? &sinks[i] : nullptr, stats);
+        if (rc != BLZ_OK && worst == BLZ_OK) worst = rc;
+        if (rc == BLZ_EDEVICE || rc == BLZ_ENOMEM || rc == BLZ_EINVAL) return rc;
+    }
+    return worst;
+}
+
+}  // extern "C"

@@ -1,0 +1,170 @@
+/*
+ * blz_prelim.h -- C ABI of the MI355X BLASTP preliminary-search engine.
+ *
+ * This is the drop-in boundary.  It replaces the hook BLAZE places in NCBI's prelim stage:
+ *   int BLAZE_GPU(BlastSeqSrc*, uint32_t threadIndex, uint32_t totalThreads, uint32_t queryLen,
+ *                 uint32_t* survivingSeqs, uint32_t* num_survivors,
+ *                 int ungapped_cutoff, int gapped_cutoff)
+ *   declared BLAZE_gpu/gpu_cpu_common.h:43, defined BLAZE_gpu/gpuBLAZE.cu:1428, called from
+ *   Blast_RunPreliminarySearchWithInterruptMT at BLAZE_ncbi/blast_engine.c:2983.
+ * BLAZE's hook only filters subjects and lets the CPU re-run the prelim core on survivors
+ * (blast_engine.c:2988-2996).  This ABI returns the per-subject HSP lists themselves, the
+ * lists that core writes to the HSP stream (blast_engine.c:2499), so the caller can hand them
+ * to the unchanged traceback stage.
+ *
+ * Everything is explicit: no side channel through BlastSeqSrc (blast_seqsrc.h:341-451), no
+ * __constant__ per-query state (gpuBLAZE.cu:81-82), so several queries can be in flight.
+ * Errors are returned as status codes; nothing here calls exit().
+ * All pointers are host pointers unless a name says otherwise.  Plain C types only.
+ */
+#ifndef BLZ_PRELIM_H
+#define BLZ_PRELIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    BLZ_OK = 0,
+    BLZ_EINVAL = 1,     /* bad argument */
+    BLZ_ENOMEM = 2,     /* host or device allocation failed */
+    BLZ_EDEVICE = 3,    /* HIP runtime error or no usable gfx950 device */
+    BLZ_ECAPACITY = 4,  /* caller's sink too small: counts are filled, *total says how many */
+    BLZ_EOVERFLOW = 5   /* some subjects exceeded the device caps and were not processed */
+} blz_status;
+
+/* Search parameters.  The reference passes the two cutoffs at run time
+ * (gpu_cpu_common.h:43; computed at blast_engine.c:2971-2979) and hard-codes the rest
+ * (gpuBLAZE.cu:37,59-72; prelim_stage.cpp:144). */
+typedef struct {
+    int32_t word_threshold;  /* neighbourhood threshold T, default 11 */
+    int32_t window;          /* two-hit window, default 40 */
+    int32_t x_ungapped;      /* raw ungapped X-drop, default 16 */
+    int32_t x_gapped;        /* raw prelim gapped X-drop, default 38 */
+    int32_t gap_open;        /* default 11 */
+    int32_t gap_extend;      /* default 1 */
+    int32_t ungapped_cutoff; /* init-hit cutoff (raw), default 41 */
+    int32_t gapped_cutoff;   /* HSP cutoff (raw), default 67 */
+} blz_params;
+
+/* One HSP.  0-based offsets, *_end one past the last aligned residue; q_gs/s_gs is the
+ * gapped start the score-only DP was anchored at (NCBI BlastHSP query/subject
+ * offset, end, gapped_start; raw score). */
+typedef struct {
+    int32_t score;
+    int32_t q_off, q_end;
+    int32_t s_off, s_end;
+    int32_t q_gs, s_gs;
+} blz_hsp;
+
+/* Per-run counters and device timings (HIP events on the run's stream). */
+typedef struct {
+    uint64_t subjects;         /* subjects scanned */
+    uint64_t residues;         /* subject residues scanned */
+    uint64_t word_hits;        /* lookup hits */
+    uint64_t ungapped_ext;     /* two-hit ungapped extensions */
+    uint64_t init_hits;        /* ungapped HSPs at or above the ungapped cutoff */
+    uint64_t gapped_subjects;  /* subjects that reached the gapped stage */
+    uint64_t gapped_ext;       /* semi-gapped extensions (left+right counted once) */
+    uint64_t dp_cells;         /* DP cells evaluated inside the X-drop bands */
+    uint64_t hsps;             /* HSPs returned */
+    uint64_t overflow_subjects;/* subjects past the device caps (retried on the wide path) */
+    double ms_seed;            /* seed/two-hit/ungapped kernel time */
+    double ms_gapped;          /* gapped-stage kernel time */
+    double ms_total;           /* device time from first launch to results on device */
+    int32_t seed_launches;     /* kernel launches of the seed stage */
+    int32_t gapped_launches;
+} blz_run_stats;
+
+/* Result sink (caller-owned).  Subjects with no HSP are omitted, as NCBI skips the stream
+ * write for an empty list (blast_engine.c:812-815, "hspcnt == 0 -> continue").
+ * On return: n_subjects entries of oids/counts (ascending oid) and their HSPs subject-major in
+ * score order.  If a capacity is short, BLZ_ECAPACITY is returned and the totals say how
+ * much room is needed. */
+typedef struct {
+    int64_t *oids;          /* [subject_cap] */
+    int32_t *counts;        /* [subject_cap] */
+    int64_t subject_cap;
+    blz_hsp *hsps;          /* [hsp_cap] */
+    int64_t hsp_cap;
+    int64_t n_subjects;     /* out */
+    int64_t n_hsps;         /* out */
+} blz_hsp_sink;
+
+typedef struct blz_ctx blz_ctx;     /* device, stream, reusable device work buffers */
+typedef struct blz_db blz_db;       /* resident database (residues + offsets in HBM) */
+typedef struct blz_query blz_query; /* prepared query: lookup tables in HBM */
+
+/* Context: binds one HIP device (gfx950) and owns a stream plus work buffers that are grown
+ * on demand and reused.  Replaces the per-call pinned/device rings BLAZE_GPU allocates and
+ * never frees (gpuBLAZE.cu:1528-1555). */
+blz_status blz_ctx_create(int32_t device, blz_ctx **out);
+void blz_ctx_destroy(blz_ctx *ctx);
+const char *blz_last_error(const blz_ctx *ctx);
+/* The hipStream_t all work of this context runs on (for callers that time or order work). */
+void *blz_ctx_stream(blz_ctx *ctx);
+
+void blz_default_params(blz_params *p);
+
+/* Upload a database once; it stays resident.  residues: concatenated NCBIstdaa letters;
+ * offsets[nsubj+1]: subject i is residues[offsets[i] .. offsets[i+1]).  Replaces the host
+ * gather and per-chunk H2D copies of BLAZE_GPU (gpuBLAZE.cu:1587-1618) and the DB arrays
+ * BLAZE allocates in prelim_stage.cpp:1204-1215. */
+blz_status blz_db_upload(blz_ctx *ctx, const uint8_t *residues, const uint64_t *offsets,
+                         int64_t nsubj, blz_db **out);
+void blz_db_free(blz_db *db);
+int64_t blz_db_num_subjects(const blz_db *db);
+uint64_t blz_db_num_residues(const blz_db *db);
+
+/* Build a query's neighbourhood lookup table on the host and upload it.  Replaces the query
+ * preprocessing of prelim_stage.cpp:340-571 and copyConstantMemory (gpuBLAZE.cu:1823-1842). */
+blz_status blz_query_prepare(blz_ctx *ctx, const uint8_t *query, int32_t qlen,
+                             const blz_params *params, blz_query **out);
+void blz_query_free(blz_query *q);
+
+/* Prelim stage of one query over the whole database, one launch per stage.
+ * Replaces BLAZE_GPU + BLAST_PreliminarySearchEngineGPU (blast_engine.c:2983-2996). */
+blz_status blz_prelim_run(blz_ctx *ctx, const blz_query *q, const blz_db *db,
+                          blz_hsp_sink *sink, blz_run_stats *stats);
+
+/* Prelim stage of nq queries over the database in one persistent work-stealing launch per
+ * stage.  sinks[i] receives query i's lists; stats (optional) sums over the batch. */
+blz_status blz_prelim_run_batch(blz_ctx *ctx, const blz_query *const *queries, int32_t nq,
+                                const blz_db *db, blz_hsp_sink *sinks, blz_run_stats *stats);
+
+/* ---- synthetic data (NCBIstdaa, Robinson-Robinson background) ----------------------- */
+
+typedef struct {
+    uint64_t seed;
+    int64_t n_subjects;
+    double len_median;      /* log-normal median */
+    double len_sigma;       /* log-normal sigma */
+    int32_t len_min, len_max;
+    double homolog_frac;    /* fraction of subjects planted as homologs */
+    double ident_min, ident_max; /* identity range of planted homologs */
+    double indel_rate;      /* per-residue indel start rate */
+    double indel_mean;      /* geometric indel length mean */
+    double repeat_frac;     /* fraction of residues in low-complexity repeat blocks */
+    int32_t repeat_period_min, repeat_period_max;
+    int32_t sort_by_length; /* 1: subjects sorted by length (ascending) */
+} blz_synth_spec;
+
+void blz_synth_default(blz_synth_spec *spec);
+/* Random query of length qlen drawn from the background. */
+void blz_synth_query(uint64_t seed, int32_t qlen, uint8_t *out);
+/* Sizes of the database a spec describes (deterministic). */
+blz_status blz_synth_db_size(const blz_synth_spec *spec, int64_t *n_subjects,
+                             uint64_t *n_residues);
+/* Fill a database; homologs are planted from the given queries (round robin).
+ * homolog_of[i] (optional) receives the query index subject i derives from, or -1. */
+blz_status blz_synth_db_fill(const blz_synth_spec *spec, const uint8_t *const *queries,
+                             const int32_t *qlens, int32_t nq, uint8_t *residues,
+                             uint64_t *offsets, int32_t *homolog_of);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,123 @@
+"""GPU vs oracle: per-subject HSP lists must be identical (bit-exact: same subjects, same HSPs,
+same order, identical offsets, gapped starts and raw scores)."""
+import os
+
+import numpy as np
+import pytest
+
+import blz
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+NTHREADS = min(16, os.cpu_count() or 1)
+
+
+@pytest.fixture(scope="module")
+def engine():
+    with blz.Engine(0) as eng:
+        yield eng
+
+
+def compare(eng, query, res, offs, p_blz=None, p_ora=None, allow_overflow=False):
+    eng.upload_db(res, offs)
+    qh = eng.prepare(query, p_blz)
+    try:
+        got, stats = eng.run(qh, allow_overflow=allow_overflow)
+    finally:
+        eng.free_query(qh)
+    counts, hsps, ost = oracle.prelim_db(query, res, offs, p_ora, nthreads=NTHREADS)
+    want = oracle.as_lists(counts, hsps)
+    have = got.as_dict()
+    missing = sorted(set(want) - set(have))
+    extra = sorted(set(have) - set(want))
+    assert not missing and not extra, f"subjects differ: missing {missing[:10]} extra {extra[:10]}"
+    bad = [oid for oid in want if not np.array_equal(want[oid], have[oid])]
+    assert not bad, f"{len(bad)} subjects differ, first {bad[0]}: want {want[bad[0]]} got {have[bad[0]]}"
+    # the device counters agree with the oracle's on the work both did
+    assert stats["word_hits"] == ost["word_hits"]
+    assert stats["ungapped_ext"] == ost["ungapped_ext"]
+    assert stats["init_hits"] == ost["init_hits"]
+    assert stats["gapped_ext"] == ost["gapped_ext"]
+    assert stats["dp_cells"] == ost["dp_cells"]
+    return stats, len(want)
+
+
+def make(seed, nq_len, n, **kw):
+    q = blz.synth_query(seed, nq_len)
+    spec = blz.synth_spec(seed=seed, n_subjects=n, **kw)
+    res, offs, hom = blz.synth_db(spec, [q])
+    return q, res, offs, hom
+
+
+def test_config0_cut(engine):
+    """Config 0 shape (300 aa query, log-normal median 300, 2% homologs at 50%), 3000 subjects."""
+    q, res, offs, hom = make(0, 300, 3000, homolog_frac=0.02, ident_min=0.5, ident_max=0.5)
+    stats, nsub = compare(engine, q, res, offs)
+    assert nsub >= 0.9 * (hom >= 0).sum()
+    assert stats["overflow_subjects"] == 0
+
+
+@pytest.mark.parametrize("qlen", [3, 17, 64, 500, 1000, 2000])
+def test_query_lengths(engine, qlen):
+    q, res, offs, _ = make(7 + qlen, qlen, 800, homolog_frac=0.05, ident_min=0.35, ident_max=0.7)
+    compare(engine, q, res, offs)
+
+
+def test_edge_subjects(engine):
+    """Subjects of length 1, 2, 3, the query itself, a reversed query and empty-range neighbours."""
+    q = blz.synth_query(3, 200)
+    subs = [q[:1], q[:2], q[:3], q.copy(), q[::-1].copy(), q[50:120].copy(), np.full(40, 1, np.uint8),
+            q[:0], np.concatenate([q, q]).copy()]
+    offs = np.zeros(len(subs) + 1, dtype=np.uint64)
+    offs[1:] = np.cumsum([len(s) for s in subs])
+    res = np.concatenate(subs).astype(np.uint8)
+    compare(engine, q, res, offs)
+
+
+def test_low_complexity(engine):
+    """Poly-residue and short-period repeats: many hits per word and per diagonal."""
+    q = np.concatenate([np.full(120, 1, np.uint8), np.tile(np.array([11, 4, 9], np.uint8), 60),
+                        blz.synth_query(5, 100)])
+    subs = [np.full(300, 1, np.uint8), np.tile(np.array([11, 4, 9], np.uint8), 100),
+            np.tile(np.array([11, 4, 9, 9], np.uint8), 70), blz.synth_query(6, 400)]
+    offs = np.zeros(len(subs) + 1, dtype=np.uint64)
+    offs[1:] = np.cumsum([len(s) for s in subs])
+    compare(engine, q, np.concatenate(subs), offs)
+
+
+def test_repeats_long_subjects(engine):
+    """Config 3 shape cut down: long subjects (median 800, up to 35,000), 20% homologs at 30-45%
+    with low-complexity repeat blocks."""
+    q, res, offs, _ = make(11, 1500, 300, len_median=800, len_max=35000, homolog_frac=0.2,
+                           ident_min=0.30, ident_max=0.45, repeat_frac=0.05)
+    compare(engine, q, res, offs)
+
+
+def test_empty_db(engine):
+    q = blz.synth_query(1, 100)
+    res = np.zeros(0, np.uint8)
+    offs = np.zeros(1, np.uint64)
+    eng = engine
+    eng.upload_db(res, offs)
+    qh = eng.prepare(q)
+    got, stats = eng.run(qh)
+    eng.free_query(qh)
+    assert len(got.oids) == 0 and stats["word_hits"] == 0
+
+
+def test_params_variants(engine):
+    """Non-default cutoffs, X-drops and gap costs are honoured identically."""
+    q, res, offs, _ = make(21, 400, 600, homolog_frac=0.1, ident_min=0.4, ident_max=0.6)
+    kw = dict(window=30, x_ungapped=20, x_gapped=25, gap_open=9, gap_extend=2, ungapped_cutoff=35,
+              gapped_cutoff=55)
+    compare(engine, q, res, offs, blz.default_params(**kw), oracle.params(**kw))
+
+
+def test_config1_full_size(engine):
+    """Config 1 at full size: 500 aa query vs 2,000,000 subjects, 1% homologs at 35-70%.
+    The oracle runs multi-threaded on the host."""
+    q, res, offs, hom = make(0, 500, 2_000_000, homolog_frac=0.01, ident_min=0.35, ident_max=0.70)
+    stats, nsub = compare(engine, q, res, offs)
+    assert stats["overflow_subjects"] == 0
+    assert nsub >= 0.9 * (hom >= 0).sum()
