@@ -52,6 +52,11 @@ class RunStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Tuning(C.Structure):
+    _fields_ = [("seed_variant", C.c_int32), ("seed_waves", C.c_int32), ("seed_blocks_per_cu", C.c_int32),
+                ("gapped_blocks", C.c_int32), ("gapped_threads", C.c_int32), ("reserved", C.c_int32 * 3)]
+
+
 class Sink(C.Structure):
     _fields_ = [("oids", C.POINTER(C.c_int64)), ("counts", C.POINTER(C.c_int32)),
                 ("subject_cap", C.c_int64), ("hsps", C.c_void_p), ("hsp_cap", C.c_int64),
@@ -73,6 +78,7 @@ EXPORTS = [
     "blz_db_upload", "blz_db_free", "blz_db_num_subjects", "blz_db_num_residues",
     "blz_query_prepare", "blz_query_free", "blz_prelim_run", "blz_prelim_run_batch",
     "blz_synth_default", "blz_synth_query", "blz_synth_db_size", "blz_synth_db_fill",
+    "blz_ctx_set_tuning", "blz_ctx_get_tuning",
 ]
 
 _lib = None
@@ -110,6 +116,10 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.blz_prelim_run.restype = C.c_int
     lib.blz_prelim_run_batch.argtypes = [vp, C.POINTER(vp), i32, vp, C.POINTER(Sink), C.POINTER(RunStats)]
     lib.blz_prelim_run_batch.restype = C.c_int
+    lib.blz_ctx_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    lib.blz_ctx_set_tuning.restype = C.c_int
+    lib.blz_ctx_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    lib.blz_ctx_get_tuning.restype = C.c_int
     lib.blz_synth_default.argtypes = [C.POINTER(SynthSpec)]
     lib.blz_synth_query.argtypes = [u64, i32, vp]
     lib.blz_synth_db_size.argtypes = [C.POINTER(SynthSpec), C.POINTER(i64), C.POINTER(u64)]
@@ -216,6 +226,20 @@ class Engine:
 
     def __exit__(self, *a):
         self.close()
+
+    def set_tuning(self, **kw):
+        t = Tuning()
+        self.lib.blz_ctx_get_tuning(self.ctx, C.byref(t))
+        for k, v in kw.items():
+            setattr(t, k, v)
+        st = self.lib.blz_ctx_set_tuning(self.ctx, C.byref(t))
+        if st:
+            self._err(st, "blz_ctx_set_tuning")
+
+    def get_tuning(self) -> dict:
+        t = Tuning()
+        self.lib.blz_ctx_get_tuning(self.ctx, C.byref(t))
+        return {k: getattr(t, k) for k, _ in Tuning._fields_ if k != "reserved"}
 
     @property
     def stream(self) -> int:

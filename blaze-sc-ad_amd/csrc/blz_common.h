@@ -7,6 +7,7 @@ namespace blz {
 constexpr int kAlpha = 28;           // NCBIstdaa letters
 constexpr int kWords = 28 * 28 * 28; // word index a*784 + b*28 + c (gpuBLAZE.cu:991)
 constexpr int kWord = 3;             // word size
+constexpr int kRelBlocks = kWords / 64 + 1;  // 64-word blocks of the compact lookup prefix
 constexpr int kHspMaxWindow = 11;    // gapped-start window (gpuBLAZE.cu:64)
 constexpr int kMinIntDp = -1073741824; // INT_MIN/2, the DP's "minus infinity"
 
@@ -24,7 +25,7 @@ struct WorkItem {
     int64_t sid;
     uint64_t off;
     int32_t n;
-    int32_t pad;
+    int32_t qi;  // query of the batch
 };
 
 // HSP record, same layout as blz_hsp in blz_prelim.h.
@@ -37,14 +38,17 @@ struct ResultItem {
     int64_t sid;
     uint64_t off;
     int32_t n;
-    int32_t pad;
+    int32_t qi;
 };
+
+// Overflow record: query index in the high bits, subject id in the low 40 bits.
+constexpr int kOvfShift = 40;
 
 // Device counters of one run (atomically accumulated by the kernels).
 struct DevCounters {
     unsigned long long word_hits, ungapped_ext, init_hits, gapped_ext, dp_cells;
     unsigned long long pool_used, hsp_used;
-    unsigned int work_count, result_count, overflow_count, next_subject, next_work, pad;
+    unsigned int work_count, result_count, overflow_count, next_subject, next_work, next_item;
 };
 
 }  // namespace blz

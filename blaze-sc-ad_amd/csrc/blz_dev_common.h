@@ -1,0 +1,107 @@
+// blz_dev_common.h -- device helpers shared by the kernels: wave utilities and the ungapped
+// X-drop walks.  Included by each .hip translation unit.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "blz_common.h"
+
+namespace blz {
+
+static __constant__ int16_t kBlosum62Dev[28][28] = {
+#include "blz_blosum62.inc"
+};
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------------------
+// Ungapped X-drop walks (NCBI s_BlastAaExtendLeft/Right; the reference restates the stop
+// rules at gpuBLAZE.cu:680-883).  Lane-serial; the matrix row comes from LDS.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ int extend_left(const int16_t (*mat)[28], const uint8_t *q,
+                                           const uint8_t *s, int q_off, int s_off, int dropoff,
+                                           int maxscore, int *length) {
+    int n = s_off < q_off ? s_off : q_off;
+    int best_i = n + 1;
+    int score = maxscore;
+    for (int i = n; i >= 0; i--) {
+        score += mat[q[q_off - n + i]][s[s_off - n + i]];
+        if (score > maxscore) {
+            maxscore = score;
+            best_i = i;
+        }
+        if (maxscore - score >= dropoff) break;
+    }
+    *length = n - best_i + 1;
+    return maxscore;
+}
+
+__device__ __forceinline__ int extend_right(const int16_t (*mat)[28], const uint8_t *q, int qlen,
+                                            const uint8_t *s, int slen, int q_off, int s_off,
+                                            int dropoff, int maxscore, int *length,
+                                            int *s_last_off) {
+    int n = slen - s_off < qlen - q_off ? slen - s_off : qlen - q_off;
+    int best_i = -1;
+    int score = maxscore;
+    int i;
+    for (i = 0; i < n; i++) {
+        score += mat[q[q_off + i]][s[s_off + i]];
+        if (score > maxscore) {
+            maxscore = score;
+            best_i = i;
+        }
+        if (score <= 0 || maxscore - score >= dropoff) break;
+    }
+    *length = best_i + 1;
+    *s_last_off = s_off + i;
+    return maxscore;
+}
+
+// Two-hit extension from the second hit (q, s) back towards the first hit, whose word ends
+// at s_left_off (NCBI s_BlastAaExtendTwoHit; reference gpuBLAZE.cu:1116-1154).
+__device__ __forceinline__ int two_hit(const int16_t (*mat)[28], const uint8_t *q, int qlen,
+                                       const uint8_t *s, int slen, int s_left_off, int s_right_off,
+                                       int q_right_off, int dropoff, int *hsp_q, int *hsp_s,
+                                       int *hsp_len, int *right_extend, int *s_last_off) {
+    int left_d = 0, right_d = 0, left_score = 0, right_score = 0, score = 0;
+#pragma unroll
+    for (int i = 0; i < kWord; i++) {
+        score += mat[q[q_right_off + i]][s[s_right_off + i]];
+        if (score > left_score) {
+            left_score = score;
+            right_d = i + 1;
+        }
+    }
+    q_right_off += right_d;
+    s_right_off += right_d;
+    right_d = 0;
+    *right_extend = 0;
+    *s_last_off = s_right_off;
+    left_score = extend_left(mat, q, s, q_right_off - 1, s_right_off - 1, dropoff, 0, &left_d);
+    if (left_d >= s_right_off - s_left_off) {
+        *right_extend = 1;
+        right_score = extend_right(mat, q, qlen, s, slen, q_right_off, s_right_off, dropoff,
+                                   left_score, &right_d, s_last_off);
+    }
+    *hsp_q = q_right_off - left_d;
+    *hsp_s = s_right_off - left_d;
+    *hsp_len = left_d + right_d;
+    return left_score > right_score ? left_score : right_score;
+}
+
+}  // namespace blz

@@ -6,6 +6,20 @@
 
 namespace blz {
 
+// One query of a batch as the kernels see it.
+struct QueryDev {
+    const uint8_t *query;
+    const uint32_t *prefix;    // lookup prefix [kWords+1]
+    const uint16_t *entries;   // query offsets per word, ascending
+    const uint16_t *rel16;     // compact prefix: offset of word w inside its 64-word block
+    const uint32_t *blockbase; // compact prefix: first entry of each 64-word block
+    uint32_t n_entries;
+    int32_t qlen;
+    uint32_t ring_mask, claim_mask;
+    int32_t window, x_ungapped, ungapped_cutoff;
+    int32_t x_gapped, gap_open, gap_extend, gapped_cutoff;
+};
+
 struct SeedArgs {
     const uint8_t *res;        // concatenated subject residues (device)
     const uint64_t *offs;      // subject offsets [nsubj+1] (device)
@@ -16,6 +30,10 @@ struct SeedArgs {
     const uint16_t *entries;   // query offsets per word, ascending
     int32_t window, x_ungapped, ungapped_cutoff;
     uint32_t ring_mask;        // diagonal ring size - 1 (power of two >= qlen + 64 + window)
+    uint32_t claim_mask;       // same-diagonal arbitration table size - 1 (power of two)
+    const uint16_t *rel16;     // compact prefix: offset of word w inside its 64-word block
+    const uint32_t *blockbase; // compact prefix: first entry of each 64-word block
+    uint32_t n_entries;        // lookup entries
     InitHit *stage;            // per-wave staging, stage_cap records per wave
     int32_t stage_cap;
     InitHit *pool;             // init hits of all subjects
@@ -24,14 +42,19 @@ struct SeedArgs {
     int64_t *ovf;              // subjects past a cap
     uint32_t ovf_cap;
     DevCounters *ctr;
+    // batched form (seed kernel 2): items are (query, block of subjects), query-major
+    const QueryDev *qd;
+    int32_t nq;
+    int32_t qi_base;           // index of qd[0] in the run's query table
+    uint32_t *item_ctr;        // this launch's work-item counter
+    uint32_t items_per_query;
+    int64_t subj_per_item;
 };
 
 struct GappedArgs {
     const uint8_t *res;
     const uint64_t *offs;
-    const uint8_t *query;
-    int32_t qlen;
-    int32_t x_gapped, gap_open, gap_extend, gapped_cutoff;
+    const QueryDev *qd;        // queries of the batch, indexed by WorkItem::qi
     InitHit *pool;
     const WorkItem *work;
     int2 *dp_scratch;          // per-lane DP ring, dp_wmask+1 entries per lane
@@ -47,7 +70,9 @@ struct GappedArgs {
 };
 
 size_t seed_lds_bytes(uint32_t ring_words, int waves_per_block);
+size_t seed2_lds_bytes(const SeedArgs &a, bool lds_table, int waves);
+hipError_t launch_seed2(const SeedArgs &a, bool lds_table, int blocks, int waves, hipStream_t st);
 hipError_t launch_seed(const SeedArgs &a, int blocks, int waves_per_block, hipStream_t st);
-hipError_t launch_gapped(const GappedArgs &a, int blocks, hipStream_t st);
+hipError_t launch_gapped(const GappedArgs &a, int blocks, int threads, hipStream_t st);
 
 }  // namespace blz

@@ -18,105 +18,9 @@
 
 #include "blz_common.h"
 #include "blz_device.h"
+#include "blz_dev_common.h"
 
 namespace blz {
-
-__constant__ int16_t kBlosum62Dev[28][28] = {
-#include "blz_blosum62.inc"
-};
-
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
-
-__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
-    const int lane = lane_id();
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t t = __shfl_up(v, o, 64);
-        if (lane >= o) v += t;
-    }
-    return v;
-}
-
-// ---------------------------------------------------------------------------------------
-// Ungapped X-drop walks (NCBI s_BlastAaExtendLeft/Right; the reference restates the stop
-// rules at gpuBLAZE.cu:680-883).  Lane-serial; the matrix row comes from LDS.
-// ---------------------------------------------------------------------------------------
-__device__ __forceinline__ int extend_left(const int16_t (*mat)[28], const uint8_t *q,
-                                           const uint8_t *s, int q_off, int s_off, int dropoff,
-                                           int maxscore, int *length) {
-    int n = s_off < q_off ? s_off : q_off;
-    int best_i = n + 1;
-    int score = maxscore;
-    for (int i = n; i >= 0; i--) {
-        score += mat[q[q_off - n + i]][s[s_off - n + i]];
-        if (score > maxscore) {
-            maxscore = score;
-            best_i = i;
-        }
-        if (maxscore - score >= dropoff) break;
-    }
-    *length = n - best_i + 1;
-    return maxscore;
-}
-
-__device__ __forceinline__ int extend_right(const int16_t (*mat)[28], const uint8_t *q, int qlen,
-                                            const uint8_t *s, int slen, int q_off, int s_off,
-                                            int dropoff, int maxscore, int *length,
-                                            int *s_last_off) {
-    int n = slen - s_off < qlen - q_off ? slen - s_off : qlen - q_off;
-    int best_i = -1;
-    int score = maxscore;
-    int i;
-    for (i = 0; i < n; i++) {
-        score += mat[q[q_off + i]][s[s_off + i]];
-        if (score > maxscore) {
-            maxscore = score;
-            best_i = i;
-        }
-        if (score <= 0 || maxscore - score >= dropoff) break;
-    }
-    *length = best_i + 1;
-    *s_last_off = s_off + i;
-    return maxscore;
-}
-
-// Two-hit extension from the second hit (q, s) back towards the first hit, whose word ends
-// at s_left_off (NCBI s_BlastAaExtendTwoHit; reference gpuBLAZE.cu:1116-1154).
-__device__ __forceinline__ int two_hit(const int16_t (*mat)[28], const uint8_t *q, int qlen,
-                                       const uint8_t *s, int slen, int s_left_off, int s_right_off,
-                                       int q_right_off, int dropoff, int *hsp_q, int *hsp_s,
-                                       int *hsp_len, int *right_extend, int *s_last_off) {
-    int left_d = 0, right_d = 0, left_score = 0, right_score = 0, score = 0;
-#pragma unroll
-    for (int i = 0; i < kWord; i++) {
-        score += mat[q[q_right_off + i]][s[s_right_off + i]];
-        if (score > left_score) {
-            left_score = score;
-            right_d = i + 1;
-        }
-    }
-    q_right_off += right_d;
-    s_right_off += right_d;
-    right_d = 0;
-    *right_extend = 0;
-    *s_last_off = s_right_off;
-    left_score = extend_left(mat, q, s, q_right_off - 1, s_right_off - 1, dropoff, 0, &left_d);
-    if (left_d >= s_right_off - s_left_off) {
-        *right_extend = 1;
-        right_score = extend_right(mat, q, qlen, s, slen, q_right_off, s_right_off, dropoff,
-                                   left_score, &right_d, s_last_off);
-    }
-    *hsp_q = q_right_off - left_d;
-    *hsp_s = s_right_off - left_d;
-    *hsp_len = left_d + right_d;
-    return left_score > right_score ? left_score : right_score;
-}
 
 // Diagonal ring entry: 0 = no hit yet; otherwise bit 31 = "extended" flag and the low bits
 // hold (wave-relative position + 1) of the last hit, or of the end barrier when flagged.
@@ -446,7 +350,7 @@ __device__ void insertion_sort(Hsp *h, int n) {
     }
 }
 
-__global__ void __launch_bounds__(64) k_gapped(GappedArgs a) {
+__global__ void __launch_bounds__(1024) k_gapped(GappedArgs a) {
     __shared__ int16_t mat[28][28];
     for (int i = threadIdx.x; i < 28 * 28; i += blockDim.x) mat[i / 28][i % 28] = kBlosum62Dev[i / 28][i % 28];
     __syncthreads();
@@ -472,8 +376,9 @@ __global__ void __launch_bounds__(64) k_gapped(GappedArgs a) {
         const uint64_t o0 = a.offs[wi.sid];
         const int slen = (int)(a.offs[wi.sid + 1] - o0);
         const uint8_t *S = a.res + o0;
-        const uint8_t *Q = a.query;
-        const int qlen = a.qlen;
+        const QueryDev &qd = a.qd[wi.qi];
+        const uint8_t *Q = qd.query;
+        const int qlen = qd.qlen;
         int ns = 0;
         bool ovf = false;
         for (int k = 0; k < n && !ovf; k++) {
@@ -491,14 +396,14 @@ __global__ void __launch_bounds__(64) k_gapped(GappedArgs a) {
             const int q_gs = gapped_start(mat, Q, S, h.q_start, h.s_start, h.len);
             const int s_gs = h.s_start + (q_gs - h.q_start);
             int la, lb, ra, rb;
-            const int left = semi_gapped(mat, Q, S, q_gs + 1, s_gs + 1, a.x_gapped, a.gap_open,
-                                         a.gap_extend, true, &la, &lb, ring, a.dp_wmask, &ovf, &cells);
+            const int left = semi_gapped(mat, Q, S, q_gs + 1, s_gs + 1, qd.x_gapped, qd.gap_open,
+                                         qd.gap_extend, true, &la, &lb, ring, a.dp_wmask, &ovf, &cells);
             const int right = semi_gapped(mat, Q + q_gs, S + s_gs, qlen - q_gs - 1, slen - s_gs - 1,
-                                          a.x_gapped, a.gap_open, a.gap_extend, false, &ra, &rb,
+                                          qd.x_gapped, qd.gap_open, qd.gap_extend, false, &ra, &rb,
                                           ring, a.dp_wmask, &ovf, &cells);
             gext++;
             const int score = left + right;
-            if (!ovf && score >= a.gapped_cutoff) {
+            if (!ovf && score >= qd.gapped_cutoff) {
                 if (ns >= a.saved_cap) {
                     ovf = true;
                     break;
@@ -508,7 +413,7 @@ __global__ void __launch_bounds__(64) k_gapped(GappedArgs a) {
         }
         if (ovf) {
             uint32_t k = atomicAdd(&a.ctr->overflow_count, 1u);
-            if (k < a.ovf_cap) a.ovf[k] = wi.sid;
+            if (k < a.ovf_cap) a.ovf[k] = ((int64_t)wi.qi << kOvfShift) | wi.sid;
             continue;
         }
         if (ns == 0) continue;
@@ -531,12 +436,12 @@ __global__ void __launch_bounds__(64) k_gapped(GappedArgs a) {
         const unsigned long long base = atomicAdd(&a.ctr->hsp_used, (unsigned long long)ns);
         if (base + ns > a.hsp_cap) {
             uint32_t k = atomicAdd(&a.ctr->overflow_count, 1u);
-            if (k < a.ovf_cap) a.ovf[k] = wi.sid;
+            if (k < a.ovf_cap) a.ovf[k] = ((int64_t)wi.qi << kOvfShift) | wi.sid;
             continue;
         }
         for (int i = 0; i < ns; i++) a.hsps[base + i] = saved[i];
         const uint32_t ri = atomicAdd(&a.ctr->result_count, 1u);
-        a.results[ri] = ResultItem{wi.sid, (uint64_t)base, ns, 0};
+        a.results[ri] = ResultItem{wi.sid, (uint64_t)base, ns, wi.qi};
     }
     atomicAdd(&a.ctr->dp_cells, cells);
     atomicAdd(&a.ctr->gapped_ext, gext);
@@ -556,8 +461,8 @@ hipError_t launch_seed(const SeedArgs &a, int blocks, int waves_per_block, hipSt
     return hipGetLastError();
 }
 
-hipError_t launch_gapped(const GappedArgs &a, int blocks, hipStream_t st) {
-    hipLaunchKernelGGL(k_gapped, dim3(blocks), dim3(64), 0, st, a);
+hipError_t launch_gapped(const GappedArgs &a, int blocks, int threads, hipStream_t st) {
+    hipLaunchKernelGGL(k_gapped, dim3(blocks), dim3(threads), 0, st, a);
     return hipGetLastError();
 }
 

@@ -32,6 +32,7 @@ struct blz_ctx {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int num_cus = 0;
     size_t lds_per_block = 0;
+    blz_tuning tune{};
     std::string err;
     // reusable device buffers
     DevCounters *ctr = nullptr;
@@ -51,6 +52,11 @@ struct blz_ctx {
     size_t hsp_cap = 0;
     ResultItem *results = nullptr;
     size_t results_cap = 0;
+    QueryDev *qdev = nullptr;      // query table of the current run
+    size_t qdev_cap = 0;
+    uint32_t *item_ctr = nullptr;  // work-item counter per seed launch
+    size_t item_ctr_cap = 0;
+    std::vector<int> batch_order;  // table index -> caller's query index
 };
 
 struct blz_db {
@@ -69,6 +75,9 @@ struct blz_query {
     uint32_t *d_prefix = nullptr;
     uint16_t *d_entries = nullptr;
     uint64_t n_entries = 0;
+    uint16_t *d_rel16 = nullptr;     // compact prefix (valid when compact_ok)
+    uint32_t *d_blockbase = nullptr;
+    bool compact_ok = false;
 };
 
 namespace {
@@ -197,13 +206,30 @@ void blz_ctx_destroy(blz_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    void *bufs[] = {ctx->ctr, ctx->stage, ctx->pool, ctx->work, ctx->ovf, ctx->dp, ctx->saved, ctx->hsps, ctx->results};
+    void *bufs[] = {ctx->ctr,   ctx->stage,   ctx->pool,    ctx->work, ctx->ovf,     ctx->dp,
+                    ctx->saved, ctx->hsps,    ctx->results, ctx->qdev, ctx->item_ctr};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+
+blz_status blz_ctx_set_tuning(blz_ctx *ctx, const blz_tuning *t) {
+    if (!ctx || !t) return BLZ_EINVAL;
+    if (t->seed_variant < 0 || t->seed_variant > 2 || t->seed_waves < 0 || t->seed_waves > 16 ||
+        t->seed_blocks_per_cu < 0 || t->gapped_blocks < 0 || t->gapped_threads < 0 || t->gapped_threads > 1024 ||
+        (t->gapped_threads % 64) != 0)
+        return BLZ_EINVAL;
+    ctx->tune = *t;
+    return BLZ_OK;
+}
+
+blz_status blz_ctx_get_tuning(const blz_ctx *ctx, blz_tuning *t) {
+    if (!ctx || !t) return BLZ_EINVAL;
+    *t = ctx->tune;
+    return BLZ_OK;
 }
 
 const char *blz_last_error(const blz_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
@@ -292,11 +318,22 @@ blz_status blz_query_prepare(blz_ctx *ctx, const uint8_t *query, int32_t qlen, c
         for (uint32_t k = nb.start[w]; k < nb.start[w + 1]; k++) prefix[nb.words[k] + 1]++;
     }
     for (int u = 0; u < kWords; u++) prefix[u + 1] += prefix[u];
-    std::vector<uint16_t> entries(std::max<uint32_t>(prefix[kWords], 1));
+    // two spare entries: the seed kernel stages the table into LDS in 32-bit words
+    std::vector<uint16_t> entries(prefix[kWords] + 2, 0);
     std::vector<uint32_t> cur(prefix.begin(), prefix.end() - 1);
     for (int q = 0; q < nw; q++) {
         const int w = query[q] * 784 + query[q + 1] * 28 + query[q + 2];
         for (uint32_t k = nb.start[w]; k < nb.start[w + 1]; k++) entries[cur[nb.words[k]]++] = (uint16_t)q;
+    }
+    // compact prefix for LDS: a u32 base per 64-word block and a u16 offset per word
+    std::vector<uint32_t> blockbase(kRelBlocks);
+    std::vector<uint16_t> rel16(kWords + 1);
+    bool compact_ok = true;
+    for (int b = 0; b < kRelBlocks; b++) blockbase[b] = prefix[std::min(b * 64, kWords)];
+    for (int u = 0; u <= kWords; u++) {
+        const uint32_t r = prefix[u] - blockbase[u >> 6];
+        if (r > 0xFFFFu) compact_ok = false;
+        rel16[u] = (uint16_t)r;
     }
     (void)hipSetDevice(ctx->device);
     blz_query *qq = new blz_query();
@@ -304,9 +341,12 @@ blz_status blz_query_prepare(blz_ctx *ctx, const uint8_t *query, int32_t qlen, c
     qq->qlen = qlen;
     qq->params = p;
     qq->n_entries = prefix[kWords];
+    qq->compact_ok = compact_ok;
     if (!check(ctx, hipMalloc((void **)&qq->d_query, qlen + 64), "query") ||
         !check(ctx, hipMalloc((void **)&qq->d_prefix, sizeof(uint32_t) * (kWords + 1)), "prefix") ||
-        !check(ctx, hipMalloc((void **)&qq->d_entries, sizeof(uint16_t) * entries.size()), "entries")) {
+        !check(ctx, hipMalloc((void **)&qq->d_entries, sizeof(uint16_t) * entries.size()), "entries") ||
+        !check(ctx, hipMalloc((void **)&qq->d_rel16, sizeof(uint16_t) * (kWords + 1)), "rel16") ||
+        !check(ctx, hipMalloc((void **)&qq->d_blockbase, sizeof(uint32_t) * kRelBlocks), "blockbase")) {
         blz_query_free(qq);
         return BLZ_ENOMEM;
     }
@@ -315,7 +355,11 @@ blz_status blz_query_prepare(blz_ctx *ctx, const uint8_t *query, int32_t qlen, c
         !check(ctx, hipMemcpy(qq->d_prefix, prefix.data(), sizeof(uint32_t) * (kWords + 1), hipMemcpyHostToDevice),
                "prefix copy") ||
         !check(ctx, hipMemcpy(qq->d_entries, entries.data(), sizeof(uint16_t) * entries.size(), hipMemcpyHostToDevice),
-               "entries copy")) {
+               "entries copy") ||
+        !check(ctx, hipMemcpy(qq->d_rel16, rel16.data(), sizeof(uint16_t) * (kWords + 1), hipMemcpyHostToDevice),
+               "rel16 copy") ||
+        !check(ctx, hipMemcpy(qq->d_blockbase, blockbase.data(), sizeof(uint32_t) * kRelBlocks, hipMemcpyHostToDevice),
+               "blockbase copy")) {
         blz_query_free(qq);
         return BLZ_EDEVICE;
     }
@@ -328,6 +372,8 @@ void blz_query_free(blz_query *q) {
     if (q->d_query) (void)hipFree(q->d_query);
     if (q->d_prefix) (void)hipFree(q->d_prefix);
     if (q->d_entries) (void)hipFree(q->d_entries);
+    if (q->d_rel16) (void)hipFree(q->d_rel16);
+    if (q->d_blockbase) (void)hipFree(q->d_blockbase);
     delete q;
 }
 
@@ -335,64 +381,204 @@ void blz_query_free(blz_query *q) {
 
 namespace {
 
-// One query over the database: seed stage, gapped stage, results copied into the sink.
-blz_status run_one(blz_ctx *ctx, const blz_query *q, const blz_db *db, blz_hsp_sink *sink,
-                   blz_run_stats *stats) {
-    const blz_params &p = q->params;
-    const uint32_t ring = ring_words_for(q->qlen, p.window);
-    int waves = kSeedWavesMax;
-    while (waves > 1 && seed_lds_bytes(ring, waves) > ctx->lds_per_block) waves--;
-    if (seed_lds_bytes(ring, waves) > ctx->lds_per_block) {
-        ctx->err = "query too long for the diagonal ring in LDS";
-        return BLZ_EINVAL;
+QueryDev query_dev(const blz_query *q) {
+    QueryDev d{};
+    d.query = q->d_query;
+    d.prefix = q->d_prefix;
+    d.entries = q->d_entries;
+    d.rel16 = q->d_rel16;
+    d.blockbase = q->d_blockbase;
+    d.n_entries = (uint32_t)q->n_entries;
+    d.qlen = q->qlen;
+    const uint32_t ring = ring_words_for(q->qlen, q->params.window);
+    d.ring_mask = ring - 1;
+    d.claim_mask = std::min<uint32_t>(ring, 1024) - 1;
+    d.window = q->params.window;
+    d.x_ungapped = q->params.x_ungapped;
+    d.ungapped_cutoff = q->params.ungapped_cutoff;
+    d.x_gapped = q->params.x_gapped;
+    d.gap_open = q->params.gap_open;
+    d.gap_extend = q->params.gap_extend;
+    d.gapped_cutoff = q->params.gapped_cutoff;
+    return d;
+}
+
+// Launch shape of the LDS-resident seed kernel for a set of queries (sizes are maxima).
+struct SeedPlan {
+    SeedArgs sa{};
+    bool lds_table = false;
+    int waves = 0;
+    int blocks = 0;
+};
+
+bool plan_seed2(blz_ctx *ctx, const std::vector<QueryDev> &qs, const std::vector<int> &idx, bool lds_table,
+                SeedPlan &pl) {
+    SeedArgs &sa = pl.sa;
+    for (int i : idx) {
+        sa.qlen = std::max(sa.qlen, qs[i].qlen);
+        sa.ring_mask = std::max(sa.ring_mask, qs[i].ring_mask);
+        sa.claim_mask = std::max(sa.claim_mask, qs[i].claim_mask);
+        sa.n_entries = std::max(sa.n_entries, qs[i].n_entries);
     }
-    const int seed_blocks = std::max(1, ctx->num_cus * 8 / waves);
-    const size_t nwaves = (size_t)seed_blocks * waves;
+    const size_t cap = ctx->lds_per_block;
+    int waves = ctx->tune.seed_waves ? ctx->tune.seed_waves : 8;
+    while (waves > 1 && seed2_lds_bytes(sa, lds_table, waves) > cap) waves--;
+    if (seed2_lds_bytes(sa, lds_table, waves) > cap) return false;
+    const size_t used = seed2_lds_bytes(sa, lds_table, waves);
+    int per_cu = std::max<int>(1, (int)(cap / used));
+    per_cu = std::min(per_cu, std::max(1, 32 / waves));
+    if (ctx->tune.seed_blocks_per_cu) per_cu = ctx->tune.seed_blocks_per_cu;
+    pl.lds_table = lds_table;
+    pl.waves = waves;
+    pl.blocks = std::max(1, ctx->num_cus * per_cu);
+    return true;
+}
+
+// Prelim stage of queries[0..nq) over the database: seed launches (one per query-size bin),
+// one gapped launch, results copied into the sinks.
+blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, const blz_db *db,
+                       blz_hsp_sink *sinks, blz_run_stats *stats) {
+    for (int i = 0; i < nq; i++)
+        if (!queries[i] || queries[i]->ctx != ctx) return BLZ_EINVAL;
+    const int variant = ctx->tune.seed_variant ? ctx->tune.seed_variant : 2;
     const size_t nsubj = (size_t)db->nsubj;
-    if (!ensure(ctx, ctx->stage, ctx->stage_records, nwaves * kStageCap, "stage") ||
-        !ensure(ctx, ctx->pool, ctx->pool_cap, std::max<size_t>(1 << 22, 2 * nsubj), "pool") ||
-        !ensure(ctx, ctx->work, ctx->work_cap, std::max<size_t>(nsubj, 1), "work") ||
+    std::vector<QueryDev> qs(nq);
+    for (int i = 0; i < nq; i++) qs[i] = query_dev(queries[i]);
+
+    // seed launch plans: query-size bins (tables in LDS / tables in global memory)
+    std::vector<SeedPlan> plans;
+    int max_waves = 1;
+    size_t max_blocks = 1;
+    if (variant == 1) {
+        if (nq != 1) {
+            ctx->err = "seed variant 1 runs one query at a time";
+            return BLZ_EINVAL;
+        }
+        SeedPlan pl;
+        const uint32_t ring = qs[0].ring_mask + 1;
+        int waves = ctx->tune.seed_waves ? std::min(ctx->tune.seed_waves, kSeedWavesMax) : kSeedWavesMax;
+        while (waves > 1 && seed_lds_bytes(ring, waves) > ctx->lds_per_block) waves--;
+        if (seed_lds_bytes(ring, waves) > ctx->lds_per_block) {
+            ctx->err = "query too long for the diagonal ring in LDS";
+            return BLZ_EINVAL;
+        }
+        pl.waves = waves;
+        pl.blocks = std::max(1, ctx->num_cus * 8 / waves);
+        plans.push_back(pl);
+    } else {
+        std::vector<int> in_lds, in_global;
+        for (int i = 0; i < nq; i++) {
+            SeedPlan probe;
+            if (queries[i]->compact_ok && plan_seed2(ctx, qs, {i}, true, probe) && probe.waves >= 2)
+                in_lds.push_back(i);
+            else
+                in_global.push_back(i);
+        }
+        for (int pass = 0; pass < 2; pass++) {
+            const std::vector<int> &idx = pass == 0 ? in_lds : in_global;
+            if (idx.empty()) continue;
+            // queries of one bin must be contiguous in the launch's item space
+            SeedPlan pl;
+            if (!plan_seed2(ctx, qs, idx, pass == 0, pl)) {
+                ctx->err = "query too long for the diagonal ring in LDS";
+                return BLZ_EINVAL;
+            }
+            pl.sa.nq = (int32_t)idx.size();
+            // remember which queries this plan covers through its first index (set below)
+            pl.sa.s_begin = idx.front();
+            plans.push_back(pl);
+        }
+        // bins must map to contiguous query ranges: reorder the query table so each bin's
+        // queries are adjacent
+        std::vector<QueryDev> ordered;
+        std::vector<int> order;
+        for (int i : in_lds) order.push_back(i);
+        for (int i : in_global) order.push_back(i);
+        for (int i : order) ordered.push_back(qs[i]);
+        qs.swap(ordered);
+        int first = 0;
+        for (auto &pl : plans) {
+            pl.sa.s_begin = first;  // temporarily: first query of the bin in the reordered table
+            first += pl.sa.nq;
+        }
+        ctx->batch_order = order;
+    }
+    if (variant == 1) ctx->batch_order.assign(1, 0);
+    for (auto &pl : plans) {
+        max_waves = std::max(max_waves, pl.waves);
+        max_blocks = std::max<size_t>(max_blocks, (size_t)pl.blocks * pl.waves);
+    }
+    const int gapped_threads = ctx->tune.gapped_threads ? ctx->tune.gapped_threads : 64;
+    const int gapped_blocks = ctx->tune.gapped_blocks ? ctx->tune.gapped_blocks : kGappedBlocks * 64 / gapped_threads;
+    const size_t gapped_lanes = (size_t)gapped_blocks * gapped_threads;
+    const size_t per_q = std::max<size_t>(nsubj, 1);
+    if (!ensure(ctx, ctx->stage, ctx->stage_records, max_blocks * kStageCap, "stage") ||
+        !ensure(ctx, ctx->pool, ctx->pool_cap, std::max<size_t>(1 << 22, 2 * per_q * (size_t)nq), "pool") ||
+        !ensure(ctx, ctx->work, ctx->work_cap, per_q * (size_t)nq, "work") ||
         !ensure(ctx, ctx->ovf, ctx->ovf_cap, 1 << 16, "ovf") ||
-        !ensure(ctx, ctx->dp, ctx->dp_entries, (size_t)kGappedBlocks * 64 * kDpRing, "dp") ||
-        !ensure(ctx, ctx->saved, ctx->saved_records, (size_t)kGappedBlocks * 64 * kSavedCap, "saved") ||
-        !ensure(ctx, ctx->hsps, ctx->hsp_cap, std::max<size_t>(1 << 22, nsubj), "hsps") ||
-        !ensure(ctx, ctx->results, ctx->results_cap, std::max<size_t>(nsubj, 1), "results"))
+        !ensure(ctx, ctx->dp, ctx->dp_entries, gapped_lanes * kDpRing, "dp") ||
+        !ensure(ctx, ctx->saved, ctx->saved_records, gapped_lanes * kSavedCap, "saved") ||
+        !ensure(ctx, ctx->hsps, ctx->hsp_cap, std::max<size_t>(1 << 22, per_q), "hsps") ||
+        !ensure(ctx, ctx->results, ctx->results_cap, per_q * (size_t)nq, "results") ||
+        !ensure(ctx, ctx->qdev, ctx->qdev_cap, (size_t)nq, "query table") ||
+        !ensure(ctx, ctx->item_ctr, ctx->item_ctr_cap, plans.size() + 1, "item counters"))
         return BLZ_ENOMEM;
 
     hipStream_t st = ctx->stream;
-    if (!check(ctx, hipMemsetAsync(ctx->ctr, 0, sizeof(DevCounters), st), "memset")) return BLZ_EDEVICE;
+    if (!check(ctx, hipMemcpyAsync(ctx->qdev, qs.data(), sizeof(QueryDev) * nq, hipMemcpyHostToDevice, st),
+               "query table") ||
+        !check(ctx, hipMemsetAsync(ctx->ctr, 0, sizeof(DevCounters), st), "memset") ||
+        !check(ctx, hipMemsetAsync(ctx->item_ctr, 0, sizeof(uint32_t) * (plans.size() + 1), st), "memset"))
+        return BLZ_EDEVICE;
+    DevCounters *ctr = ctx->ctr;  // one counter block for the run; one item counter per seed launch
 
-    SeedArgs sa{};
-    sa.res = db->res;
-    sa.offs = db->offs;
-    sa.s_begin = 0;
-    sa.s_end = db->nsubj;
-    sa.query = q->d_query;
-    sa.qlen = q->qlen;
-    sa.prefix = q->d_prefix;
-    sa.entries = q->d_entries;
-    sa.window = p.window;
-    sa.x_ungapped = p.x_ungapped;
-    sa.ungapped_cutoff = p.ungapped_cutoff;
-    sa.ring_mask = ring - 1;
-    sa.stage = ctx->stage;
-    sa.stage_cap = kStageCap;
-    sa.pool = ctx->pool;
-    sa.pool_cap = ctx->pool_cap;
-    sa.work = ctx->work;
-    sa.ovf = ctx->ovf;
-    sa.ovf_cap = (uint32_t)ctx->ovf_cap;
-    sa.ctr = ctx->ctr;
-
+    if (!check(ctx, hipEventRecord(ctx->ev[0], st), "event")) return BLZ_EDEVICE;
+    for (size_t pi = 0; pi < plans.size() && nsubj > 0; pi++) {
+        SeedPlan &pl = plans[pi];
+        SeedArgs &sa = pl.sa;
+        sa.res = db->res;
+        sa.offs = db->offs;
+        sa.stage = ctx->stage;
+        sa.stage_cap = kStageCap;
+        sa.pool = ctx->pool;
+        sa.pool_cap = ctx->pool_cap;
+        sa.work = ctx->work;
+        sa.ovf = ctx->ovf;
+        sa.ovf_cap = (uint32_t)ctx->ovf_cap;
+        sa.ctr = ctr;
+        hipError_t e;
+        if (variant == 1) {
+            const QueryDev &d = qs[0];
+            sa.s_begin = 0;
+            sa.s_end = db->nsubj;
+            sa.query = d.query;
+            sa.qlen = d.qlen;
+            sa.prefix = d.prefix;
+            sa.entries = d.entries;
+            sa.window = d.window;
+            sa.x_ungapped = d.x_ungapped;
+            sa.ungapped_cutoff = d.ungapped_cutoff;
+            sa.ring_mask = d.ring_mask;
+            e = launch_seed(sa, pl.blocks, pl.waves, st);
+        } else {
+            const int first = (int)sa.s_begin;
+            sa.qd = ctx->qdev + first;
+            sa.s_begin = 0;
+            sa.s_end = db->nsubj;
+            const int64_t target_items = (int64_t)pl.blocks * 8;
+            sa.subj_per_item = std::max<int64_t>(16, (db->nsubj * sa.nq + target_items - 1) / target_items);
+            sa.subj_per_item = std::min<int64_t>(sa.subj_per_item, std::max<int64_t>(db->nsubj, 1));
+            sa.items_per_query = (uint32_t)((db->nsubj + sa.subj_per_item - 1) / sa.subj_per_item);
+            sa.qi_base = first;
+            sa.item_ctr = ctx->item_ctr + pi;
+            e = launch_seed2(sa, pl.lds_table, pl.blocks, pl.waves, st);
+        }
+        if (!check(ctx, e, "k_seed launch")) return BLZ_EDEVICE;
+    }
     GappedArgs ga{};
     ga.res = db->res;
     ga.offs = db->offs;
-    ga.query = q->d_query;
-    ga.qlen = q->qlen;
-    ga.x_gapped = p.x_gapped;
-    ga.gap_open = p.gap_open;
-    ga.gap_extend = p.gap_extend;
-    ga.gapped_cutoff = p.gapped_cutoff;
+    ga.qd = ctx->qdev;
     ga.pool = ctx->pool;
     ga.work = ctx->work;
     ga.dp_scratch = ctx->dp;
@@ -404,57 +590,64 @@ blz_status run_one(blz_ctx *ctx, const blz_query *q, const blz_db *db, blz_hsp_s
     ga.results = ctx->results;
     ga.ovf = ctx->ovf;
     ga.ovf_cap = (uint32_t)ctx->ovf_cap;
-    ga.ctr = ctx->ctr;
-
-    if (!check(ctx, hipEventRecord(ctx->ev[0], st), "event") ||
-        (db->nsubj > 0 && !check(ctx, launch_seed(sa, seed_blocks, waves, st), "k_seed launch")) ||
-        !check(ctx, hipEventRecord(ctx->ev[1], st), "event") ||
-        (db->nsubj > 0 && !check(ctx, launch_gapped(ga, kGappedBlocks, st), "k_gapped launch")) ||
+    ga.ctr = ctr;
+    if (!check(ctx, hipEventRecord(ctx->ev[1], st), "event") ||
+        (nsubj > 0 && !check(ctx, launch_gapped(ga, gapped_blocks, gapped_threads, st), "k_gapped launch")) ||
         !check(ctx, hipEventRecord(ctx->ev[2], st), "event"))
         return BLZ_EDEVICE;
 
     DevCounters c{};
-    if (!check(ctx, hipMemcpyAsync(&c, ctx->ctr, sizeof(c), hipMemcpyDeviceToHost, st), "counters") ||
+    if (!check(ctx, hipMemcpyAsync(&c, ctr, sizeof(c), hipMemcpyDeviceToHost, st), "counters") ||
         !check(ctx, hipStreamSynchronize(st), "sync"))
         return BLZ_EDEVICE;
-
     std::vector<ResultItem> res(c.result_count);
-    std::vector<Hsp> hs(c.hsp_used);
+    std::vector<Hsp> hs(c.hsp_used <= ctx->hsp_cap ? c.hsp_used : 0);
     if ((c.result_count &&
          !check(ctx, hipMemcpyAsync(res.data(), ctx->results, sizeof(ResultItem) * c.result_count,
                                     hipMemcpyDeviceToHost, st), "results")) ||
-        (c.hsp_used && c.hsp_used <= ctx->hsp_cap &&
-         !check(ctx, hipMemcpyAsync(hs.data(), ctx->hsps, sizeof(Hsp) * c.hsp_used, hipMemcpyDeviceToHost, st),
+        (!hs.empty() &&
+         !check(ctx, hipMemcpyAsync(hs.data(), ctx->hsps, sizeof(Hsp) * hs.size(), hipMemcpyDeviceToHost, st),
                 "hsps")) ||
         !check(ctx, hipStreamSynchronize(st), "sync"))
         return BLZ_EDEVICE;
-
     float ms_seed = 0, ms_gapped = 0;
     (void)hipEventElapsedTime(&ms_seed, ctx->ev[0], ctx->ev[1]);
     (void)hipEventElapsedTime(&ms_gapped, ctx->ev[1], ctx->ev[2]);
 
-    std::sort(res.begin(), res.end(), [](const ResultItem &a, const ResultItem &b) { return a.sid < b.sid; });
-    int64_t total = 0;
-    for (auto &r : res) total += r.n;
+    // per query (in the caller's order), subjects ascending
+    std::sort(res.begin(), res.end(), [&](const ResultItem &a, const ResultItem &b) {
+        const int qa = ctx->batch_order[a.qi], qb = ctx->batch_order[b.qi];
+        return qa != qb ? qa < qb : a.sid < b.sid;
+    });
     blz_status rc = BLZ_OK;
-    if (sink) {
-        sink->n_subjects = (int64_t)res.size();
-        sink->n_hsps = total;
-        if ((int64_t)res.size() > sink->subject_cap || total > sink->hsp_cap) {
-            rc = BLZ_ECAPACITY;
-        } else {
-            int64_t o = 0;
-            for (size_t i = 0; i < res.size(); i++) {
-                sink->oids[i] = res[i].sid;
-                sink->counts[i] = res[i].n;
-                std::memcpy(sink->hsps + o, hs.data() + res[i].off, sizeof(Hsp) * res[i].n);
-                o += res[i].n;
+    int64_t total = 0;
+    size_t r0 = 0;
+    for (int qcaller = 0; qcaller < nq; qcaller++) {
+        size_t r1 = r0;
+        int64_t nh = 0;
+        while (r1 < res.size() && ctx->batch_order[res[r1].qi] == qcaller) nh += res[r1++].n;
+        total += nh;
+        if (sinks) {
+            blz_hsp_sink *sink = &sinks[qcaller];
+            sink->n_subjects = (int64_t)(r1 - r0);
+            sink->n_hsps = nh;
+            if ((int64_t)(r1 - r0) > sink->subject_cap || nh > sink->hsp_cap) {
+                rc = BLZ_ECAPACITY;
+            } else {
+                int64_t o = 0;
+                for (size_t i = r0; i < r1; i++) {
+                    sink->oids[i - r0] = res[i].sid;
+                    sink->counts[i - r0] = res[i].n;
+                    std::memcpy(sink->hsps + o, hs.data() + res[i].off, sizeof(Hsp) * res[i].n);
+                    o += res[i].n;
+                }
             }
         }
+        r0 = r1;
     }
     if (stats) {
-        stats->subjects += (uint64_t)db->nsubj;
-        stats->residues += db->nres;
+        stats->subjects += (uint64_t)db->nsubj * (uint64_t)nq;
+        stats->residues += db->nres * (uint64_t)nq;
         stats->word_hits += c.word_hits;
         stats->ungapped_ext += c.ungapped_ext;
         stats->init_hits += c.init_hits;
@@ -466,7 +659,7 @@ blz_status run_one(blz_ctx *ctx, const blz_query *q, const blz_db *db, blz_hsp_s
         stats->ms_seed += ms_seed;
         stats->ms_gapped += ms_gapped;
         stats->ms_total += ms_seed + ms_gapped;
-        stats->seed_launches += 1;
+        stats->seed_launches += (int32_t)plans.size();
         stats->gapped_launches += 1;
     }
     if (c.overflow_count > 0 && rc == BLZ_OK) {
@@ -484,20 +677,24 @@ blz_status blz_prelim_run(blz_ctx *ctx, const blz_query *q, const blz_db *db, bl
                           blz_run_stats *stats) {
     if (!ctx || !q || !db || q->ctx != ctx || db->ctx != ctx) return BLZ_EINVAL;
     (void)hipSetDevice(ctx->device);
-    return run_one(ctx, q, db, sink, stats);
+    return run_queries(ctx, &q, 1, db, sink, stats);
 }
 
 blz_status blz_prelim_run_batch(blz_ctx *ctx, const blz_query *const *queries, int32_t nq, const blz_db *db,
                                 blz_hsp_sink *sinks, blz_run_stats *stats) {
-    if (!ctx || !queries || nq < 0 || !db) return BLZ_EINVAL;
+    if (!ctx || !queries || nq < 0 || !db || db->ctx != ctx) return BLZ_EINVAL;
+    if (nq == 0) return BLZ_OK;
     (void)hipSetDevice(ctx->device);
-    blz_status worst = BLZ_OK;
-    for (int32_t i = 0; i < nq; i++) {
-        blz_status rc = run_one(ctx, queries[i], db, sinks ? &sinks[i] : nullptr, stats);
-        if (rc != BLZ_OK && worst == BLZ_OK) worst = rc;
-        if (rc == BLZ_EDEVICE || rc == BLZ_ENOMEM || rc == BLZ_EINVAL) return rc;
+    if ((ctx->tune.seed_variant ? ctx->tune.seed_variant : 2) == 1) {
+        blz_status worst = BLZ_OK;
+        for (int32_t i = 0; i < nq; i++) {
+            blz_status rc = run_queries(ctx, &queries[i], 1, db, sinks ? &sinks[i] : nullptr, stats);
+            if (rc == BLZ_EDEVICE || rc == BLZ_ENOMEM || rc == BLZ_EINVAL) return rc;
+            if (rc != BLZ_OK && worst == BLZ_OK) worst = rc;
+        }
+        return worst;
     }
-    return worst;
+    return run_queries(ctx, queries, nq, db, sinks, stats);
 }
 
 }  // extern "C"

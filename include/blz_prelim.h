@@ -109,6 +109,21 @@ void *blz_ctx_stream(blz_ctx *ctx);
 
 void blz_default_params(blz_params *p);
 
+/* Launch shape and variant choices of a context (picked by the autotuner in blz_tune.py; no
+ * reference counterpart: the reference hard-codes 64 threads per subject, gpuBLAZE.cu:1639).
+ * Zero fields keep the built-in choice. */
+typedef struct {
+    int32_t seed_variant;      /* 1: global tables, in-place extension; 2: LDS tables, batched */
+    int32_t seed_waves;        /* waves per workgroup of the seed kernel */
+    int32_t seed_blocks_per_cu;/* resident workgroups per CU the seed launch is sized for */
+    int32_t gapped_blocks;     /* workgroups of the gapped kernel */
+    int32_t gapped_threads;    /* threads per workgroup of the gapped kernel */
+    int32_t reserved[3];
+} blz_tuning;
+
+blz_status blz_ctx_set_tuning(blz_ctx *ctx, const blz_tuning *t);
+blz_status blz_ctx_get_tuning(const blz_ctx *ctx, blz_tuning *t);
+
 /* Upload a database once; it stays resident.  residues: concatenated NCBIstdaa letters;
  * offsets[nsubj+1]: subject i is residues[offsets[i] .. offsets[i+1]).  Replaces the host
  * gather and per-chunk H2D copies of BLAZE_GPU (gpuBLAZE.cu:1587-1618) and the DB arrays

@@ -13,20 +13,37 @@ pytestmark = pytest.mark.gpu
 NTHREADS = min(16, os.cpu_count() or 1)
 
 
-@pytest.fixture(scope="module")
-def engine():
+# seed-stage variants: 1 = global tables with in-place extension, 2 = LDS tables with batched
+# extension
+@pytest.fixture(scope="module", params=[1, 2], ids=["seed_v1", "seed_v2"])
+def engine(request):
+    print(f"\n[engine] seed variant {request.param}", flush=True)
     with blz.Engine(0) as eng:
+        eng.set_tuning(seed_variant=request.param)
         yield eng
 
 
+_ORACLE_CACHE = {}
+
+
+def oracle_run(query, res, offs, p_ora):
+    key = (query.tobytes(), res.size, int(offs[-1]), hash(res[: 1 << 16].tobytes()), len(offs),
+           None if p_ora is None else tuple(getattr(p_ora, f) for f, _ in p_ora._fields_))
+    if key not in _ORACLE_CACHE:
+        _ORACLE_CACHE.clear()
+        _ORACLE_CACHE[key] = oracle.prelim_db(query, res, offs, p_ora, nthreads=NTHREADS)
+    return _ORACLE_CACHE[key]
+
+
 def compare(eng, query, res, offs, p_blz=None, p_ora=None, allow_overflow=False):
+    print(f"[case] query {len(query)} aa, {len(offs) - 1} subjects, {int(offs[-1])} residues", flush=True)
     eng.upload_db(res, offs)
     qh = eng.prepare(query, p_blz)
     try:
         got, stats = eng.run(qh, allow_overflow=allow_overflow)
     finally:
         eng.free_query(qh)
-    counts, hsps, ost = oracle.prelim_db(query, res, offs, p_ora, nthreads=NTHREADS)
+    counts, hsps, ost = oracle_run(query, res, offs, p_ora)
     want = oracle.as_lists(counts, hsps)
     have = got.as_dict()
     missing = sorted(set(want) - set(have))
@@ -121,3 +138,28 @@ def test_config1_full_size(engine):
     stats, nsub = compare(engine, q, res, offs)
     assert stats["overflow_subjects"] == 0
     assert nsub >= 0.9 * (hom >= 0).sum()
+
+
+def test_batch_matches_single_queries(engine):
+    """Several queries of mixed length (tables in LDS and in global memory) in one batched run
+    give, per query, exactly the oracle's lists."""
+    if engine.get_tuning()["seed_variant"] == 1:
+        pytest.skip("variant 1 runs one query at a time")
+    qs = [blz.synth_query(100 + i, L) for i, L in enumerate([40, 300, 777, 1200, 3000])]
+    spec = blz.synth_spec(seed=5, n_subjects=1500, homolog_frac=0.05, ident_min=0.35, ident_max=0.7)
+    res, offs, _ = blz.synth_db(spec, qs)
+    engine.upload_db(res, offs)
+    qhs = [engine.prepare(q) for q in qs]
+    try:
+        outs, stats = engine.run_batch(qhs)
+    finally:
+        for h in qhs:
+            engine.free_query(h)
+    assert stats["seed_launches"] >= 1
+    for q, got in zip(qs, outs):
+        counts, hsps, _ = oracle.prelim_db(q, res, offs, nthreads=NTHREADS)
+        want = oracle.as_lists(counts, hsps)
+        have = got.as_dict()
+        assert sorted(want) == sorted(have)
+        for oid in want:
+            assert np.array_equal(want[oid], have[oid]), (len(q), oid)
