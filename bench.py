@@ -87,6 +87,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", default="", help="JSON of blz_tuning fields, e.g. '{\"seed_table\": 2}'")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every GPU box here has one, and the path shards by database "
@@ -100,6 +101,8 @@ def main() -> int:
         f"({time.perf_counter() - t0:.1f} s)")
 
     with blz.Engine(0) as eng:
+        if args.tune:
+            eng.set_tuning(**json.loads(args.tune))
         eng.upload_db(res, offs)
         qh = eng.prepare(q)
         for i in range(args.warmup):
@@ -116,6 +119,7 @@ def main() -> int:
             cells.append(st["dp_cells"])
         elapsed = time.perf_counter() - t_start
         eng.free_query(qh)
+        tuning = eng.get_tuning()
     log(f"timed {args.steps} steps in {elapsed:.3f} s; last stats {st}")
 
     ms_step = 1000.0 * elapsed / args.steps
@@ -150,6 +154,7 @@ def main() -> int:
         "latency_ms": {"p50": 1000 * float(np.percentile(lat, 50)), "p99": 1000 * float(np.percentile(lat, 99))},
         "kernel_ms": {"k_seed": seed_ms, "k_gapped": gapped_ms},
         "bit_exact_sample": exact,
+        "tuning": tuning,
         "roofline": {"kernel": kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,

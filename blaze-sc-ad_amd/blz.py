@@ -54,7 +54,8 @@ class RunStats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("seed_variant", C.c_int32), ("seed_waves", C.c_int32), ("seed_blocks_per_cu", C.c_int32),
-                ("gapped_blocks", C.c_int32), ("gapped_threads", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("gapped_blocks", C.c_int32), ("gapped_threads", C.c_int32), ("seed_table", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
 
 class Sink(C.Structure):

@@ -2,6 +2,11 @@
 #pragma once
 #include <stdint.h>
 
+#ifndef __host__
+#define __host__
+#define __device__
+#endif
+
 namespace blz {
 
 constexpr int kAlpha = 28;           // NCBIstdaa letters
@@ -25,8 +30,16 @@ struct WorkItem {
     int64_t sid;
     uint64_t off;
     int32_t n;
-    int32_t qi;  // query of the batch
+    int32_t qi;      // query of the batch
+    int32_t bucket;  // expected gapped-stage cost class, 0 = heaviest (from the best init hit)
+    int32_t pad;
 };
+
+constexpr int kBuckets = 4;
+// cost class of a subject from its best ungapped score
+__host__ __device__ inline int cost_bucket(int best_ungapped) {
+    return best_ungapped >= 150 ? 0 : best_ungapped >= 80 ? 1 : best_ungapped >= 55 ? 2 : 3;
+}
 
 // HSP record, same layout as blz_hsp in blz_prelim.h.
 struct Hsp {
@@ -49,6 +62,8 @@ struct DevCounters {
     unsigned long long word_hits, ungapped_ext, init_hits, gapped_ext, dp_cells;
     unsigned long long pool_used, hsp_used;
     unsigned int work_count, result_count, overflow_count, next_subject, next_work, next_item;
+    unsigned int bucket_count[kBuckets], bucket_cursor[kBuckets];
+    unsigned int redo_count, next_redo;
 };
 
 }  // namespace blz

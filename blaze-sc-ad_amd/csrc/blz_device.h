@@ -39,6 +39,7 @@ struct SeedArgs {
     InitHit *pool;             // init hits of all subjects
     unsigned long long pool_cap;
     WorkItem *work;            // one item per subject with init hits
+    uint32_t work_cap;
     int64_t *ovf;              // subjects past a cap
     uint32_t ovf_cap;
     DevCounters *ctr;
@@ -57,7 +58,13 @@ struct GappedArgs {
     const QueryDev *qd;        // queries of the batch, indexed by WorkItem::qi
     InitHit *pool;
     const WorkItem *work;
-    int2 *dp_scratch;          // per-lane DP ring, dp_wmask+1 entries per lane
+    const uint32_t *order;     // work indices to process (cost order, or the redo list)
+    const uint32_t *count;     // how many of them (device counter) ...
+    uint32_t count_cap;        // ... at most this many
+    uint32_t *next;            // work-stealing cursor over order[]
+    uint32_t *redo;            // work indices whose DP band outgrew the LDS ring
+    uint32_t redo_cap;
+    int2 *dp_scratch;          // wide variant: per-lane DP ring, dp_wmask+1 entries per lane
     int32_t dp_wmask;
     Hsp *hsp_scratch;          // per-lane saved HSPs, saved_cap per lane
     int32_t saved_cap;
@@ -70,9 +77,17 @@ struct GappedArgs {
 };
 
 size_t seed_lds_bytes(uint32_t ring_words, int waves_per_block);
-size_t seed2_lds_bytes(const SeedArgs &a, bool lds_table, int waves);
-hipError_t launch_seed2(const SeedArgs &a, bool lds_table, int blocks, int waves, hipStream_t st);
+size_t seed2_lds_bytes(const SeedArgs &a, int table, int waves);
+hipError_t launch_seed2(const SeedArgs &a, int table, int blocks, int waves, hipStream_t st);
 hipError_t launch_seed(const SeedArgs &a, int blocks, int waves_per_block, hipStream_t st);
-hipError_t launch_gapped(const GappedArgs &a, int blocks, int threads, hipStream_t st);
+// cost-ordered work list: order[] receives the work indices, heaviest class first
+hipError_t launch_order(const WorkItem *work, DevCounters *ctr, uint32_t *order, uint32_t work_cap, int blocks,
+                        hipStream_t st);
+// gapped stage with the DP band in LDS (16-bit scores relative to the row's best); items whose
+// band outgrows the ring go to a.redo
+hipError_t launch_gapped_lds(const GappedArgs &a, int blocks, hipStream_t st);
+// gapped stage with the DP band in global memory (wide rings), for the redo list
+hipError_t launch_gapped_wide(const GappedArgs &a, int blocks, int threads, hipStream_t st);
+constexpr int kLdsRing = 128;  // LDS DP ring entries per lane
 
 }  // namespace blz

@@ -1,0 +1,422 @@
+// blz_gapped.hip -- gapped stage: one lane per subject of the work list.
+//
+// Per subject (NCBI BLAST_GetGappedScore, called at BLAZE_ncbi/blast_engine.c:767):
+//   init hits in score order; an init hit inside an HSP already saved with at least its score
+//   is skipped; otherwise the gapped start is found on its diagonal, the score-only semi-gapped
+//   X-drop DP runs left (reversed prefixes, start included) and right (suffixes), and the HSP is
+//   saved when left + right reaches the gapped cutoff.  Then the common-endpoint purge
+//   (blast_engine.c:792) and the score sort (blast_engine.c:801).
+// Reference counterpart: gpuBLAZE_everything + semiGappedKernel*, BLAZE_gpu/gpuBLAZE.cu:94-677,
+// 1254-1397 (which runs the full rectangle, not the X-drop band, and only flags survivors).
+//
+// Two DP storages, same arithmetic:
+//   LDS ring (k_gapped_lds): kLdsRing cells per lane, each one 32-bit word holding the cell's
+//     best and best-gap scores as 16-bit offsets from the best score at the start of the row
+//     that wrote it.  Every live cell holds a value within [best - X - gap_open - gap_extend,
+//     best + 11], so 16 bits are exact; anything further below the running best can never
+//     reach the X-drop window again and is stored as minus infinity.  Lanes of a wave take
+//     consecutive entries of a cost-ordered work list, so they run DPs of similar size.
+//   Global ring (k_gapped_wide): 32-bit pairs in a per-lane ring of dp_wmask+1 cells, for the
+//     few subjects whose band does not fit the LDS ring.
+#include <hip/hip_runtime.h>
+
+#include "blz_common.h"
+#include "blz_dev_common.h"
+#include "blz_device.h"
+
+namespace blz {
+
+namespace {
+
+// --- DP storage policies ------------------------------------------------------------------
+// get(b) -> (best, best_gap) of column b as written by the previous row;
+// put(b, best, best_gap) for the current row.  start_row() is called before each row with
+// the best score at the start of that row.
+
+struct GlobalRing {
+    int2 *ring;
+    int wmask;
+    __device__ __forceinline__ void start_row(int) {}
+    __device__ __forceinline__ int2 get(int b) const { return ring[b & wmask]; }
+    __device__ __forceinline__ void put(int b, int h, int e) { ring[b & wmask] = make_int2(h, e); }
+    __device__ __forceinline__ void put_h(int b, int h) { ring[b & wmask].x = h; }
+    __device__ __forceinline__ int capacity() const { return wmask + 1; }
+};
+
+struct LdsRing16 {
+    uint32_t *ring;  // [kLdsRing][64], this lane's column
+    int base_prev = 0, base_cur = 0;
+    static constexpr int kNeg16 = -32768;
+    __device__ __forceinline__ void start_row(int best) {
+        base_prev = base_cur;
+        base_cur = best;
+    }
+    __device__ __forceinline__ static int dec(int v16, int base) { return v16 == kNeg16 ? kMinIntDp : base + v16; }
+    __device__ __forceinline__ static uint32_t enc1(int v, int base) {
+        const int d = v - base;
+        return (uint32_t)(uint16_t)(int16_t)(d < -32767 ? kNeg16 : d);
+    }
+    __device__ __forceinline__ int2 get(int b) const {
+        const uint32_t w = ring[(b & (kLdsRing - 1)) * 64];
+        return make_int2(dec((int16_t)(w & 0xFFFFu), base_prev), dec((int16_t)(w >> 16), base_prev));
+    }
+    __device__ __forceinline__ void put(int b, int h, int e) {
+        ring[(b & (kLdsRing - 1)) * 64] = enc1(h, base_cur) | (enc1(e, base_cur) << 16);
+    }
+    // a cell that failed the X-drop: both scores become minus infinity (its gap score can no
+    // longer reach the X-drop window either)
+    __device__ __forceinline__ void put_h(int b, int) { ring[(b & (kLdsRing - 1)) * 64] = 0x80008000u; }
+    __device__ __forceinline__ int capacity() const { return kLdsRing; }
+};
+
+// Score-only semi-gapped X-drop DP (NCBI Blast_SemiGappedAlign, score-only path).  Rows walk
+// A, the live band [first_b_index, b_size) walks B.  Sets *ovf when the band outgrows R.
+template <class R>
+__device__ int semi_gapped(const int16_t (*mat)[28], const uint8_t *A, const uint8_t *B, int M, int N,
+                           int x_dropoff, int gap_open, int gap_extend, bool reverse, int *a_off, int *b_off,
+                           R &ring, bool *ovf, unsigned long long *cells) {
+    const int goe = gap_open + gap_extend;
+    *a_off = 0;
+    *b_off = 0;
+    if (x_dropoff < goe) x_dropoff = goe;
+    if (N <= 0 || M <= 0) return 0;
+    const int W = ring.capacity();
+    ring.start_row(0);
+    ring.start_row(0);
+    int score = -goe;
+    ring.put(0, 0, -goe);
+    int i;
+    for (i = 1; i <= N; i++) {
+        if (score < -x_dropoff) break;
+        if (i >= W) {
+            *ovf = true;
+            return 0;
+        }
+        ring.put(i, score, score - goe);
+        score -= gap_extend;
+    }
+    int b_size = i;
+    int best_score = 0;
+    int first_b_index = 0;
+    unsigned long long nc = 0;
+    for (int a_index = 1; a_index <= M; a_index++) {
+        ring.start_row(best_score);
+        const int16_t *mrow = mat[reverse ? A[M - a_index] : A[a_index]];
+        int sc = kMinIntDp;
+        int score_gap_row = kMinIntDp;
+        int last_b_index = first_b_index;
+        for (int b_index = first_b_index; b_index < b_size; b_index++) {
+            const int2 cell = ring.get(b_index);
+            int score_gap_col = cell.y;
+            // diagonal score of the next column; the last column (b_index == N) has none
+            const int next_score =
+                b_index < N ? cell.x + mrow[reverse ? B[N - 1 - b_index] : B[b_index + 1]] : kMinIntDp;
+            nc++;
+            if (sc < score_gap_col) sc = score_gap_col;
+            if (sc < score_gap_row) sc = score_gap_row;
+            if (best_score - sc > x_dropoff) {
+                if (b_index == first_b_index)
+                    first_b_index++;
+                else
+                    ring.put_h(b_index, kMinIntDp);
+            } else {
+                last_b_index = b_index;
+                if (sc > best_score) {
+                    best_score = sc;
+                    *a_off = a_index;
+                    *b_off = b_index;
+                }
+                score_gap_row -= gap_extend;
+                score_gap_col -= gap_extend;
+                const int open = sc - goe;
+                const int e = open > score_gap_col ? open : score_gap_col;
+                score_gap_row = open > score_gap_row ? open : score_gap_row;
+                ring.put(b_index, sc, e);
+            }
+            sc = next_score;
+        }
+        if (first_b_index == b_size) break;
+        if (last_b_index < b_size - 1) {
+            b_size = last_b_index + 1;
+        } else {
+            while (score_gap_row >= best_score - x_dropoff && b_size <= N) {
+                if (b_size - first_b_index >= W) {
+                    *ovf = true;
+                    return 0;
+                }
+                ring.put(b_size, score_gap_row, score_gap_row - goe);
+                score_gap_row -= gap_extend;
+                b_size++;
+            }
+        }
+        if (b_size <= N) {
+            if (b_size - first_b_index >= W) {
+                *ovf = true;
+                return 0;
+            }
+            ring.put(b_size, kMinIntDp, kMinIntDp);
+            b_size++;
+        }
+    }
+    *cells += nc;
+    return best_score;
+}
+
+__device__ __forceinline__ bool init_before(const InitHit &a, const InitHit &b) {
+    if (a.score != b.score) return a.score > b.score;
+    if (a.s_start != b.s_start) return a.s_start < b.s_start;
+    if (a.len != b.len) return a.len > b.len;
+    return a.q_start < b.q_start;
+}
+
+// Gapped start: middle of a short segment, else the centre of its best 11-letter window when
+// that window scores above 0, else the segment start (reference restatement at
+// gpuBLAZE.cu:1166-1205).
+__device__ __forceinline__ int gapped_start(const int16_t (*mat)[28], const uint8_t *q, const uint8_t *s,
+                                            int q_start, int s_start, int len) {
+    if (len <= kHspMaxWindow) return q_start + len / 2;
+    int score = 0;
+    for (int i = 0; i < kHspMaxWindow; i++) score += mat[q[q_start + i]][s[s_start + i]];
+    int max_score = score;
+    int max_offset = q_start + kHspMaxWindow - 1;
+    for (int i = q_start + kHspMaxWindow; i < q_start + len; i++) {
+        const int k = i - q_start;
+        score -= mat[q[i - kHspMaxWindow]][s[s_start + k - kHspMaxWindow]];
+        score += mat[q[i]][s[s_start + k]];
+        if (score > max_score) {
+            max_score = score;
+            max_offset = i;
+        }
+    }
+    return max_score > 0 ? max_offset - kHspMaxWindow / 2 : q_start;
+}
+
+// purge orders and the final order (NCBI s_QueryOffsetCompareHSPs, s_QueryEndCompareHSPs,
+// ScoreCompareHSPs; the gapped start breaks remaining ties)
+__device__ __forceinline__ int cmp_start(const Hsp &a, const Hsp &b) {
+    if (a.q_off != b.q_off) return a.q_off < b.q_off ? -1 : 1;
+    if (a.s_off != b.s_off) return a.s_off < b.s_off ? -1 : 1;
+    if (a.score != b.score) return a.score > b.score ? -1 : 1;
+    if (a.q_end != b.q_end) return a.q_end > b.q_end ? -1 : 1;
+    if (a.s_end != b.s_end) return a.s_end > b.s_end ? -1 : 1;
+    if (a.q_gs != b.q_gs) return a.q_gs < b.q_gs ? -1 : 1;
+    if (a.s_gs != b.s_gs) return a.s_gs < b.s_gs ? -1 : 1;
+    return 0;
+}
+__device__ __forceinline__ int cmp_end(const Hsp &a, const Hsp &b) {
+    if (a.q_end != b.q_end) return a.q_end < b.q_end ? -1 : 1;
+    if (a.s_end != b.s_end) return a.s_end < b.s_end ? -1 : 1;
+    if (a.score != b.score) return a.score > b.score ? -1 : 1;
+    if (a.q_off != b.q_off) return a.q_off > b.q_off ? -1 : 1;
+    if (a.s_off != b.s_off) return a.s_off > b.s_off ? -1 : 1;
+    if (a.q_gs != b.q_gs) return a.q_gs < b.q_gs ? -1 : 1;
+    if (a.s_gs != b.s_gs) return a.s_gs < b.s_gs ? -1 : 1;
+    return 0;
+}
+__device__ __forceinline__ int cmp_score(const Hsp &a, const Hsp &b) {
+    if (a.score != b.score) return a.score > b.score ? -1 : 1;
+    if (a.s_off != b.s_off) return a.s_off < b.s_off ? -1 : 1;
+    if (a.s_end != b.s_end) return a.s_end > b.s_end ? -1 : 1;
+    if (a.q_off != b.q_off) return a.q_off < b.q_off ? -1 : 1;
+    if (a.q_end != b.q_end) return a.q_end > b.q_end ? -1 : 1;
+    if (a.q_gs != b.q_gs) return a.q_gs < b.q_gs ? -1 : 1;
+    if (a.s_gs != b.s_gs) return a.s_gs < b.s_gs ? -1 : 1;
+    return 0;
+}
+
+template <int (*CMP)(const Hsp &, const Hsp &)>
+__device__ void insertion_sort(Hsp *h, int n) {
+    for (int i = 1; i < n; i++) {
+        Hsp x = h[i];
+        int j = i - 1;
+        while (j >= 0 && CMP(x, h[j]) < 0) {
+            h[j + 1] = h[j];
+            j--;
+        }
+        h[j + 1] = x;
+    }
+}
+
+enum ItemOutcome { kDone = 0, kRedo = 1, kOverflow = 2 };  // kRedo: band or HSP count past this pass's caps
+
+// Whole gapped stage of one subject.  Returns kRedo if the DP band outgrew the ring (nothing
+// was written), kOverflow if a cap was hit.
+template <class R>
+__device__ int gapped_item(const GappedArgs &a, const int16_t (*mat)[28], uint32_t widx, R &ring, Hsp *saved,
+                           unsigned long long &cells, unsigned long long &gext) {
+    const WorkItem wi = a.work[widx];
+    InitHit *ih = a.pool + wi.off;
+    const int n = wi.n;
+    for (int i = 1; i < n; i++) {  // score order of the init hits (sorted in place once)
+        InitHit x = ih[i];
+        int j = i - 1;
+        while (j >= 0 && init_before(x, ih[j])) {
+            ih[j + 1] = ih[j];
+            j--;
+        }
+        ih[j + 1] = x;
+    }
+    const uint64_t o0 = a.offs[wi.sid];
+    const int slen = (int)(a.offs[wi.sid + 1] - o0);
+    const uint8_t *S = a.res + o0;
+    const QueryDev &qd = a.qd[wi.qi];
+    const uint8_t *Q = qd.query;
+    const int qlen = qd.qlen;
+    int ns = 0;
+    bool ovf = false;
+    unsigned long long my_cells = 0, my_ext = 0;
+    for (int k = 0; k < n && !ovf; k++) {
+        const InitHit h = ih[k];
+        const int q0 = h.q_start, q1 = h.q_start + h.len - 1;
+        const int s0 = h.s_start, s1 = h.s_start + h.len - 1;
+        bool skip = false;
+        for (int t = 0; t < ns && !skip; t++) {
+            const Hsp &u = saved[t];
+            skip = h.score <= u.score && u.q_off <= q0 && q0 <= u.q_end && u.s_off <= s0 && s0 <= u.s_end &&
+                   u.q_off <= q1 && q1 <= u.q_end && u.s_off <= s1 && s1 <= u.s_end;
+        }
+        if (skip) continue;
+        const int q_gs = gapped_start(mat, Q, S, h.q_start, h.s_start, h.len);
+        const int s_gs = h.s_start + (q_gs - h.q_start);
+        int la, lb, ra, rb;
+        const int left = semi_gapped(mat, Q, S, q_gs + 1, s_gs + 1, qd.x_gapped, qd.gap_open, qd.gap_extend, true,
+                                     &la, &lb, ring, &ovf, &my_cells);
+        if (ovf) break;
+        const int right = semi_gapped(mat, Q + q_gs, S + s_gs, qlen - q_gs - 1, slen - s_gs - 1, qd.x_gapped,
+                                      qd.gap_open, qd.gap_extend, false, &ra, &rb, ring, &ovf, &my_cells);
+        if (ovf) break;
+        my_ext++;
+        const int score = left + right;
+        if (score >= qd.gapped_cutoff) {
+            if (ns >= a.saved_cap) return kRedo;  // more HSPs than this pass can hold
+            saved[ns++] = Hsp{score, q_gs + 1 - la, q_gs + 1 + ra, s_gs + 1 - lb, s_gs + 1 + rb, q_gs, s_gs};
+        }
+    }
+    if (ovf) return kRedo;  // counters are only taken from the pass that completes the subject
+    cells += my_cells;
+    gext += my_ext;
+    if (ns == 0) return kDone;
+    insertion_sort<cmp_start>(saved, ns);
+    int m = 0;
+    for (int i = 0; i < ns; i++) {
+        if (m > 0 && saved[m - 1].q_off == saved[i].q_off && saved[m - 1].s_off == saved[i].s_off) continue;
+        saved[m++] = saved[i];
+    }
+    ns = m;
+    insertion_sort<cmp_end>(saved, ns);
+    m = 0;
+    for (int i = 0; i < ns; i++) {
+        if (m > 0 && saved[m - 1].q_end == saved[i].q_end && saved[m - 1].s_end == saved[i].s_end) continue;
+        saved[m++] = saved[i];
+    }
+    ns = m;
+    insertion_sort<cmp_score>(saved, ns);
+    const unsigned long long base = atomicAdd(&a.ctr->hsp_used, (unsigned long long)ns);
+    if (base + ns > a.hsp_cap) return kOverflow;
+    for (int i = 0; i < ns; i++) a.hsps[base + i] = saved[i];
+    const uint32_t ri = atomicAdd(&a.ctr->result_count, 1u);
+    a.results[ri] = ResultItem{wi.sid, (uint64_t)base, ns, wi.qi};
+    return kDone;
+}
+
+__device__ __forceinline__ void record_overflow(const GappedArgs &a, uint32_t widx) {
+    const WorkItem wi = a.work[widx];
+    const uint32_t k = atomicAdd(&a.ctr->overflow_count, 1u);
+    if (k < a.ovf_cap) a.ovf[k] = ((int64_t)wi.qi << kOvfShift) | wi.sid;
+}
+
+}  // namespace
+
+// Work list in cost order: histogram counts per class come from the seed stage.
+__global__ void __launch_bounds__(256) k_order(const WorkItem *work, DevCounters *ctr, uint32_t *order,
+                                               uint32_t work_cap) {
+    const uint32_t n = ctr->work_count < work_cap ? ctr->work_count : work_cap;
+    uint32_t base[kBuckets];
+    uint32_t acc = 0;
+    for (int b = 0; b < kBuckets; b++) {
+        base[b] = acc;
+        acc += ctr->bucket_count[b];
+    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int b = work[i].bucket;
+        order[base[b] + atomicAdd(&ctr->bucket_cursor[b], 1u)] = i;
+    }
+}
+
+__global__ void __launch_bounds__(64) k_gapped_lds(GappedArgs a) {
+    __shared__ int16_t mat[28][28];
+    __shared__ uint32_t ring_mem[kLdsRing * 64];
+    for (int i = threadIdx.x; i < 28 * 28; i += blockDim.x) mat[i / 28][i % 28] = kBlosum62Dev[i / 28][i % 28];
+    __syncthreads();
+    const int lane = lane_id();
+    const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    Hsp *saved = a.hsp_scratch + gtid * (uint64_t)a.saved_cap;
+    LdsRing16 ring;
+    ring.ring = ring_mem + lane;
+    const uint32_t n = *a.count < a.count_cap ? *a.count : a.count_cap;
+    unsigned long long cells = 0, gext = 0;
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(a.next, 64u);
+        base = __shfl(base, 0, 64);
+        if (base >= n) break;
+        const uint32_t i = base + lane;
+        if (i < n) {
+            const uint32_t widx = a.order[i];
+            const int r = gapped_item(a, mat, widx, ring, saved, cells, gext);
+            if (r == kRedo) {
+                const uint32_t k = atomicAdd(&a.ctr->redo_count, 1u);
+                if (k < a.redo_cap) a.redo[k] = widx;
+                else record_overflow(a, widx);
+            } else if (r == kOverflow) {
+                record_overflow(a, widx);
+            }
+        }
+    }
+    unsigned long long c = cells, g = gext;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        c += __shfl_xor(c, o, 64);
+        g += __shfl_xor(g, o, 64);
+    }
+    if (lane == 0) {
+        atomicAdd(&a.ctr->dp_cells, c);
+        atomicAdd(&a.ctr->gapped_ext, g);
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_gapped_wide(GappedArgs a) {
+    __shared__ int16_t mat[28][28];
+    for (int i = threadIdx.x; i < 28 * 28; i += blockDim.x) mat[i / 28][i % 28] = kBlosum62Dev[i / 28][i % 28];
+    __syncthreads();
+    const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
+    GlobalRing ring{a.dp_scratch + gtid * (uint64_t)(a.dp_wmask + 1), a.dp_wmask};
+    Hsp *saved = a.hsp_scratch + gtid * (uint64_t)a.saved_cap;
+    const uint32_t n = *a.count < a.count_cap ? *a.count : a.count_cap;
+    unsigned long long cells = 0, gext = 0;
+    for (uint64_t i = gtid; i < n; i += nthreads) {
+        const uint32_t widx = a.order[i];
+        if (gapped_item(a, mat, widx, ring, saved, cells, gext) != kDone) record_overflow(a, widx);
+    }
+    atomicAdd(&a.ctr->dp_cells, cells);
+    atomicAdd(&a.ctr->gapped_ext, gext);
+}
+
+hipError_t launch_order(const WorkItem *work, DevCounters *ctr, uint32_t *order, uint32_t work_cap, int blocks,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k_order, dim3(blocks), dim3(256), 0, st, work, ctr, order, work_cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_gapped_lds(const GappedArgs &a, int blocks, hipStream_t st) {
+    hipLaunchKernelGGL(k_gapped_lds, dim3(blocks), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_gapped_wide(const GappedArgs &a, int blocks, int threads, hipStream_t st) {
+    hipLaunchKernelGGL(k_gapped_wide, dim3(blocks), dim3(threads), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace blz

@@ -48,6 +48,8 @@ struct blz_ctx {
     size_t dp_entries = 0;
     Hsp *saved = nullptr;
     size_t saved_records = 0;
+    Hsp *saved_wide = nullptr;
+    size_t saved_wide_records = 0;
     Hsp *hsps = nullptr;
     size_t hsp_cap = 0;
     ResultItem *results = nullptr;
@@ -56,6 +58,10 @@ struct blz_ctx {
     size_t qdev_cap = 0;
     uint32_t *item_ctr = nullptr;  // work-item counter per seed launch
     size_t item_ctr_cap = 0;
+    uint32_t *order = nullptr;     // work list in cost order
+    size_t order_cap = 0;
+    uint32_t *redo = nullptr;      // work items for the wide gapped pass
+    size_t redo_cap = 0;
     std::vector<int> batch_order;  // table index -> caller's query index
 };
 
@@ -82,11 +88,14 @@ struct blz_query {
 
 namespace {
 
-constexpr int kStageCap = 1024;   // init hits per subject kept on the device
-constexpr int kDpRing = 1024;     // DP band ring per lane (power of two)
-constexpr int kSavedCap = 128;    // saved gapped HSPs per subject
+constexpr int kStageCap = 4096;   // init hits per subject kept on the device
+constexpr int kDpRing = 4096;     // wide gapped pass: DP band ring per lane (power of two)
+constexpr int kSavedCap = 128;    // LDS gapped pass: HSPs per subject before the purge
+constexpr int kSavedCapWide = 4096;  // wide gapped pass: HSPs per subject before the purge
 constexpr int kSeedWavesMax = 4;
-constexpr int kGappedBlocks = 1024;
+constexpr int kGappedBlocks = 64;        // wide gapped pass (redo list): blocks x 64 lanes
+constexpr size_t kRedoCap = 1 << 20;
+constexpr size_t kWorkCapMax = 48u << 20; // work items (subjects with init hits) per run
 
 bool check(blz_ctx *ctx, hipError_t e, const char *what) {
     if (e == hipSuccess) return true;
@@ -207,7 +216,8 @@ void blz_ctx_destroy(blz_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     void *bufs[] = {ctx->ctr,   ctx->stage,   ctx->pool,    ctx->work, ctx->ovf,     ctx->dp,
-                    ctx->saved, ctx->hsps,    ctx->results, ctx->qdev, ctx->item_ctr};
+                    ctx->saved, ctx->hsps,    ctx->results, ctx->qdev, ctx->item_ctr,
+                    ctx->order, ctx->redo, ctx->saved_wide};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &e : ctx->ev)
@@ -220,7 +230,7 @@ blz_status blz_ctx_set_tuning(blz_ctx *ctx, const blz_tuning *t) {
     if (!ctx || !t) return BLZ_EINVAL;
     if (t->seed_variant < 0 || t->seed_variant > 2 || t->seed_waves < 0 || t->seed_waves > 16 ||
         t->seed_blocks_per_cu < 0 || t->gapped_blocks < 0 || t->gapped_threads < 0 || t->gapped_threads > 1024 ||
-        (t->gapped_threads % 64) != 0)
+        (t->gapped_threads % 64) != 0 || t->seed_table < 0 || t->seed_table > 3)
         return BLZ_EINVAL;
     ctx->tune = *t;
     return BLZ_OK;
@@ -392,7 +402,7 @@ QueryDev query_dev(const blz_query *q) {
     d.qlen = q->qlen;
     const uint32_t ring = ring_words_for(q->qlen, q->params.window);
     d.ring_mask = ring - 1;
-    d.claim_mask = std::min<uint32_t>(ring, 1024) - 1;
+    d.claim_mask = std::min<uint32_t>(ring, 256) - 1;
     d.window = q->params.window;
     d.x_ungapped = q->params.x_ungapped;
     d.ungapped_cutoff = q->params.ungapped_cutoff;
@@ -403,15 +413,16 @@ QueryDev query_dev(const blz_query *q) {
     return d;
 }
 
-// Launch shape of the LDS-resident seed kernel for a set of queries (sizes are maxima).
+// Launch shape of the seed kernel for a set of queries (sizes are maxima).  table: where the
+// lookup tables live (2: prefix and entries in LDS, 1: entries in LDS, 0: global memory).
 struct SeedPlan {
     SeedArgs sa{};
-    bool lds_table = false;
+    int table = 0;
     int waves = 0;
     int blocks = 0;
 };
 
-bool plan_seed2(blz_ctx *ctx, const std::vector<QueryDev> &qs, const std::vector<int> &idx, bool lds_table,
+bool plan_seed2(blz_ctx *ctx, const std::vector<QueryDev> &qs, const std::vector<int> &idx, int table,
                 SeedPlan &pl) {
     SeedArgs &sa = pl.sa;
     for (int i : idx) {
@@ -422,16 +433,29 @@ bool plan_seed2(blz_ctx *ctx, const std::vector<QueryDev> &qs, const std::vector
     }
     const size_t cap = ctx->lds_per_block;
     int waves = ctx->tune.seed_waves ? ctx->tune.seed_waves : 8;
-    while (waves > 1 && seed2_lds_bytes(sa, lds_table, waves) > cap) waves--;
-    if (seed2_lds_bytes(sa, lds_table, waves) > cap) return false;
-    const size_t used = seed2_lds_bytes(sa, lds_table, waves);
+    while (waves > 1 && seed2_lds_bytes(sa, table, waves) > cap) waves--;
+    if (seed2_lds_bytes(sa, table, waves) > cap) return false;
+    const size_t used = seed2_lds_bytes(sa, table, waves);
     int per_cu = std::max<int>(1, (int)(cap / used));
     per_cu = std::min(per_cu, std::max(1, 32 / waves));
     if (ctx->tune.seed_blocks_per_cu) per_cu = ctx->tune.seed_blocks_per_cu;
-    pl.lds_table = lds_table;
+    pl.table = table;
     pl.waves = waves;
     pl.blocks = std::max(1, ctx->num_cus * per_cu);
     return true;
+}
+
+// Table placement for one query: the tuned one if it fits, else the richest that leaves room
+// for at least min_waves waves per workgroup.
+int pick_table(blz_ctx *ctx, const std::vector<QueryDev> &qs, int i, bool compact_ok) {
+    const int want = ctx->tune.seed_table;  // 0 auto, else table + 1
+    for (int table = 2; table >= 0; table--) {
+        if (want && table != want - 1) continue;
+        if (table == 2 && !compact_ok) continue;
+        SeedPlan probe;
+        if (plan_seed2(ctx, qs, {i}, table, probe) && (probe.waves >= 4 || table == 0 || want)) return table;
+    }
+    return 0;
 }
 
 // Prelim stage of queries[0..nq) over the database: seed launches (one per query-size bin),
@@ -466,41 +490,26 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
         pl.blocks = std::max(1, ctx->num_cus * 8 / waves);
         plans.push_back(pl);
     } else {
-        std::vector<int> in_lds, in_global;
-        for (int i = 0; i < nq; i++) {
-            SeedPlan probe;
-            if (queries[i]->compact_ok && plan_seed2(ctx, qs, {i}, true, probe) && probe.waves >= 2)
-                in_lds.push_back(i);
-            else
-                in_global.push_back(i);
-        }
-        for (int pass = 0; pass < 2; pass++) {
-            const std::vector<int> &idx = pass == 0 ? in_lds : in_global;
+        std::vector<int> bins[3];
+        for (int i = 0; i < nq; i++) bins[pick_table(ctx, qs, i, queries[i]->compact_ok)].push_back(i);
+        // each bin's queries are made contiguous in the query table; the plan records its first
+        std::vector<int> order;
+        for (int table = 2; table >= 0; table--) {
+            const std::vector<int> &idx = bins[table];
             if (idx.empty()) continue;
-            // queries of one bin must be contiguous in the launch's item space
             SeedPlan pl;
-            if (!plan_seed2(ctx, qs, idx, pass == 0, pl)) {
+            if (!plan_seed2(ctx, qs, idx, table, pl)) {
                 ctx->err = "query too long for the diagonal ring in LDS";
                 return BLZ_EINVAL;
             }
             pl.sa.nq = (int32_t)idx.size();
-            // remember which queries this plan covers through its first index (set below)
-            pl.sa.s_begin = idx.front();
+            pl.sa.s_begin = (int64_t)order.size();  // first query of the bin in the reordered table
+            for (int i : idx) order.push_back(i);
             plans.push_back(pl);
         }
-        // bins must map to contiguous query ranges: reorder the query table so each bin's
-        // queries are adjacent
         std::vector<QueryDev> ordered;
-        std::vector<int> order;
-        for (int i : in_lds) order.push_back(i);
-        for (int i : in_global) order.push_back(i);
         for (int i : order) ordered.push_back(qs[i]);
         qs.swap(ordered);
-        int first = 0;
-        for (auto &pl : plans) {
-            pl.sa.s_begin = first;  // temporarily: first query of the bin in the reordered table
-            first += pl.sa.nq;
-        }
         ctx->batch_order = order;
     }
     if (variant == 1) ctx->batch_order.assign(1, 0);
@@ -508,18 +517,21 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
         max_waves = std::max(max_waves, pl.waves);
         max_blocks = std::max<size_t>(max_blocks, (size_t)pl.blocks * pl.waves);
     }
-    const int gapped_threads = ctx->tune.gapped_threads ? ctx->tune.gapped_threads : 64;
-    const int gapped_blocks = ctx->tune.gapped_blocks ? ctx->tune.gapped_blocks : kGappedBlocks * 64 / gapped_threads;
-    const size_t gapped_lanes = (size_t)gapped_blocks * gapped_threads;
+    const size_t lds_lanes = (size_t)(ctx->tune.gapped_blocks ? ctx->tune.gapped_blocks : ctx->num_cus * 4) * 64;
+    const size_t wide_lanes = (size_t)kGappedBlocks * 64;
     const size_t per_q = std::max<size_t>(nsubj, 1);
+    const size_t work_cap = std::min(per_q * (size_t)nq, kWorkCapMax);
     if (!ensure(ctx, ctx->stage, ctx->stage_records, max_blocks * kStageCap, "stage") ||
-        !ensure(ctx, ctx->pool, ctx->pool_cap, std::max<size_t>(1 << 22, 2 * per_q * (size_t)nq), "pool") ||
-        !ensure(ctx, ctx->work, ctx->work_cap, per_q * (size_t)nq, "work") ||
+        !ensure(ctx, ctx->pool, ctx->pool_cap, std::max<size_t>(1 << 22, 2 * work_cap), "pool") ||
+        !ensure(ctx, ctx->work, ctx->work_cap, work_cap, "work") ||
+        !ensure(ctx, ctx->order, ctx->order_cap, work_cap, "order") ||
+        !ensure(ctx, ctx->redo, ctx->redo_cap, kRedoCap, "redo") ||
         !ensure(ctx, ctx->ovf, ctx->ovf_cap, 1 << 16, "ovf") ||
-        !ensure(ctx, ctx->dp, ctx->dp_entries, gapped_lanes * kDpRing, "dp") ||
-        !ensure(ctx, ctx->saved, ctx->saved_records, gapped_lanes * kSavedCap, "saved") ||
+        !ensure(ctx, ctx->dp, ctx->dp_entries, wide_lanes * kDpRing, "dp") ||
+        !ensure(ctx, ctx->saved, ctx->saved_records, lds_lanes * kSavedCap, "saved") ||
+        !ensure(ctx, ctx->saved_wide, ctx->saved_wide_records, wide_lanes * kSavedCapWide, "saved wide") ||
         !ensure(ctx, ctx->hsps, ctx->hsp_cap, std::max<size_t>(1 << 22, per_q), "hsps") ||
-        !ensure(ctx, ctx->results, ctx->results_cap, per_q * (size_t)nq, "results") ||
+        !ensure(ctx, ctx->results, ctx->results_cap, work_cap, "results") ||
         !ensure(ctx, ctx->qdev, ctx->qdev_cap, (size_t)nq, "query table") ||
         !ensure(ctx, ctx->item_ctr, ctx->item_ctr_cap, plans.size() + 1, "item counters"))
         return BLZ_ENOMEM;
@@ -543,6 +555,7 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
         sa.pool = ctx->pool;
         sa.pool_cap = ctx->pool_cap;
         sa.work = ctx->work;
+        sa.work_cap = (uint32_t)ctx->work_cap;
         sa.ovf = ctx->ovf;
         sa.ovf_cap = (uint32_t)ctx->ovf_cap;
         sa.ctr = ctr;
@@ -571,7 +584,7 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
             sa.items_per_query = (uint32_t)((db->nsubj + sa.subj_per_item - 1) / sa.subj_per_item);
             sa.qi_base = first;
             sa.item_ctr = ctx->item_ctr + pi;
-            e = launch_seed2(sa, pl.lds_table, pl.blocks, pl.waves, st);
+            e = launch_seed2(sa, pl.table, pl.blocks, pl.waves, st);
         }
         if (!check(ctx, e, "k_seed launch")) return BLZ_EDEVICE;
     }
@@ -581,6 +594,12 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
     ga.qd = ctx->qdev;
     ga.pool = ctx->pool;
     ga.work = ctx->work;
+    ga.order = ctx->order;
+    ga.count = &ctr->work_count;
+    ga.count_cap = (uint32_t)ctx->work_cap;
+    ga.next = &ctr->next_work;
+    ga.redo = ctx->redo;
+    ga.redo_cap = (uint32_t)ctx->redo_cap;
     ga.dp_scratch = ctx->dp;
     ga.dp_wmask = kDpRing - 1;
     ga.hsp_scratch = ctx->saved;
@@ -591,8 +610,20 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
     ga.ovf = ctx->ovf;
     ga.ovf_cap = (uint32_t)ctx->ovf_cap;
     ga.ctr = ctr;
+    GappedArgs gw = ga;  // wide pass over the redo list
+    gw.order = ctx->redo;
+    gw.count = &ctr->redo_count;
+    gw.count_cap = (uint32_t)ctx->redo_cap;
+    gw.next = &ctr->next_redo;
+    gw.redo = nullptr;
+    gw.redo_cap = 0;
+    gw.hsp_scratch = ctx->saved_wide;
+    gw.saved_cap = kSavedCapWide;
+    const int lds_blocks = ctx->tune.gapped_blocks ? ctx->tune.gapped_blocks : ctx->num_cus * 4;
     if (!check(ctx, hipEventRecord(ctx->ev[1], st), "event") ||
-        (nsubj > 0 && !check(ctx, launch_gapped(ga, gapped_blocks, gapped_threads, st), "k_gapped launch")) ||
+        (nsubj > 0 && !check(ctx, launch_order(ctx->work, ctr, ctx->order, (uint32_t)ctx->work_cap, 512, st), "k_order launch")) ||
+        (nsubj > 0 && !check(ctx, launch_gapped_lds(ga, lds_blocks, st), "k_gapped_lds launch")) ||
+        (nsubj > 0 && !check(ctx, launch_gapped_wide(gw, kGappedBlocks, 64, st), "k_gapped_wide launch")) ||
         !check(ctx, hipEventRecord(ctx->ev[2], st), "event"))
         return BLZ_EDEVICE;
 

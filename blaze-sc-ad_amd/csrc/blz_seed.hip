@@ -1,19 +1,25 @@
-// blz_seed.hip -- seed stage with LDS-resident query tables and batched ungapped extensions.
+// blz_seed.hip -- seed stage: word lookup, two-hit diagonal state, batched ungapped X-drop.
 //
-// A workgroup stages the query's lookup table (compact: u16 offsets inside 64-word blocks plus
-// one u32 base per block), the query letters and BLOSUM62 in LDS once; each of its waves then
-// takes subjects from a device counter.  Per subject a wave:
-//   1. reads 64 consecutive subject words per step (one coalesced load per lane) and expands
-//      their query hits in rounds of 64 (lane = hit);
-//   2. applies each hit to its diagonal's two-hit state in a per-wave LDS ring; hits of one
+// Work items are (query, block of subjects), taken query-major from a device counter by
+// persistent workgroups.  A workgroup stages its item's query in LDS: letters, BLOSUM62 and,
+// depending on the table placement kTable, the lookup entries (u16 query offsets per word) and
+// the compact lookup prefix (u32 base per 64-word block + u16 offset per word):
+//   kTable 2: prefix and entries in LDS, 1: entries in LDS, prefix in global memory,
+//   0: both in global memory (long queries).
+// Each wave then takes subjects of the item from an LDS counter.  Per subject a wave:
+//   1. copies the subject into its LDS buffer (when it fits) with 16-byte loads;
+//   2. reads 64 consecutive words per step (lane = position), finds each word's hit range, lays
+//      the step's hits out in an LDS list with a wave prefix sum and expands them 64 at a time
+//      (lane = hit);
+//   3. applies each hit to its diagonal's two-hit state in a per-wave LDS ring; hits of one
 //      round on the same diagonal are applied earliest-first (LDS atomicMin on the hit rank,
 //      hashed on the diagonal);
-//   3. does NOT extend a pair on the spot: the pair becomes a task and its diagonal is marked
+//   4. does not extend a pair on the spot: the pair becomes a task and its diagonal is marked
 //      pending; later hits on a pending diagonal are queued behind it.  When the task queue
 //      fills, and at the end of the subject, all tasks are extended at once, one lane each,
 //      their outcome written back to the diagonals, and the queued hits replayed in order.
-// The result is the same state machine as NCBI's two-hit word finder (per diagonal, hits in
-// subject order); only independent diagonals are interleaved.
+// The result is NCBI's two-hit word finder (per diagonal, hits in subject order); only
+// independent diagonals are interleaved.
 // Reference counterpart: gpuBLAZE_full, BLAZE_gpu/gpuBLAZE.cu:885-1252.
 #include <hip/hip_runtime.h>
 
@@ -30,9 +36,13 @@ constexpr uint32_t kPending = 0x40000000u;  // extension queued: low bits = task
 constexpr uint32_t kPosMask = 0x3FFFFFFFu;
 constexpr int kTaskCap = 128;
 constexpr int kDeferCap = 256;
+constexpr int kHitBuf = 256;                // hits of one step laid out in LDS
+constexpr int kSubjBuf = 2048;              // subject letters staged in LDS per wave
 
 struct Task {
-    int32_t q, s, s_left, slot;
+    uint32_t q_slot;  // query offset | diagonal slot << 16
+    int32_t s;        // subject offset of the second hit
+    int32_t s_left;   // end of the first hit's word
 };
 
 struct WaveState {
@@ -40,19 +50,31 @@ struct WaveState {
     uint32_t *claim;
     uint32_t *excl;
     uint32_t *p0;
+    uint32_t *cnt;  // [0] tasks, [1] deferred, [2] init hits
+    uint32_t *hits;
     Task *task;
     int2 *defer;
-    uint32_t *cnt;  // [0] tasks, [1] deferred, [2] init hits
+    uint8_t *subj;
 };
 
-__device__ __forceinline__ uint32_t lds_bytes_align(uint32_t b) { return (b + 15u) & ~15u; }
+__host__ __device__ constexpr uint32_t al16(uint32_t b) { return (b + 15u) & ~15u; }
+
+__host__ __device__ inline uint32_t wave_lds_bytes(uint32_t M, uint32_t C) {
+    return al16(4 * (M + C + 68 + 64 + 4 + kHitBuf) + sizeof(Task) * kTaskCap + sizeof(int2) * kDeferCap +
+                kSubjBuf);
+}
+
+__host__ __device__ inline uint32_t block_lds_bytes(int table, uint32_t qlen, uint32_t n_entries) {
+    uint32_t b = 16 + al16(28 * 28 * 2) + al16(qlen + 16);
+    if (table >= 1) b += al16(n_entries * 2 + 4);
+    if (table >= 2) b += al16((kWords + 1) * 2) + al16(kRelBlocks * 4);
+    return b;
+}
 
 struct Ctx {
     const int16_t (*mat)[28];
     const uint8_t *Q;
     int qlen;
-    const uint8_t *S;
-    int slen;
     uint32_t abs0;
     uint32_t ring_mask;
     uint32_t claim_mask;
@@ -63,7 +85,103 @@ struct Ctx {
     int stage_cap;
 };
 
-// Two-hit state machine for one hit (q, s) on a diagonal this lane owns for now.
+// Ungapped walks over letters held in one address space (LDS or global), four letters per
+// step: the loads of a step are independent, so one memory latency covers four positions.
+// Same stop rules as extend_left / extend_right.
+template <class SP>
+__device__ __forceinline__ int walk_left(const Ctx &c, SP S, int q_off, int s_off, int dropoff, int *length) {
+    const int n = s_off < q_off ? s_off : q_off;
+    int maxscore = 0, score = 0, best = -1;
+    for (int k0 = 0; k0 <= n; k0 += 4) {
+        int sc[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int k = k0 + j <= n ? k0 + j : n;
+            sc[j] = c.mat[c.Q[q_off - k]][S[s_off - k]];
+        }
+        bool stop = false;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (!stop && k0 + j <= n) {
+                score += sc[j];
+                if (score > maxscore) {
+                    maxscore = score;
+                    best = k0 + j;
+                }
+                if (maxscore - score >= dropoff) stop = true;
+            }
+        }
+        if (stop) break;
+    }
+    *length = best + 1;
+    return maxscore;
+}
+
+template <class SP>
+__device__ __forceinline__ int walk_right(const Ctx &c, SP S, int slen, int q_off, int s_off, int dropoff,
+                                          int maxscore, int *length, int *s_last_off) {
+    const int n = slen - s_off < c.qlen - q_off ? slen - s_off : c.qlen - q_off;
+    int score = maxscore, best = -1, i = n;
+    for (int k0 = 0; k0 < n; k0 += 4) {
+        int sc[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int k = k0 + j < n ? k0 + j : n - 1;
+            sc[j] = c.mat[c.Q[q_off + k]][S[s_off + k]];
+        }
+        bool stop = false;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (!stop && k0 + j < n) {
+                score += sc[j];
+                if (score > maxscore) {
+                    maxscore = score;
+                    best = k0 + j;
+                }
+                if (score <= 0 || maxscore - score >= dropoff) {
+                    stop = true;
+                    i = k0 + j;
+                }
+            }
+        }
+        if (stop) break;
+    }
+    *length = best + 1;
+    *s_last_off = s_off + i;
+    return maxscore;
+}
+
+// NCBI s_BlastAaExtendTwoHit (reference restatement gpuBLAZE.cu:1116-1154).
+template <class SP>
+__device__ __forceinline__ int two_hit_walk(const Ctx &c, SP S, int slen, int s_left_off, int s_right_off,
+                                            int q_right_off, int *hsp_q, int *hsp_s, int *hsp_len,
+                                            int *right_extend, int *s_last_off) {
+    int left_d = 0, right_d = 0, left_score = 0, right_score = 0, score = 0;
+#pragma unroll
+    for (int i = 0; i < kWord; i++) {
+        score += c.mat[c.Q[q_right_off + i]][S[s_right_off + i]];
+        if (score > left_score) {
+            left_score = score;
+            right_d = i + 1;
+        }
+    }
+    q_right_off += right_d;
+    s_right_off += right_d;
+    right_d = 0;
+    *right_extend = 0;
+    *s_last_off = s_right_off;
+    left_score = walk_left(c, S, q_right_off - 1, s_right_off - 1, c.xdrop, &left_d);
+    if (left_d >= s_right_off - s_left_off) {
+        *right_extend = 1;
+        right_score = walk_right(c, S, slen, q_right_off, s_right_off, c.xdrop, left_score, &right_d, s_last_off);
+    }
+    *hsp_q = q_right_off - left_d;
+    *hsp_s = s_right_off - left_d;
+    *hsp_len = left_d + right_d;
+    return left_score > right_score ? left_score : right_score;
+}
+
+// Two-hit state machine for one hit (q, s).
 __device__ __forceinline__ void apply_hit(const Ctx &c, WaveState &w, int q, int s) {
     const uint32_t slot = (uint32_t)(q - s) & c.ring_mask;
     const uint32_t st = w.ring[slot];
@@ -84,13 +202,12 @@ __device__ __forceinline__ void apply_hit(const Ctx &c, WaveState &w, int q, int
     }
     if (diff < (uint32_t)kWord) return;
     const uint32_t t = atomicAdd(&w.cnt[0], 1u);
-    w.task[t] = Task{q, s, (int)(st - 1u - c.abs0) + kWord, (int)slot};
+    w.task[t] = Task{(uint32_t)q | (slot << 16), s, (int)(st - 1u - c.abs0) + kWord};
     w.ring[slot] = kPending | t;
 }
 
 // Applies up to one hit per lane; same-diagonal hits go earliest-rank first.
-__device__ __forceinline__ void resolve_round(const Ctx &c, WaveState &w, bool pend, int q, int s,
-                                              uint32_t rank) {
+__device__ __forceinline__ void resolve_round(const Ctx &c, WaveState &w, bool pend, int q, int s, uint32_t rank) {
     const uint32_t h = (uint32_t)(q - s) & c.claim_mask;
     while (__ballot(pend)) {
         if (pend) atomicMin(&w.claim[h], rank);
@@ -108,7 +225,8 @@ __device__ __forceinline__ void resolve_round(const Ctx &c, WaveState &w, bool p
 
 // Extends every queued pair (one lane each), writes the outcome to its diagonal, then replays
 // the hits queued behind pending diagonals; repeats until nothing is pending.
-__device__ void flush(const Ctx &c, WaveState &w, unsigned long long &n_ext) {
+template <class SP>
+__device__ void flush(const Ctx &c, WaveState &w, SP S, int slen, unsigned long long &n_ext) {
     const int lane = lane_id();
     for (;;) {
         wave_sync();
@@ -119,15 +237,16 @@ __device__ void flush(const Ctx &c, WaveState &w, unsigned long long &n_ext) {
             const uint32_t t = t0 + lane;
             if (t < nt) {
                 const Task tk = w.task[t];
+                const int q = (int)(tk.q_slot & 0xFFFFu);
+                const uint32_t slot = tk.q_slot >> 16;
                 int hq, hs, hl, rext, slast;
-                const int score = two_hit(c.mat, c.Q, c.qlen, c.S, c.slen, tk.s_left, tk.s, tk.q, c.xdrop, &hq,
-                                          &hs, &hl, &rext, &slast);
+                const int score = two_hit_walk(c, S, slen, tk.s_left, tk.s, q, &hq, &hs, &hl, &rext, &slast);
                 n_ext++;
                 if (score >= c.cutoff) {
                     const uint32_t k = atomicAdd(&w.cnt[2], 1u);
                     if (k < (uint32_t)c.stage_cap) c.stage[k] = InitHit{hq, hs, hl, score};
                 }
-                w.ring[tk.slot] = rext ? ((c.abs0 + (uint32_t)slast - 2u + 1u) | kFlag) : (c.abs0 + (uint32_t)tk.s + 1u);
+                w.ring[slot] = rext ? ((c.abs0 + (uint32_t)slast - 2u + 1u) | kFlag) : (c.abs0 + (uint32_t)tk.s + 1u);
             }
         }
         wave_sync();
@@ -153,9 +272,80 @@ __device__ void flush(const Ctx &c, WaveState &w, unsigned long long &n_ext) {
     }
 }
 
+// Lookup tables of the current query, wherever kTable puts them.
+struct Tables {
+    const uint16_t *rel;  // LDS (kTable 2)
+    const uint32_t *bb;   // LDS (kTable 2)
+    const uint32_t *gprefix;
+    const uint16_t *ent;  // LDS (kTable >= 1) or global
+};
+
+template <int kTable>
+__device__ __forceinline__ void word_range(const Tables &t, uint32_t wd, uint32_t &pa, uint32_t &nh) {
+    if (kTable == 2) {
+        pa = t.bb[wd >> 6] + t.rel[wd];
+        nh = t.bb[(wd + 1) >> 6] + t.rel[wd + 1] - pa;
+    } else {
+        pa = t.gprefix[wd];
+        nh = t.gprefix[wd + 1] - pa;
+    }
+}
+
+// Seed scan of one subject whose letters are read through S (LDS buffer or global memory).
+template <int kTable, class SP>
+__device__ void scan_subject(const Ctx &c, WaveState &w, const Tables &t, SP S, int slen, unsigned long long &n_hits,
+                             unsigned long long &n_ext) {
+    const int lane = lane_id();
+    for (int c0 = 0; c0 + kWord <= slen; c0 += 64) {
+        const int s = c0 + lane;
+        uint32_t pa = 0, nh = 0;
+        if (s + kWord <= slen) word_range<kTable>(t, S[s] * 784u + S[s + 1] * 28u + S[s + 2], pa, nh);
+        const uint32_t inc = wave_inclusive_sum(nh);
+        const uint32_t H = __shfl(inc, 63, 64);
+        const uint32_t ex = inc - nh;
+        n_hits += nh;
+        if (H <= (uint32_t)kHitBuf) {
+            // lay the hits out: entry = query offset << 6 | lane of the subject position
+            for (uint32_t k = 0; k < nh; k++) w.hits[ex + k] = ((uint32_t)t.ent[pa + k] << 6) | (uint32_t)lane;
+            wave_sync();
+            for (uint32_t r = 0; r < H; r += 64) {
+                if (w.cnt[0] > (uint32_t)(kTaskCap - 64) || w.cnt[1] > 64u) flush(c, w, S, slen, n_ext);
+                const uint32_t g = r + lane;
+                const bool pend = g < H;
+                const uint32_t e = pend ? w.hits[g] : 0u;
+                resolve_round(c, w, pend, (int)(e >> 6), c0 + (int)(e & 63u), g);
+            }
+        } else {
+            // dense step (low-complexity letters): find each hit's position by binary search
+            w.excl[lane] = ex;
+            w.p0[lane] = pa;
+            wave_sync();
+            for (uint32_t r = 0; r < H; r += 64) {
+                if (w.cnt[0] > (uint32_t)(kTaskCap - 64) || w.cnt[1] > 64u) flush(c, w, S, slen, n_ext);
+                const uint32_t g = r + lane;
+                const bool pend = g < H;
+                int src = 0, qo = 0;
+                if (pend) {
+                    int lo = 0, hi = 63;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (w.excl[mid] <= g) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    src = lo;
+                    qo = t.ent[w.p0[src] + (g - w.excl[src])];
+                }
+                resolve_round(c, w, pend, qo, c0 + src, g);
+            }
+        }
+        wave_sync();
+    }
+    flush(c, w, S, slen, n_ext);
+}
+
 }  // namespace
 
-template <bool kLdsTable>
+template <int kTable>
 __global__ void __launch_bounds__(1024) k_seed2(SeedArgs a) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int lane = lane_id();
@@ -166,36 +356,37 @@ __global__ void __launch_bounds__(1024) k_seed2(SeedArgs a) {
     const uint32_t M = a.ring_mask + 1;
     const uint32_t C = a.claim_mask + 1;
 
-    // block-shared section
     uint32_t off = 0;
-    int32_t *blk = reinterpret_cast<int32_t *>(smem + off);  // [0] item, [1] next subject of the item
+    int32_t *blk = reinterpret_cast<int32_t *>(smem);  // [0] item, [1] next subject of the item
     off += 16;
     int16_t(*mat)[28] = reinterpret_cast<int16_t(*)[28]>(smem + off);
-    off += lds_bytes_align(28 * 28 * 2);
+    off += al16(28 * 28 * 2);
     uint8_t *Qs = smem + off;
-    off += lds_bytes_align((uint32_t)a.qlen + 16);
+    off += al16((uint32_t)a.qlen + 16);
+    uint16_t *ent_l = nullptr;
     uint16_t *rel = nullptr;
     uint32_t *bb = nullptr;
-    uint16_t *ent = nullptr;
-    if (kLdsTable) {
-        rel = reinterpret_cast<uint16_t *>(smem + off);
-        off += lds_bytes_align((kWords + 1) * 2);
-        bb = reinterpret_cast<uint32_t *>(smem + off);
-        off += lds_bytes_align(kRelBlocks * 4);
-        ent = reinterpret_cast<uint16_t *>(smem + off);
-        off += lds_bytes_align(a.n_entries * 2 + 4);
+    if (kTable >= 1) {
+        ent_l = reinterpret_cast<uint16_t *>(smem + off);
+        off += al16(a.n_entries * 2 + 4);
     }
-    // per-wave section
-    const uint32_t wave_bytes = 4 * (M + C + 68 + 64 + 4) + sizeof(Task) * kTaskCap + sizeof(int2) * kDeferCap;
-    uint8_t *wb = smem + off + (size_t)wave * lds_bytes_align(wave_bytes);
+    if (kTable >= 2) {
+        rel = reinterpret_cast<uint16_t *>(smem + off);
+        off += al16((kWords + 1) * 2);
+        bb = reinterpret_cast<uint32_t *>(smem + off);
+        off += al16(kRelBlocks * 4);
+    }
+    uint8_t *wb = smem + off + (size_t)wave * wave_lds_bytes(M, C);
     WaveState w;
     w.ring = reinterpret_cast<uint32_t *>(wb);
     w.claim = w.ring + M;
     w.excl = w.claim + C;
     w.p0 = w.excl + 68;
     w.cnt = w.p0 + 64;
-    w.task = reinterpret_cast<Task *>(w.cnt + 4);
+    w.hits = w.cnt + 4;
+    w.task = reinterpret_cast<Task *>(w.hits + kHitBuf);
     w.defer = reinterpret_cast<int2 *>(w.task + kTaskCap);
+    w.subj = reinterpret_cast<uint8_t *>(w.defer + kDeferCap);
 
     for (int i = threadIdx.x; i < 28 * 28; i += blockDim.x) mat[i / 28][i % 28] = kBlosum62Dev[i / 28][i % 28];
     for (uint32_t i = lane; i < M; i += 64) w.ring[i] = 0;
@@ -212,9 +403,10 @@ __global__ void __launch_bounds__(1024) k_seed2(SeedArgs a) {
     unsigned long long n_hits = 0, n_ext = 0;
     int cur_q = -1;
     const uint32_t n_items = (uint32_t)a.nq * a.items_per_query;
+    Tables t{};
 
     for (;;) {
-        __syncthreads();  // the previous item (and its query tables) is finished by every wave
+        __syncthreads();  // every wave is done with the previous item (and its query tables)
         if (threadIdx.x == 0) {
             blk[0] = (int32_t)atomicAdd(a.item_ctr, 1u);
             blk[1] = 0;
@@ -226,14 +418,16 @@ __global__ void __launch_bounds__(1024) k_seed2(SeedArgs a) {
         const int64_t sub0 = a.s_begin + (int64_t)(item % a.items_per_query) * a.subj_per_item;
         const int64_t sub1 = sub0 + a.subj_per_item < a.s_end ? sub0 + a.subj_per_item : a.s_end;
         const QueryDev &qd = a.qd[qi];
-        if (qi != cur_q) {  // stage this query's tables
+        if (qi != cur_q) {  // stage this query
             for (int i = threadIdx.x; i < qd.qlen; i += blockDim.x) Qs[i] = qd.query[i];
-            if (kLdsTable) {
+            if (kTable >= 1) {
+                const uint32_t *src = reinterpret_cast<const uint32_t *>(qd.entries);
+                uint32_t *dst = reinterpret_cast<uint32_t *>(ent_l);
+                for (uint32_t i = threadIdx.x; i < (qd.n_entries + 1) / 2; i += blockDim.x) dst[i] = src[i];
+            }
+            if (kTable >= 2) {
                 for (int i = threadIdx.x; i <= kWords; i += blockDim.x) rel[i] = qd.rel16[i];
                 for (int i = threadIdx.x; i < kRelBlocks; i += blockDim.x) bb[i] = qd.blockbase[i];
-                const uint32_t *src = reinterpret_cast<const uint32_t *>(qd.entries);
-                uint32_t *dst = reinterpret_cast<uint32_t *>(ent);
-                for (uint32_t i = threadIdx.x; i < (qd.n_entries + 1) / 2; i += blockDim.x) dst[i] = src[i];
             }
             __syncthreads();
             cur_q = qi;
@@ -244,8 +438,10 @@ __global__ void __launch_bounds__(1024) k_seed2(SeedArgs a) {
         c.window = qd.window;
         c.xdrop = qd.x_ungapped;
         c.cutoff = qd.ungapped_cutoff;
-        const uint32_t *gprefix = qd.prefix;
-        const uint16_t *gent = qd.entries;
+        t.rel = rel;
+        t.bb = bb;
+        t.gprefix = qd.prefix;
+        t.ent = kTable >= 1 ? ent_l : qd.entries;
 
         for (;;) {
             int64_t sid = 0;
@@ -254,51 +450,29 @@ __global__ void __launch_bounds__(1024) k_seed2(SeedArgs a) {
             if (sid >= sub1) break;
             const uint64_t o0 = a.offs[sid];
             const int slen = (int)(a.offs[sid + 1] - o0);
-            c.S = a.res + o0;
-            c.slen = slen;
+            const uint8_t *Sg = a.res + o0;
             c.abs0 = abs_base;
             abs_base += (uint32_t)slen + 128u;
-
-            for (int c0 = 0; c0 + kWord <= slen; c0 += 64) {
-                const int s = c0 + lane;
-                uint32_t pa = 0, nh = 0;
-                if (s + kWord <= slen) {
-                    const uint32_t wd = c.S[s] * 784u + c.S[s + 1] * 28u + c.S[s + 2];
-                    if (kLdsTable) {
-                        pa = bb[wd >> 6] + rel[wd];
-                        nh = bb[(wd + 1) >> 6] + rel[wd + 1] - pa;
-                    } else {
-                        pa = gprefix[wd];
-                        nh = gprefix[wd + 1] - pa;
+            if (slen <= kSubjBuf) {
+                // stage the subject: 16-byte loads from a 16-byte aligned start, then realign
+                const uint64_t a0 = o0 & ~uint64_t(15);
+                const int lead = (int)(o0 - a0);
+                const int nvec = (lead + slen + 15) >> 4;
+                const uint4 *src = reinterpret_cast<const uint4 *>(a.res + a0);
+                for (int v = lane; v < nvec; v += 64) {
+                    const uint4 x = src[v];
+                    const uint32_t xw[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                    for (int k = 0; k < 16; k++) {
+                        const int p = v * 16 + k - lead;
+                        if (p >= 0 && p < slen) w.subj[p] = (uint8_t)(xw[k >> 2] >> (8 * (k & 3)));
                     }
                 }
-                const uint32_t inc = wave_inclusive_sum(nh);
-                const uint32_t H = __shfl(inc, 63, 64);
-                w.excl[lane] = inc - nh;
-                w.p0[lane] = pa;
                 wave_sync();
-                n_hits += nh;
-                for (uint32_t r = 0; r < H; r += 64) {
-                    if (w.cnt[0] > (uint32_t)(kTaskCap - 64) || w.cnt[1] > 64u) flush(c, w, n_ext);
-                    const uint32_t g = r + lane;
-                    const bool pend = g < H;
-                    int src = 0, qo = 0;
-                    if (pend) {
-                        int lo = 0, hi = 63;
-                        while (lo < hi) {
-                            const int mid = (lo + hi + 1) >> 1;
-                            if (w.excl[mid] <= g) lo = mid;
-                            else hi = mid - 1;
-                        }
-                        src = lo;
-                        const uint32_t e = w.p0[src] + (g - w.excl[src]);
-                        qo = kLdsTable ? ent[e] : gent[e];
-                    }
-                    resolve_round(c, w, pend, qo, c0 + src, g);
-                }
-                wave_sync();
+                scan_subject<kTable>(c, w, t, (const uint8_t *)w.subj, slen, n_hits, n_ext);
+            } else {
+                scan_subject<kTable>(c, w, t, Sg, slen, n_hits, n_ext);
             }
-            flush(c, w, n_ext);
 
             // hand the subject's init hits to the gapped stage
             wave_sync();
@@ -321,11 +495,28 @@ __global__ void __launch_bounds__(1024) k_seed2(SeedArgs a) {
                             if (k < a.ovf_cap) a.ovf[k] = tag;
                         }
                     } else {
-                        for (uint32_t k = lane; k < nih; k += 64) a.pool[base + k] = c.stage[k];
+                        int best = 0;
+                        for (uint32_t k = lane; k < nih; k += 64) {
+                            const InitHit h = c.stage[k];
+                            a.pool[base + k] = h;
+                            best = h.score > best ? h.score : best;
+                        }
+#pragma unroll
+                        for (int o = 32; o > 0; o >>= 1) {
+                            const int tt = __shfl_xor(best, o, 64);
+                            best = tt > best ? tt : best;
+                        }
                         if (lane == 0) {
                             const uint32_t wi = atomicAdd(&a.ctr->work_count, 1u);
-                            a.work[wi] = WorkItem{sid, (uint64_t)base, (int32_t)nih, a.qi_base + qi};
-                            atomicAdd(&a.ctr->init_hits, (unsigned long long)nih);
+                            if (wi < a.work_cap) {
+                                const int bucket = cost_bucket(best);
+                                a.work[wi] = WorkItem{sid, (uint64_t)base, (int32_t)nih, a.qi_base + qi, bucket, 0};
+                                atomicAdd(&a.ctr->bucket_count[bucket], 1u);
+                                atomicAdd(&a.ctr->init_hits, (unsigned long long)nih);
+                            } else {
+                                const uint32_t k = atomicAdd(&a.ctr->overflow_count, 1u);
+                                if (k < a.ovf_cap) a.ovf[k] = tag;
+                            }
                         }
                     }
                 }
@@ -347,21 +538,19 @@ __global__ void __launch_bounds__(1024) k_seed2(SeedArgs a) {
     }
 }
 
-size_t seed2_lds_bytes(const SeedArgs &a, bool lds_table, int waves) {
-    auto al = [](size_t b) { return (b + 15) & ~size_t(15); };
-    size_t b = 16 + al(28 * 28 * 2) + al((size_t)a.qlen + 16);
-    if (lds_table) b += al((kWords + 1) * 2) + al(kRelBlocks * 4) + al((size_t)a.n_entries * 2 + 4);
-    const size_t wave_bytes = 4 * ((size_t)a.ring_mask + 1 + a.claim_mask + 1 + 68 + 64 + 4) +
-                              sizeof(Task) * kTaskCap + sizeof(int2) * kDeferCap;
-    return b + (size_t)waves * al(wave_bytes);
+size_t seed2_lds_bytes(const SeedArgs &a, int table, int waves) {
+    return block_lds_bytes(table, (uint32_t)a.qlen, a.n_entries) +
+           (size_t)waves * wave_lds_bytes(a.ring_mask + 1, a.claim_mask + 1);
 }
 
-hipError_t launch_seed2(const SeedArgs &a, bool lds_table, int blocks, int waves, hipStream_t st) {
-    const size_t lds = seed2_lds_bytes(a, lds_table, waves);
-    if (lds_table)
-        hipLaunchKernelGGL(k_seed2<true>, dim3(blocks), dim3(64 * waves), lds, st, a);
+hipError_t launch_seed2(const SeedArgs &a, int table, int blocks, int waves, hipStream_t st) {
+    const size_t lds = seed2_lds_bytes(a, table, waves);
+    if (table == 2)
+        hipLaunchKernelGGL(k_seed2<2>, dim3(blocks), dim3(64 * waves), lds, st, a);
+    else if (table == 1)
+        hipLaunchKernelGGL(k_seed2<1>, dim3(blocks), dim3(64 * waves), lds, st, a);
     else
-        hipLaunchKernelGGL(k_seed2<false>, dim3(blocks), dim3(64 * waves), lds, st, a);
+        hipLaunchKernelGGL(k_seed2<0>, dim3(blocks), dim3(64 * waves), lds, st, a);
     return hipGetLastError();
 }
 

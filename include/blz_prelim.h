@@ -117,8 +117,9 @@ typedef struct {
     int32_t seed_waves;        /* waves per workgroup of the seed kernel */
     int32_t seed_blocks_per_cu;/* resident workgroups per CU the seed launch is sized for */
     int32_t gapped_blocks;     /* workgroups of the gapped kernel */
-    int32_t gapped_threads;    /* threads per workgroup of the gapped kernel */
-    int32_t reserved[3];
+    int32_t gapped_threads;    /* threads per workgroup of the wide gapped kernel */
+    int32_t seed_table;        /* lookup tables: 1 global, 2 entries in LDS, 3 all in LDS (0 auto) */
+    int32_t reserved[2];
 } blz_tuning;
 
 blz_status blz_ctx_set_tuning(blz_ctx *ctx, const blz_tuning *t);

@@ -217,9 +217,11 @@ def test_semi_gapped_no_xdrop_equals_gotoh(reverse):
         assert (sc, a, b) == want
 
 
-def clean_xdrop(A, B, x, go=11, ge=1):
+def clean_xdrop(A, B, x, go=11, ge=1, dead_gap_to_neg=False):
     """Independent formulation of the banded X-drop: each row computed as vectors with a prefix
-    max for the horizontal gaps and for the running best, band bounds from the surviving cells."""
+    max for the horizontal gaps and for the running best, band bounds from the surviving cells.
+    dead_gap_to_neg also drops the vertical-gap score of a cell that fails the X-drop (the
+    16-bit LDS storage of the GPU kernel does this)."""
     goe = go + ge
     NEG = -(1 << 30)
     m, n = len(A), len(B)
@@ -262,6 +264,8 @@ def clean_xdrop(A, B, x, go=11, ge=1):
                 newE[b] = int(max(Hrow[k] - goe, E[k] - ge))
             elif b > row[k_first]:
                 newH[b] = NEG
+                if dead_gap_to_neg:
+                    newE[b] = NEG
         gap_end = max(run, NEG)
         H, Eg = newH, newE
         first = int(row[k_first])
@@ -294,6 +298,22 @@ def test_semi_gapped_matches_clean_formulation():
         sc, a, b, _ = oracle.semi_gapped(A, B, x, reverse=True, M=len(A), N=len(B))
         want = clean_xdrop(A[::-1], B[::-1], x)
         assert (sc, a, b) == want, trial
+
+
+def test_semi_gapped_dead_gap_scores_do_not_matter():
+    """Dropping the gap score of cells that failed the X-drop leaves score and end point unchanged:
+    such a score is below best - X and stays there, so it can decide no later cell."""
+    rng = np.random.default_rng(14)
+    for trial in range(150):
+        m = int(rng.integers(5, 80))
+        A = rand_seq(rng, m)
+        B = mutate(rng, A, float(rng.uniform(0.2, 0.9)))
+        if trial % 2 == 0:
+            k = int(rng.integers(1, m))
+            B = np.concatenate([B[:k], rand_seq(rng, int(rng.integers(1, 9))), B[k:]])
+        x = int(rng.choice([12, 20, 38, 60]))
+        sc, a, b, _ = oracle.semi_gapped(A, B, x, reverse=True, M=len(A), N=len(B))
+        assert (sc, a, b) == clean_xdrop(A[::-1], B[::-1], x, dead_gap_to_neg=True), trial
 
 
 def test_semi_gapped_xdrop_bounds():
