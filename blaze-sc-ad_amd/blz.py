@@ -55,7 +55,7 @@ class RunStats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("seed_variant", C.c_int32), ("seed_waves", C.c_int32), ("seed_blocks_per_cu", C.c_int32),
                 ("gapped_blocks", C.c_int32), ("gapped_threads", C.c_int32), ("seed_table", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("gapped_wave_classes", C.c_int32), ("query_packing", C.c_int32)]
 
 
 class Sink(C.Structure):
@@ -240,7 +240,7 @@ class Engine:
     def get_tuning(self) -> dict:
         t = Tuning()
         self.lib.blz_ctx_get_tuning(self.ctx, C.byref(t))
-        return {k: getattr(t, k) for k, _ in Tuning._fields_ if k != "reserved"}
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
 
     @property
     def stream(self) -> int:

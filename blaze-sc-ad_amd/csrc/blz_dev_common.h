@@ -19,14 +19,43 @@ __device__ __forceinline__ void wave_sync() {
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 
+// Inclusive prefix sum over the 64 lanes with DPP: row_shr 1/2/4/8 inside each 16-lane row,
+// then row_bcast:15 (rows 1 and 3 add the last lane of rows 0 and 2) and row_bcast:31 (rows 2
+// and 3 add lane 31).  Lanes whose DPP source is outside the row read 0.
 __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
-    const int lane = lane_id();
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t t = __shfl_up(v, o, 64);
-        if (lane >= o) v += t;
-    }
-    return v;
+    int x = (int)v;
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);
+    return (uint32_t)x;
+}
+
+// Inclusive prefix max over the 64 lanes with DPP (same lane pattern as wave_inclusive_sum);
+// lanes whose source is outside the row read INT_MIN.
+__device__ __forceinline__ int wave_inclusive_max(int x) {
+    constexpr int kMin = -2147483647 - 1;
+    x = max(x, __builtin_amdgcn_update_dpp(kMin, x, 0x111, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(kMin, x, 0x112, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(kMin, x, 0x114, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(kMin, x, 0x118, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(kMin, x, 0x142, 0xa, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(kMin, x, 0x143, 0xc, 0xf, false));
+    return x;
+}
+
+// Each lane receives the value of the lane below it (DPP wave_shr:1); lane 0 receives `in`.
+__device__ __forceinline__ int wave_shift_up1(int x, int in) {
+    return __builtin_amdgcn_update_dpp(in, x, 0x138, 0xf, 0xf, false);
+}
+
+// value of lane 63 / lane 0, read into a scalar register
+__device__ __forceinline__ uint32_t last_lane(uint32_t v) { return (uint32_t)__builtin_amdgcn_readlane((int)v, 63); }
+__device__ __forceinline__ uint32_t first_lane(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ uint64_t first_lane64(uint64_t v) {
+    return ((uint64_t)first_lane((uint32_t)(v >> 32)) << 32) | first_lane((uint32_t)v);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -18,7 +18,14 @@ struct QueryDev {
     uint32_t ring_mask, claim_mask;
     int32_t window, x_ungapped, ungapped_cutoff;
     int32_t x_gapped, gap_open, gap_extend, gapped_cutoff;
+    // a pack of queries scanned together (seed stage): query offsets of the concatenation
+    // belong to member m when member_start[m] <= q < member_start[m+1]; member_idx[m] is the
+    // member's index in the run's query table.  A single query is a pack of one.
+    const uint32_t *member_start;
+    const uint32_t *member_idx;
+    int32_t n_members;
 };
+constexpr int kMaxMembers = 32;
 
 struct SeedArgs {
     const uint8_t *res;        // concatenated subject residues (device)
@@ -45,8 +52,7 @@ struct SeedArgs {
     DevCounters *ctr;
     // batched form (seed kernel 2): items are (query, block of subjects), query-major
     const QueryDev *qd;
-    int32_t nq;
-    int32_t qi_base;           // index of qd[0] in the run's query table
+    int32_t nq;                // packs of this launch
     uint32_t *item_ctr;        // this launch's work-item counter
     uint32_t items_per_query;
     int64_t subj_per_item;
@@ -61,7 +67,9 @@ struct GappedArgs {
     const uint32_t *order;     // work indices to process (cost order, or the redo list)
     const uint32_t *count;     // how many of them (device counter) ...
     uint32_t count_cap;        // ... at most this many
-    uint32_t *next;            // work-stealing cursor over order[]
+    uint32_t *next;            // work-stealing cursor over order[] (lane-serial passes)
+    uint32_t *next_heavy;      // work-stealing cursor of the wave pass
+    int32_t heavy_classes;     // cost classes [0, heavy_classes) go to the wave pass
     uint32_t *redo;            // work indices whose DP band outgrew the LDS ring
     uint32_t redo_cap;
     int2 *dp_scratch;          // wide variant: per-lane DP ring, dp_wmask+1 entries per lane
@@ -88,6 +96,9 @@ hipError_t launch_order(const WorkItem *work, DevCounters *ctr, uint32_t *order,
 hipError_t launch_gapped_lds(const GappedArgs &a, int blocks, hipStream_t st);
 // gapped stage with the DP band in global memory (wide rings), for the redo list
 hipError_t launch_gapped_wide(const GappedArgs &a, int blocks, int threads, hipStream_t st);
-constexpr int kLdsRing = 128;  // LDS DP ring entries per lane
+constexpr int kLdsRing = 128;   // LDS DP ring entries per lane (lane-serial pass)
+constexpr int kWaveRing = 1024; // LDS DP ring entries per wave (wave pass)
+// gapped stage with one wave per subject of the heavy cost classes
+hipError_t launch_gapped_wave(const GappedArgs &a, int blocks, hipStream_t st);
 
 }  // namespace blz

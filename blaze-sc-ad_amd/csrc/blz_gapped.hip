@@ -239,23 +239,143 @@ __device__ void insertion_sort(Hsp *h, int n) {
 
 enum ItemOutcome { kDone = 0, kRedo = 1, kOverflow = 2 };  // kRedo: band or HSP count past this pass's caps
 
-// Whole gapped stage of one subject.  Returns kRedo if the DP band outgrew the ring (nothing
-// was written), kOverflow if a cap was hit.
-template <class R>
+// Wave-parallel semi-gapped X-drop DP for one extension (all 64 lanes call it with the same
+// arguments).  Rows are swept in order; the cells of a row's band are spread over the lanes, 64
+// per step.  Within a row: D(b) = H_prev(b-1) + s(A_a, B_b) (none for the band's first cell),
+// E(b) = the column gap of the previous row, the row gap F(b) = max over b' < b of
+// max(D, E)(b') - open - extend * (b - b') is a prefix max, H = max(D, E, F), and a cell survives
+// when (best before it in row order) - H <= X, the running best being a prefix max too.  The band
+// bounds and the gap run appended past a surviving last cell follow the same rules as the
+// lane-serial loop; the row-vector form is pinned against NCBI's loop in tests/test_oracle.py.
+// Cells that fail the X-drop store minus infinity for both scores (see LdsRing16).
+__device__ int wave_semi_gapped(const int16_t (*mat)[28], const uint8_t *A, const uint8_t *B, int M, int N,
+                                int x_dropoff, int gap_open, int gap_extend, bool reverse, int *a_off, int *b_off,
+                                int2 *ring, int wmask, bool *ovf, unsigned long long *cells) {
+    const int lane = lane_id();
+    const int goe = gap_open + gap_extend;
+    const int ge = gap_extend;
+    *a_off = 0;
+    *b_off = 0;
+    if (x_dropoff < goe) x_dropoff = goe;
+    const int X = x_dropoff;
+    if (N <= 0 || M <= 0) return 0;
+    const int W = wmask + 1;
+    // row 0: a run of gaps in A while it stays within the X-drop
+    const int k0 = (X - goe) / ge + 1;
+    int size = (N < k0 ? N : k0) + 1;
+    if (size > W) {
+        *ovf = true;
+        return 0;
+    }
+    for (int b = lane; b < size; b += 64) {
+        const int h = b == 0 ? 0 : -goe - (b - 1) * ge;
+        ring[b & wmask] = make_int2(h, h - goe);
+    }
+    wave_sync();
+    int best = 0, first = 0, ba = 0, bb = 0;
+    unsigned long long nc = 0;
+    for (int a = 1; a <= M; a++) {
+        const int16_t *mrow = mat[reverse ? A[M - a] : A[a]];
+        int carry_g = kMinIntDp;   // max of max(D,E)(b') + b' * extend over earlier steps
+        int carry_h = best;        // running best over earlier steps
+        int carry_hp = kMinIntDp;  // previous row's H left of this step (none for the first cell)
+        int fa = -1, la = -1, rmax = kMinIntDp, rmax_b = -1;
+        for (int c0 = first; c0 < size; c0 += 64) {
+            const int b = c0 + lane;
+            const bool valid = b < size;
+            const int2 prev = valid ? ring[b & wmask] : make_int2(kMinIntDp, kMinIntDp);
+            const int hp_left = wave_shift_up1(prev.x, carry_hp);
+            const int sb = (valid && b >= 1 && b <= N) ? mrow[reverse ? B[N - b] : B[b]] : 0;
+            const int de = valid ? max(hp_left + sb, prev.y) : kMinIntDp;
+            const int g = valid ? de + b * ge : kMinIntDp;
+            const int gin = wave_inclusive_max(g);
+            const int gex = max(wave_shift_up1(gin, kMinIntDp), carry_g);
+            const int f = gex - goe - (b - 1) * ge;
+            const int h = valid ? max(de, f) : kMinIntDp;
+            const int hin = wave_inclusive_max(h);
+            const int hex = max(wave_shift_up1(hin, kMinIntDp), carry_h);
+            const bool alive = valid && hex - h <= X;
+            const uint64_t am = __ballot(alive);
+            if (am) {
+                if (fa < 0) fa = c0 + (int)__builtin_ctzll(am);
+                la = c0 + 63 - (int)__builtin_clzll(am);
+                const int cmax = (int)last_lane((uint32_t)wave_inclusive_max(alive ? h : kMinIntDp));
+                if (cmax > rmax) {
+                    rmax = cmax;
+                    rmax_b = c0 + (int)__builtin_ctzll(__ballot(alive && h == cmax));
+                }
+            }
+            if (valid)
+                ring[b & wmask] = alive ? make_int2(h, max(h - goe, prev.y - ge)) : make_int2(kMinIntDp, kMinIntDp);
+            carry_g = max(carry_g, (int)last_lane((uint32_t)gin));
+            carry_h = max(carry_h, (int)last_lane((uint32_t)hin));
+            carry_hp = (int)last_lane((uint32_t)prev.x);
+        }
+        nc += (unsigned long long)(size - first);
+        if (fa < 0) break;
+        if (rmax > best) {
+            best = rmax;
+            ba = a;
+            bb = rmax_b;
+        }
+        first = fa;
+        if (la < size - 1) {
+            size = la + 1;
+        } else {
+            const int gr = carry_g - goe - (size - 1) * ge;  // row gap one past the last cell
+            if (gr >= best - X) {
+                int k = (gr - (best - X)) / ge + 1;
+                if (k > N + 1 - size) k = N + 1 - size;
+                if (size + k - first > W) {
+                    *ovf = true;
+                    return 0;
+                }
+                for (int i = lane; i < k; i += 64) {
+                    const int hv = gr - i * ge;
+                    ring[(size + i) & wmask] = make_int2(hv, hv - goe);
+                }
+                size += k;
+            }
+        }
+        if (size <= N) {
+            if (size + 1 - first > W) {
+                *ovf = true;
+                return 0;
+            }
+            if (lane == 0) ring[size & wmask] = make_int2(kMinIntDp, kMinIntDp);
+            size++;
+        }
+        wave_sync();
+    }
+    *a_off = ba;
+    *b_off = bb;
+    *cells += nc;
+    return best;
+}
+
+// Whole gapped stage of one subject, by one lane (kWave false) or by a whole wave (kWave true:
+// every lane runs the same control flow, the DP is shared across the lanes and lane 0 does the
+// sorting and the writes).  Returns kRedo if the DP band or the HSP count outgrew this pass's
+// storage (nothing was written), kOverflow if an output pool is full.
+template <bool kWave, class R>
 __device__ int gapped_item(const GappedArgs &a, const int16_t (*mat)[28], uint32_t widx, R &ring, Hsp *saved,
                            unsigned long long &cells, unsigned long long &gext) {
+    const bool writer = !kWave || lane_id() == 0;
     const WorkItem wi = a.work[widx];
     InitHit *ih = a.pool + wi.off;
     const int n = wi.n;
-    for (int i = 1; i < n; i++) {  // score order of the init hits (sorted in place once)
-        InitHit x = ih[i];
-        int j = i - 1;
-        while (j >= 0 && init_before(x, ih[j])) {
-            ih[j + 1] = ih[j];
-            j--;
+    if (writer) {
+        for (int i = 1; i < n; i++) {  // score order of the init hits (sorted in place once)
+            InitHit x = ih[i];
+            int j = i - 1;
+            while (j >= 0 && init_before(x, ih[j])) {
+                ih[j + 1] = ih[j];
+                j--;
+            }
+            ih[j + 1] = x;
         }
-        ih[j + 1] = x;
     }
+    if (kWave) wave_sync();
     const uint64_t o0 = a.offs[wi.sid];
     const int slen = (int)(a.offs[wi.sid + 1] - o0);
     const uint8_t *S = a.res + o0;
@@ -278,45 +398,65 @@ __device__ int gapped_item(const GappedArgs &a, const int16_t (*mat)[28], uint32
         if (skip) continue;
         const int q_gs = gapped_start(mat, Q, S, h.q_start, h.s_start, h.len);
         const int s_gs = h.s_start + (q_gs - h.q_start);
-        int la, lb, ra, rb;
-        const int left = semi_gapped(mat, Q, S, q_gs + 1, s_gs + 1, qd.x_gapped, qd.gap_open, qd.gap_extend, true,
-                                     &la, &lb, ring, &ovf, &my_cells);
-        if (ovf) break;
-        const int right = semi_gapped(mat, Q + q_gs, S + s_gs, qlen - q_gs - 1, slen - s_gs - 1, qd.x_gapped,
-                                      qd.gap_open, qd.gap_extend, false, &ra, &rb, ring, &ovf, &my_cells);
+        int la, lb, ra, rb, left, right;
+        if constexpr (kWave) {
+            left = wave_semi_gapped(mat, Q, S, q_gs + 1, s_gs + 1, qd.x_gapped, qd.gap_open, qd.gap_extend, true, &la,
+                                    &lb, ring.ring, ring.wmask, &ovf, &my_cells);
+            if (ovf) break;
+            right = wave_semi_gapped(mat, Q + q_gs, S + s_gs, qlen - q_gs - 1, slen - s_gs - 1, qd.x_gapped,
+                                     qd.gap_open, qd.gap_extend, false, &ra, &rb, ring.ring, ring.wmask, &ovf,
+                                     &my_cells);
+        } else {
+            left = semi_gapped(mat, Q, S, q_gs + 1, s_gs + 1, qd.x_gapped, qd.gap_open, qd.gap_extend, true, &la,
+                               &lb, ring, &ovf, &my_cells);
+            if (ovf) break;
+            right = semi_gapped(mat, Q + q_gs, S + s_gs, qlen - q_gs - 1, slen - s_gs - 1, qd.x_gapped, qd.gap_open,
+                                qd.gap_extend, false, &ra, &rb, ring, &ovf, &my_cells);
+        }
         if (ovf) break;
         my_ext++;
         const int score = left + right;
         if (score >= qd.gapped_cutoff) {
             if (ns >= a.saved_cap) return kRedo;  // more HSPs than this pass can hold
-            saved[ns++] = Hsp{score, q_gs + 1 - la, q_gs + 1 + ra, s_gs + 1 - lb, s_gs + 1 + rb, q_gs, s_gs};
+            if (writer) saved[ns] = Hsp{score, q_gs + 1 - la, q_gs + 1 + ra, s_gs + 1 - lb, s_gs + 1 + rb, q_gs, s_gs};
+            ns++;
+            if (kWave) wave_sync();
         }
     }
     if (ovf) return kRedo;  // counters are only taken from the pass that completes the subject
-    cells += my_cells;
-    gext += my_ext;
+    if (writer) {
+        cells += my_cells;
+        gext += my_ext;
+    }
     if (ns == 0) return kDone;
-    insertion_sort<cmp_start>(saved, ns);
-    int m = 0;
-    for (int i = 0; i < ns; i++) {
-        if (m > 0 && saved[m - 1].q_off == saved[i].q_off && saved[m - 1].s_off == saved[i].s_off) continue;
-        saved[m++] = saved[i];
+    int outcome = kDone;
+    if (writer) {
+        insertion_sort<cmp_start>(saved, ns);
+        int m = 0;
+        for (int i = 0; i < ns; i++) {
+            if (m > 0 && saved[m - 1].q_off == saved[i].q_off && saved[m - 1].s_off == saved[i].s_off) continue;
+            saved[m++] = saved[i];
+        }
+        ns = m;
+        insertion_sort<cmp_end>(saved, ns);
+        m = 0;
+        for (int i = 0; i < ns; i++) {
+            if (m > 0 && saved[m - 1].q_end == saved[i].q_end && saved[m - 1].s_end == saved[i].s_end) continue;
+            saved[m++] = saved[i];
+        }
+        ns = m;
+        insertion_sort<cmp_score>(saved, ns);
+        const unsigned long long base = atomicAdd(&a.ctr->hsp_used, (unsigned long long)ns);
+        if (base + ns > a.hsp_cap) {
+            outcome = kOverflow;
+        } else {
+            for (int i = 0; i < ns; i++) a.hsps[base + i] = saved[i];
+            const uint32_t ri = atomicAdd(&a.ctr->result_count, 1u);
+            a.results[ri] = ResultItem{wi.sid, (uint64_t)base, ns, wi.qi};
+        }
     }
-    ns = m;
-    insertion_sort<cmp_end>(saved, ns);
-    m = 0;
-    for (int i = 0; i < ns; i++) {
-        if (m > 0 && saved[m - 1].q_end == saved[i].q_end && saved[m - 1].s_end == saved[i].s_end) continue;
-        saved[m++] = saved[i];
-    }
-    ns = m;
-    insertion_sort<cmp_score>(saved, ns);
-    const unsigned long long base = atomicAdd(&a.ctr->hsp_used, (unsigned long long)ns);
-    if (base + ns > a.hsp_cap) return kOverflow;
-    for (int i = 0; i < ns; i++) a.hsps[base + i] = saved[i];
-    const uint32_t ri = atomicAdd(&a.ctr->result_count, 1u);
-    a.results[ri] = ResultItem{wi.sid, (uint64_t)base, ns, wi.qi};
-    return kDone;
+    if (kWave) outcome = (int)first_lane((uint32_t)outcome);
+    return outcome;
 }
 
 __device__ __forceinline__ void record_overflow(const GappedArgs &a, uint32_t widx) {
@@ -354,16 +494,19 @@ __global__ void __launch_bounds__(64) k_gapped_lds(GappedArgs a) {
     LdsRing16 ring;
     ring.ring = ring_mem + lane;
     const uint32_t n = *a.count < a.count_cap ? *a.count : a.count_cap;
+    // the heavy cost classes (first in the order) are done by k_gapped_wave
+    uint32_t light0 = 0;
+    for (int b = 0; b < a.heavy_classes; b++) light0 += a.ctr->bucket_count[b];
     unsigned long long cells = 0, gext = 0;
     for (;;) {
         uint32_t base = 0;
         if (lane == 0) base = atomicAdd(a.next, 64u);
-        base = __shfl(base, 0, 64);
+        base = first_lane(base) + light0;
         if (base >= n) break;
         const uint32_t i = base + lane;
         if (i < n) {
             const uint32_t widx = a.order[i];
-            const int r = gapped_item(a, mat, widx, ring, saved, cells, gext);
+            const int r = gapped_item<false>(a, mat, widx, ring, saved, cells, gext);
             if (r == kRedo) {
                 const uint32_t k = atomicAdd(&a.ctr->redo_count, 1u);
                 if (k < a.redo_cap) a.redo[k] = widx;
@@ -397,10 +540,56 @@ __global__ void __launch_bounds__(1024) k_gapped_wide(GappedArgs a) {
     unsigned long long cells = 0, gext = 0;
     for (uint64_t i = gtid; i < n; i += nthreads) {
         const uint32_t widx = a.order[i];
-        if (gapped_item(a, mat, widx, ring, saved, cells, gext) != kDone) record_overflow(a, widx);
+        if (gapped_item<false>(a, mat, widx, ring, saved, cells, gext) != kDone) record_overflow(a, widx);
     }
     atomicAdd(&a.ctr->dp_cells, cells);
     atomicAdd(&a.ctr->gapped_ext, gext);
+}
+
+// Subjects of the heavy cost classes: one wave per subject, the DP band across the lanes.
+// Takes order[0, heavy) where heavy = the item count of the classes below a.heavy_classes.
+__global__ void __launch_bounds__(64) k_gapped_wave(GappedArgs a) {
+    __shared__ int16_t mat[28][28];
+    __shared__ int2 ring_mem[kWaveRing];
+    for (int i = threadIdx.x; i < 28 * 28; i += blockDim.x) mat[i / 28][i % 28] = kBlosum62Dev[i / 28][i % 28];
+    __syncthreads();
+    const int lane = lane_id();
+    uint32_t heavy = 0;
+    for (int b = 0; b < a.heavy_classes; b++) heavy += a.ctr->bucket_count[b];
+    const uint32_t n = *a.count < a.count_cap ? *a.count : a.count_cap;
+    if (heavy > n) heavy = n;
+    struct {
+        int2 *ring;
+        int wmask;
+    } ring{ring_mem, kWaveRing - 1};
+    Hsp *saved = a.hsp_scratch + (uint64_t)blockIdx.x * a.saved_cap;
+    unsigned long long cells = 0, gext = 0;
+    for (;;) {
+        uint32_t i = 0;
+        if (lane == 0) i = atomicAdd(a.next_heavy, 1u);
+        i = first_lane(i);
+        if (i >= heavy) break;
+        const uint32_t widx = a.order[i];
+        const int r = gapped_item<true>(a, mat, widx, ring, saved, cells, gext);
+        if (lane == 0) {
+            if (r == kRedo) {
+                const uint32_t k = atomicAdd(&a.ctr->redo_count, 1u);
+                if (k < a.redo_cap) a.redo[k] = widx;
+                else record_overflow(a, widx);
+            } else if (r == kOverflow) {
+                record_overflow(a, widx);
+            }
+        }
+    }
+    if (lane == 0) {
+        atomicAdd(&a.ctr->dp_cells, cells);
+        atomicAdd(&a.ctr->gapped_ext, gext);
+    }
+}
+
+hipError_t launch_gapped_wave(const GappedArgs &a, int blocks, hipStream_t st) {
+    hipLaunchKernelGGL(k_gapped_wave, dim3(blocks), dim3(64), 0, st, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_order(const WorkItem *work, DevCounters *ctr, uint32_t *order, uint32_t work_cap, int blocks,

@@ -26,6 +26,37 @@ const int16_t kBlosum62[28][28] = {
 
 using namespace blz;
 
+namespace {
+
+// Device copy of a neighbourhood lookup table over a letter string.  Words may only start inside
+// the segments [start, end) -- one segment per query of a pack, so no word spans two queries.
+struct DevTables {
+    uint8_t *d_query = nullptr;
+    int32_t len = 0;
+    uint32_t *d_prefix = nullptr;
+    uint16_t *d_entries = nullptr;
+    uint64_t n_entries = 0;
+    uint16_t *d_rel16 = nullptr;
+    uint32_t *d_blockbase = nullptr;
+    bool compact_ok = false;
+    void release() {
+        void *bufs[] = {d_query, d_prefix, d_entries, d_rel16, d_blockbase};
+        for (void *b : bufs)
+            if (b) (void)hipFree(b);
+        *this = DevTables{};
+    }
+};
+
+// Several short queries scanned as one concatenation (their lookup tables merged); the
+// gapped stage still runs per query.
+struct Pack {
+    std::vector<int> members;       // caller's query indices
+    std::vector<uint32_t> starts;   // member m occupies [starts[m], starts[m] + len_m) of the concatenation
+    DevTables t;                    // built for packs of two or more
+};
+
+}  // namespace
+
 struct blz_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -62,7 +93,11 @@ struct blz_ctx {
     size_t order_cap = 0;
     uint32_t *redo = nullptr;      // work items for the wide gapped pass
     size_t redo_cap = 0;
-    std::vector<int> batch_order;  // table index -> caller's query index
+    uint32_t *pack_meta = nullptr; // member starts and indices of the run's packs
+    size_t pack_meta_cap = 0;
+    // packs of the last batch, reused while the same queries come again
+    std::vector<const blz_query *> pack_key;
+    std::vector<Pack> packs;
 };
 
 struct blz_db {
@@ -77,13 +112,8 @@ struct blz_query {
     blz_ctx *ctx = nullptr;
     int32_t qlen = 0;
     blz_params params{};
-    uint8_t *d_query = nullptr;
-    uint32_t *d_prefix = nullptr;
-    uint16_t *d_entries = nullptr;
-    uint64_t n_entries = 0;
-    uint16_t *d_rel16 = nullptr;     // compact prefix (valid when compact_ok)
-    uint32_t *d_blockbase = nullptr;
-    bool compact_ok = false;
+    std::vector<uint8_t> letters;
+    DevTables t;
 };
 
 namespace {
@@ -92,6 +122,7 @@ constexpr int kStageCap = 4096;   // init hits per subject kept on the device
 constexpr int kDpRing = 4096;     // wide gapped pass: DP band ring per lane (power of two)
 constexpr int kSavedCap = 128;    // LDS gapped pass: HSPs per subject before the purge
 constexpr int kSavedCapWide = 4096;  // wide gapped pass: HSPs per subject before the purge
+constexpr int kSavedCapWave = 1024;  // wave gapped pass: HSPs per subject before the purge
 constexpr int kSeedWavesMax = 4;
 constexpr int kGappedBlocks = 64;        // wide gapped pass (redo list): blocks x 64 lanes
 constexpr size_t kRedoCap = 1 << 20;
@@ -166,6 +197,64 @@ uint32_t ring_words_for(int qlen, int window) {
     return m;
 }
 
+// Lookup table: for every word u, the offsets q (ascending) whose word has u in its
+// neighbourhood (prelim_stage.cpp:360-386; flattened like :534-571 but with a u32 prefix), plus
+// the compact prefix the seed kernel keeps in LDS (u32 base per 64-word block, u16 per word).
+blz_status build_tables(blz_ctx *ctx, const uint8_t *letters, int len, const std::vector<std::pair<int, int>> &segs,
+                        int threshold, DevTables &t) {
+    const Neighbourhoods &nb = neighbourhoods(threshold);
+    std::vector<uint32_t> prefix(kWords + 1, 0);
+    for (const auto &sg : segs)
+        for (int q = sg.first; q + kWord <= sg.second; q++) {
+            const int w = letters[q] * 784 + letters[q + 1] * 28 + letters[q + 2];
+            for (uint32_t k = nb.start[w]; k < nb.start[w + 1]; k++) prefix[nb.words[k] + 1]++;
+        }
+    for (int u = 0; u < kWords; u++) prefix[u + 1] += prefix[u];
+    // two spare entries: the seed kernel stages the table into LDS in 32-bit words
+    std::vector<uint16_t> entries(prefix[kWords] + 2, 0);
+    std::vector<uint32_t> cur(prefix.begin(), prefix.end() - 1);
+    for (const auto &sg : segs)
+        for (int q = sg.first; q + kWord <= sg.second; q++) {
+            const int w = letters[q] * 784 + letters[q + 1] * 28 + letters[q + 2];
+            for (uint32_t k = nb.start[w]; k < nb.start[w + 1]; k++) entries[cur[nb.words[k]]++] = (uint16_t)q;
+        }
+    std::vector<uint32_t> blockbase(kRelBlocks);
+    std::vector<uint16_t> rel16(kWords + 1);
+    bool compact_ok = true;
+    for (int b = 0; b < kRelBlocks; b++) blockbase[b] = prefix[std::min(b * 64, kWords)];
+    for (int u = 0; u <= kWords; u++) {
+        const uint32_t r = prefix[u] - blockbase[u >> 6];
+        if (r > 0xFFFFu) compact_ok = false;
+        rel16[u] = (uint16_t)r;
+    }
+    t.len = len;
+    t.n_entries = prefix[kWords];
+    t.compact_ok = compact_ok;
+    if (!check(ctx, hipMalloc((void **)&t.d_query, len + 64), "query") ||
+        !check(ctx, hipMalloc((void **)&t.d_prefix, sizeof(uint32_t) * (kWords + 1)), "prefix") ||
+        !check(ctx, hipMalloc((void **)&t.d_entries, sizeof(uint16_t) * entries.size()), "entries") ||
+        !check(ctx, hipMalloc((void **)&t.d_rel16, sizeof(uint16_t) * (kWords + 1)), "rel16") ||
+        !check(ctx, hipMalloc((void **)&t.d_blockbase, sizeof(uint32_t) * kRelBlocks), "blockbase")) {
+        t.release();
+        return BLZ_ENOMEM;
+    }
+    if (!check(ctx, hipMemcpy(t.d_query, letters, len, hipMemcpyHostToDevice), "query copy") ||
+        !check(ctx, hipMemset(t.d_query + len, 0, 64), "query pad") ||
+        !check(ctx, hipMemcpy(t.d_prefix, prefix.data(), sizeof(uint32_t) * (kWords + 1), hipMemcpyHostToDevice),
+               "prefix copy") ||
+        !check(ctx, hipMemcpy(t.d_entries, entries.data(), sizeof(uint16_t) * entries.size(), hipMemcpyHostToDevice),
+               "entries copy") ||
+        !check(ctx, hipMemcpy(t.d_rel16, rel16.data(), sizeof(uint16_t) * (kWords + 1), hipMemcpyHostToDevice),
+               "rel16 copy") ||
+        !check(ctx, hipMemcpy(t.d_blockbase, blockbase.data(), sizeof(uint32_t) * kRelBlocks, hipMemcpyHostToDevice),
+               "blockbase copy")) {
+        t.release();
+        return BLZ_EDEVICE;
+    }
+    return BLZ_OK;
+}
+
+
 }  // namespace
 
 extern "C" {
@@ -217,7 +306,8 @@ void blz_ctx_destroy(blz_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     void *bufs[] = {ctx->ctr,   ctx->stage,   ctx->pool,    ctx->work, ctx->ovf,     ctx->dp,
                     ctx->saved, ctx->hsps,    ctx->results, ctx->qdev, ctx->item_ctr,
-                    ctx->order, ctx->redo, ctx->saved_wide};
+                    ctx->order, ctx->redo, ctx->saved_wide, ctx->pack_meta};
+    for (auto &pk : ctx->packs) pk.t.release();
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &e : ctx->ev)
@@ -230,8 +320,14 @@ blz_status blz_ctx_set_tuning(blz_ctx *ctx, const blz_tuning *t) {
     if (!ctx || !t) return BLZ_EINVAL;
     if (t->seed_variant < 0 || t->seed_variant > 2 || t->seed_waves < 0 || t->seed_waves > 16 ||
         t->seed_blocks_per_cu < 0 || t->gapped_blocks < 0 || t->gapped_threads < 0 || t->gapped_threads > 1024 ||
-        (t->gapped_threads % 64) != 0 || t->seed_table < 0 || t->seed_table > 3)
+        (t->gapped_threads % 64) != 0 || t->seed_table < 0 || t->seed_table > 3 || t->gapped_wave_classes < 0 ||
+        t->gapped_wave_classes > kBuckets + 1 || t->query_packing < 0 || t->query_packing > 2)
         return BLZ_EINVAL;
+    if (t->query_packing != ctx->tune.query_packing) {  // cached packs were made under the old choice
+        for (auto &pk : ctx->packs) pk.t.release();
+        ctx->packs.clear();
+        ctx->pack_key.clear();
+    }
     ctx->tune = *t;
     return BLZ_OK;
 }
@@ -255,8 +351,8 @@ blz_status blz_db_upload(blz_ctx *ctx, const uint8_t *residues, const uint64_t *
         return BLZ_EINVAL;
     }
     for (int64_t i = 0; i < nsubj; i++)
-        if (offsets[i + 1] < offsets[i] || offsets[i + 1] - offsets[i] > (1u << 20)) {
-            ctx->err = "subject offsets must be non-decreasing and subjects at most 1M letters";
+        if (offsets[i + 1] < offsets[i] || offsets[i + 1] - offsets[i] > 65535u) {
+            ctx->err = "subject offsets must be non-decreasing and subjects shorter than 65536 letters";
             return BLZ_EINVAL;
         }
     const uint64_t nres = offsets[nsubj];
@@ -318,60 +414,16 @@ blz_status blz_query_prepare(blz_ctx *ctx, const uint8_t *query, int32_t qlen, c
             ctx->err = "query residue outside NCBIstdaa (>= 28)";
             return BLZ_EINVAL;
         }
-    const Neighbourhoods &nb = neighbourhoods(p.word_threshold);
-    // lookup table: for every word u, the query offsets q (ascending) whose word has u in its
-    // neighbourhood (prelim_stage.cpp:360-386; flattened like :534-571 but with a u32 prefix)
-    std::vector<uint32_t> prefix(kWords + 1, 0);
-    const int nw = qlen >= kWord ? qlen - kWord + 1 : 0;
-    for (int q = 0; q < nw; q++) {
-        const int w = query[q] * 784 + query[q + 1] * 28 + query[q + 2];
-        for (uint32_t k = nb.start[w]; k < nb.start[w + 1]; k++) prefix[nb.words[k] + 1]++;
-    }
-    for (int u = 0; u < kWords; u++) prefix[u + 1] += prefix[u];
-    // two spare entries: the seed kernel stages the table into LDS in 32-bit words
-    std::vector<uint16_t> entries(prefix[kWords] + 2, 0);
-    std::vector<uint32_t> cur(prefix.begin(), prefix.end() - 1);
-    for (int q = 0; q < nw; q++) {
-        const int w = query[q] * 784 + query[q + 1] * 28 + query[q + 2];
-        for (uint32_t k = nb.start[w]; k < nb.start[w + 1]; k++) entries[cur[nb.words[k]]++] = (uint16_t)q;
-    }
-    // compact prefix for LDS: a u32 base per 64-word block and a u16 offset per word
-    std::vector<uint32_t> blockbase(kRelBlocks);
-    std::vector<uint16_t> rel16(kWords + 1);
-    bool compact_ok = true;
-    for (int b = 0; b < kRelBlocks; b++) blockbase[b] = prefix[std::min(b * 64, kWords)];
-    for (int u = 0; u <= kWords; u++) {
-        const uint32_t r = prefix[u] - blockbase[u >> 6];
-        if (r > 0xFFFFu) compact_ok = false;
-        rel16[u] = (uint16_t)r;
-    }
     (void)hipSetDevice(ctx->device);
     blz_query *qq = new blz_query();
     qq->ctx = ctx;
     qq->qlen = qlen;
     qq->params = p;
-    qq->n_entries = prefix[kWords];
-    qq->compact_ok = compact_ok;
-    if (!check(ctx, hipMalloc((void **)&qq->d_query, qlen + 64), "query") ||
-        !check(ctx, hipMalloc((void **)&qq->d_prefix, sizeof(uint32_t) * (kWords + 1)), "prefix") ||
-        !check(ctx, hipMalloc((void **)&qq->d_entries, sizeof(uint16_t) * entries.size()), "entries") ||
-        !check(ctx, hipMalloc((void **)&qq->d_rel16, sizeof(uint16_t) * (kWords + 1)), "rel16") ||
-        !check(ctx, hipMalloc((void **)&qq->d_blockbase, sizeof(uint32_t) * kRelBlocks), "blockbase")) {
-        blz_query_free(qq);
-        return BLZ_ENOMEM;
-    }
-    if (!check(ctx, hipMemcpy(qq->d_query, query, qlen, hipMemcpyHostToDevice), "query copy") ||
-        !check(ctx, hipMemset(qq->d_query + qlen, 0, 64), "query pad") ||
-        !check(ctx, hipMemcpy(qq->d_prefix, prefix.data(), sizeof(uint32_t) * (kWords + 1), hipMemcpyHostToDevice),
-               "prefix copy") ||
-        !check(ctx, hipMemcpy(qq->d_entries, entries.data(), sizeof(uint16_t) * entries.size(), hipMemcpyHostToDevice),
-               "entries copy") ||
-        !check(ctx, hipMemcpy(qq->d_rel16, rel16.data(), sizeof(uint16_t) * (kWords + 1), hipMemcpyHostToDevice),
-               "rel16 copy") ||
-        !check(ctx, hipMemcpy(qq->d_blockbase, blockbase.data(), sizeof(uint32_t) * kRelBlocks, hipMemcpyHostToDevice),
-               "blockbase copy")) {
-        blz_query_free(qq);
-        return BLZ_EDEVICE;
+    qq->letters.assign(query, query + qlen);
+    const blz_status st = build_tables(ctx, query, qlen, {{0, qlen}}, p.word_threshold, qq->t);
+    if (st != BLZ_OK) {
+        delete qq;
+        return st;
     }
     *out = qq;
     return BLZ_OK;
@@ -379,11 +431,17 @@ blz_status blz_query_prepare(blz_ctx *ctx, const uint8_t *query, int32_t qlen, c
 
 void blz_query_free(blz_query *q) {
     if (!q) return;
-    if (q->d_query) (void)hipFree(q->d_query);
-    if (q->d_prefix) (void)hipFree(q->d_prefix);
-    if (q->d_entries) (void)hipFree(q->d_entries);
-    if (q->d_rel16) (void)hipFree(q->d_rel16);
-    if (q->d_blockbase) (void)hipFree(q->d_blockbase);
+    if (q->ctx) {
+        // a cached pack may hold this query; drop the cache
+        for (const blz_query *k : q->ctx->pack_key)
+            if (k == q) {
+                for (auto &pk : q->ctx->packs) pk.t.release();
+                q->ctx->packs.clear();
+                q->ctx->pack_key.clear();
+                break;
+            }
+    }
+    q->t.release();
     delete q;
 }
 
@@ -393,16 +451,16 @@ namespace {
 
 QueryDev query_dev(const blz_query *q) {
     QueryDev d{};
-    d.query = q->d_query;
-    d.prefix = q->d_prefix;
-    d.entries = q->d_entries;
-    d.rel16 = q->d_rel16;
-    d.blockbase = q->d_blockbase;
-    d.n_entries = (uint32_t)q->n_entries;
+    d.query = q->t.d_query;
+    d.prefix = q->t.d_prefix;
+    d.entries = q->t.d_entries;
+    d.rel16 = q->t.d_rel16;
+    d.blockbase = q->t.d_blockbase;
+    d.n_entries = (uint32_t)q->t.n_entries;
     d.qlen = q->qlen;
     const uint32_t ring = ring_words_for(q->qlen, q->params.window);
     d.ring_mask = ring - 1;
-    d.claim_mask = std::min<uint32_t>(ring, 256) - 1;
+    d.claim_mask = std::min<uint32_t>(ring, 128) - 1;
     d.window = q->params.window;
     d.x_ungapped = q->params.x_ungapped;
     d.ungapped_cutoff = q->params.ungapped_cutoff;
@@ -410,7 +468,62 @@ QueryDev query_dev(const blz_query *q) {
     d.gap_open = q->params.gap_open;
     d.gap_extend = q->params.gap_extend;
     d.gapped_cutoff = q->params.gapped_cutoff;
+    d.n_members = 1;
     return d;
+}
+
+bool same_params(const blz_params &a, const blz_params &b) { return std::memcmp(&a, &b, sizeof(a)) == 0; }
+
+// Concatenated letters of a pack: at most what keeps the diagonal ring at 1,024 words.
+int pack_len(int window) { return std::max(64, 1024 - 128 - window); }
+
+// Groups the batch's short queries (by length) into packs; a query longer than half a pack
+// forms its own.  Packs of several queries get merged lookup tables over the concatenation,
+// with window + 3 letter-0 separators so no two-hit pair or X-drop walk crosses queries.
+blz_status make_packs(blz_ctx *ctx, const blz_query *const *queries, int nq, std::vector<Pack> &packs) {
+    std::vector<int> idx(nq);
+    for (int i = 0; i < nq; i++) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return queries[x]->qlen < queries[y]->qlen; });
+    packs.clear();
+    const bool packing = nq > 1 && ctx->tune.query_packing != 1;
+    for (size_t k = 0; k < idx.size();) {
+        const blz_query *q0 = queries[idx[k]];
+        Pack pk;
+        const int gap = q0->params.window + 3;
+        const int max_len = pack_len(q0->params.window);
+        int len = 0;
+        while (k < idx.size()) {
+            const blz_query *q = queries[idx[k]];
+            const int add = (pk.members.empty() ? 0 : gap) + q->qlen;
+            if (!pk.members.empty() &&
+                (!packing || q->qlen > max_len / 2 || len + add > max_len || (int)pk.members.size() >= kMaxMembers ||
+                 !same_params(q->params, q0->params)))
+                break;
+            pk.starts.push_back((uint32_t)(len + (pk.members.empty() ? 0 : gap)));
+            pk.members.push_back(idx[k]);
+            len += add;
+            k++;
+            if (!packing || q->qlen > max_len / 2) break;
+        }
+        if (pk.members.size() > 1) {
+            std::vector<uint8_t> cat((size_t)len, 0);
+            std::vector<std::pair<int, int>> segs;
+            for (size_t m = 0; m < pk.members.size(); m++) {
+                const blz_query *q = queries[pk.members[m]];
+                std::memcpy(cat.data() + pk.starts[m], q->letters.data(), q->qlen);
+                segs.push_back({(int)pk.starts[m], (int)pk.starts[m] + q->qlen});
+            }
+            const blz_status st = build_tables(ctx, cat.data(), len, segs, q0->params.word_threshold, pk.t);
+            if (st != BLZ_OK) {
+                for (auto &p : packs) p.t.release();
+                packs.clear();
+                return st;
+            }
+        }
+        pk.starts.push_back((uint32_t)len);
+        packs.push_back(std::move(pk));
+    }
+    return BLZ_OK;
 }
 
 // Launch shape of the seed kernel for a set of queries (sizes are maxima).  table: where the
@@ -432,7 +545,20 @@ bool plan_seed2(blz_ctx *ctx, const std::vector<QueryDev> &qs, const std::vector
         sa.n_entries = std::max(sa.n_entries, qs[i].n_entries);
     }
     const size_t cap = ctx->lds_per_block;
-    int waves = ctx->tune.seed_waves ? ctx->tune.seed_waves : 8;
+    int waves = ctx->tune.seed_waves;
+    if (!waves) {  // most resident waves per CU (LDS is per CU); fewer waves per group on ties
+        int best_total = 0;
+        for (int w = 2; w <= 16; w++) {
+            const size_t need = seed2_lds_bytes(sa, table, w);
+            if (need > cap) break;
+            const int total = std::min<int>(32, w * (int)(cap / need));
+            if (total > best_total) {
+                best_total = total;
+                waves = w;
+            }
+        }
+        if (!waves) waves = 1;
+    }
     while (waves > 1 && seed2_lds_bytes(sa, table, waves) > cap) waves--;
     if (seed2_lds_bytes(sa, table, waves) > cap) return false;
     const size_t used = seed2_lds_bytes(sa, table, waves);
@@ -466,11 +592,13 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
         if (!queries[i] || queries[i]->ctx != ctx) return BLZ_EINVAL;
     const int variant = ctx->tune.seed_variant ? ctx->tune.seed_variant : 2;
     const size_t nsubj = (size_t)db->nsubj;
+    // query table: [0, nq) the caller's queries (gapped stage), then one entry per pack (seed)
     std::vector<QueryDev> qs(nq);
     for (int i = 0; i < nq; i++) qs[i] = query_dev(queries[i]);
 
-    // seed launch plans: query-size bins (tables in LDS / tables in global memory)
     std::vector<SeedPlan> plans;
+    std::vector<uint32_t> meta;  // per pack: member starts (n+1) then member indices (n)
+    std::vector<size_t> meta_off;
     int max_waves = 1;
     size_t max_blocks = 1;
     if (variant == 1) {
@@ -490,29 +618,66 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
         pl.blocks = std::max(1, ctx->num_cus * 8 / waves);
         plans.push_back(pl);
     } else {
+        std::vector<const blz_query *> key(queries, queries + nq);
+        if (key != ctx->pack_key) {
+            for (auto &pk : ctx->packs) pk.t.release();
+            ctx->packs.clear();
+            ctx->pack_key.clear();
+            const blz_status st = make_packs(ctx, queries, nq, ctx->packs);
+            if (st != BLZ_OK) return st;
+            ctx->pack_key = key;
+        }
+        const std::vector<Pack> &packs = ctx->packs;
+        std::vector<QueryDev> pq(packs.size());
+        std::vector<bool> compact(packs.size());
+        for (size_t k = 0; k < packs.size(); k++) {
+            const Pack &pk = packs[k];
+            const blz_query *q0 = queries[pk.members[0]];
+            QueryDev d = query_dev(q0);
+            const DevTables &t = pk.members.size() > 1 ? pk.t : q0->t;
+            d.query = t.d_query;
+            d.prefix = t.d_prefix;
+            d.entries = t.d_entries;
+            d.rel16 = t.d_rel16;
+            d.blockbase = t.d_blockbase;
+            d.n_entries = (uint32_t)t.n_entries;
+            d.qlen = t.len;
+            const uint32_t ring = ring_words_for(t.len, q0->params.window);
+            d.ring_mask = ring - 1;
+            d.claim_mask = std::min<uint32_t>(ring, 128) - 1;
+            d.n_members = (int32_t)pk.members.size();
+            meta_off.push_back(meta.size());
+            for (uint32_t v : pk.starts) meta.push_back(v);
+            for (int m : pk.members) meta.push_back((uint32_t)m);
+            pq[k] = d;
+            compact[k] = t.compact_ok;
+        }
         std::vector<int> bins[3];
-        for (int i = 0; i < nq; i++) bins[pick_table(ctx, qs, i, queries[i]->compact_ok)].push_back(i);
-        // each bin's queries are made contiguous in the query table; the plan records its first
+        for (size_t k = 0; k < packs.size(); k++) bins[pick_table(ctx, pq, (int)k, compact[k])].push_back((int)k);
+        // each bin's packs are made contiguous after the queries; the plan records its first
         std::vector<int> order;
         for (int table = 2; table >= 0; table--) {
             const std::vector<int> &idx = bins[table];
             if (idx.empty()) continue;
             SeedPlan pl;
-            if (!plan_seed2(ctx, qs, idx, table, pl)) {
+            if (!plan_seed2(ctx, pq, idx, table, pl)) {
                 ctx->err = "query too long for the diagonal ring in LDS";
                 return BLZ_EINVAL;
             }
             pl.sa.nq = (int32_t)idx.size();
-            pl.sa.s_begin = (int64_t)order.size();  // first query of the bin in the reordered table
-            for (int i : idx) order.push_back(i);
+            pl.sa.s_begin = nq + (int64_t)order.size();  // first pack of the bin in the query table
+            for (int k : idx) order.push_back(k);
             plans.push_back(pl);
         }
-        std::vector<QueryDev> ordered;
-        for (int i : order) ordered.push_back(qs[i]);
-        qs.swap(ordered);
-        ctx->batch_order = order;
+        if (!ensure(ctx, ctx->pack_meta, ctx->pack_meta_cap, std::max<size_t>(meta.size(), 1), "pack table"))
+            return BLZ_ENOMEM;
+        for (int k : order) {
+            QueryDev d = pq[k];
+            d.member_start = ctx->pack_meta + meta_off[k];
+            d.member_idx = d.member_start + d.n_members + 1;
+            qs.push_back(d);
+        }
     }
-    if (variant == 1) ctx->batch_order.assign(1, 0);
     for (auto &pl : plans) {
         max_waves = std::max(max_waves, pl.waves);
         max_blocks = std::max<size_t>(max_blocks, (size_t)pl.blocks * pl.waves);
@@ -532,13 +697,15 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
         !ensure(ctx, ctx->saved_wide, ctx->saved_wide_records, wide_lanes * kSavedCapWide, "saved wide") ||
         !ensure(ctx, ctx->hsps, ctx->hsp_cap, std::max<size_t>(1 << 22, per_q), "hsps") ||
         !ensure(ctx, ctx->results, ctx->results_cap, work_cap, "results") ||
-        !ensure(ctx, ctx->qdev, ctx->qdev_cap, (size_t)nq, "query table") ||
+        !ensure(ctx, ctx->qdev, ctx->qdev_cap, qs.size(), "query table") ||
         !ensure(ctx, ctx->item_ctr, ctx->item_ctr_cap, plans.size() + 1, "item counters"))
         return BLZ_ENOMEM;
 
     hipStream_t st = ctx->stream;
-    if (!check(ctx, hipMemcpyAsync(ctx->qdev, qs.data(), sizeof(QueryDev) * nq, hipMemcpyHostToDevice, st),
+    if (!check(ctx, hipMemcpyAsync(ctx->qdev, qs.data(), sizeof(QueryDev) * qs.size(), hipMemcpyHostToDevice, st),
                "query table") ||
+        (!meta.empty() && !check(ctx, hipMemcpyAsync(ctx->pack_meta, meta.data(), sizeof(uint32_t) * meta.size(),
+                                                      hipMemcpyHostToDevice, st), "pack table")) ||
         !check(ctx, hipMemsetAsync(ctx->ctr, 0, sizeof(DevCounters), st), "memset") ||
         !check(ctx, hipMemsetAsync(ctx->item_ctr, 0, sizeof(uint32_t) * (plans.size() + 1), st), "memset"))
         return BLZ_EDEVICE;
@@ -582,7 +749,6 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
             sa.subj_per_item = std::max<int64_t>(16, (db->nsubj * sa.nq + target_items - 1) / target_items);
             sa.subj_per_item = std::min<int64_t>(sa.subj_per_item, std::max<int64_t>(db->nsubj, 1));
             sa.items_per_query = (uint32_t)((db->nsubj + sa.subj_per_item - 1) / sa.subj_per_item);
-            sa.qi_base = first;
             sa.item_ctr = ctx->item_ctr + pi;
             e = launch_seed2(sa, pl.table, pl.blocks, pl.waves, st);
         }
@@ -598,6 +764,8 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
     ga.count = &ctr->work_count;
     ga.count_cap = (uint32_t)ctx->work_cap;
     ga.next = &ctr->next_work;
+    ga.next_heavy = &ctr->next_heavy;
+    ga.heavy_classes = ctx->tune.gapped_wave_classes > 0 ? ctx->tune.gapped_wave_classes - 1 : 2;
     ga.redo = ctx->redo;
     ga.redo_cap = (uint32_t)ctx->redo_cap;
     ga.dp_scratch = ctx->dp;
@@ -615,13 +783,18 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
     gw.count = &ctr->redo_count;
     gw.count_cap = (uint32_t)ctx->redo_cap;
     gw.next = &ctr->next_redo;
+    gw.heavy_classes = 0;
     gw.redo = nullptr;
     gw.redo_cap = 0;
     gw.hsp_scratch = ctx->saved_wide;
     gw.saved_cap = kSavedCapWide;
     const int lds_blocks = ctx->tune.gapped_blocks ? ctx->tune.gapped_blocks : ctx->num_cus * 4;
+    const int heavy_blocks = ctx->num_cus * 8;
+    GappedArgs gh = ga;  // wave pass: per-block saved list of kSavedCapWave
+    gh.saved_cap = kSavedCapWave;
     if (!check(ctx, hipEventRecord(ctx->ev[1], st), "event") ||
         (nsubj > 0 && !check(ctx, launch_order(ctx->work, ctr, ctx->order, (uint32_t)ctx->work_cap, 512, st), "k_order launch")) ||
+        (nsubj > 0 && ga.heavy_classes > 0 && !check(ctx, launch_gapped_wave(gh, heavy_blocks, st), "k_gapped_wave launch")) ||
         (nsubj > 0 && !check(ctx, launch_gapped_lds(ga, lds_blocks, st), "k_gapped_lds launch")) ||
         (nsubj > 0 && !check(ctx, launch_gapped_wide(gw, kGappedBlocks, 64, st), "k_gapped_wide launch")) ||
         !check(ctx, hipEventRecord(ctx->ev[2], st), "event"))
@@ -646,9 +819,8 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
     (void)hipEventElapsedTime(&ms_gapped, ctx->ev[1], ctx->ev[2]);
 
     // per query (in the caller's order), subjects ascending
-    std::sort(res.begin(), res.end(), [&](const ResultItem &a, const ResultItem &b) {
-        const int qa = ctx->batch_order[a.qi], qb = ctx->batch_order[b.qi];
-        return qa != qb ? qa < qb : a.sid < b.sid;
+    std::sort(res.begin(), res.end(), [](const ResultItem &a, const ResultItem &b) {
+        return a.qi != b.qi ? a.qi < b.qi : a.sid < b.sid;
     });
     blz_status rc = BLZ_OK;
     int64_t total = 0;
@@ -656,7 +828,7 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
     for (int qcaller = 0; qcaller < nq; qcaller++) {
         size_t r1 = r0;
         int64_t nh = 0;
-        while (r1 < res.size() && ctx->batch_order[res[r1].qi] == qcaller) nh += res[r1++].n;
+        while (r1 < res.size() && res[r1].qi == qcaller) nh += res[r1++].n;
         total += nh;
         if (sinks) {
             blz_hsp_sink *sink = &sinks[qcaller];

@@ -36,13 +36,12 @@ constexpr uint32_t kPending = 0x40000000u;  // extension queued: low bits = task
 constexpr uint32_t kPosMask = 0x3FFFFFFFu;
 constexpr int kTaskCap = 128;
 constexpr int kDeferCap = 256;
-constexpr int kHitBuf = 256;                // hits of one step laid out in LDS
-constexpr int kSubjBuf = 2048;              // subject letters staged in LDS per wave
+constexpr int kHitBuf = 128;                // hits of one step laid out in LDS
+constexpr int kSubjBuf = 1024;              // subject letters staged in LDS per wave
 
 struct Task {
     uint32_t q_slot;  // query offset | diagonal slot << 16
-    int32_t s;        // subject offset of the second hit
-    int32_t s_left;   // end of the first hit's word
+    uint32_t s_pack;  // subject offset of the second hit | end of the first hit's word << 16
 };
 
 struct WaveState {
@@ -53,14 +52,14 @@ struct WaveState {
     uint32_t *cnt;  // [0] tasks, [1] deferred, [2] init hits
     uint32_t *hits;
     Task *task;
-    int2 *defer;
+    uint32_t *defer;  // query offset | subject offset << 16
     uint8_t *subj;
 };
 
 __host__ __device__ constexpr uint32_t al16(uint32_t b) { return (b + 15u) & ~15u; }
 
 __host__ __device__ inline uint32_t wave_lds_bytes(uint32_t M, uint32_t C) {
-    return al16(4 * (M + C + 68 + 64 + 4 + kHitBuf) + sizeof(Task) * kTaskCap + sizeof(int2) * kDeferCap +
+    return al16(4 * (M + C + 68 + 64 + 4 + kHitBuf) + sizeof(Task) * kTaskCap + sizeof(uint32_t) * kDeferCap +
                 kSubjBuf);
 }
 
@@ -188,7 +187,7 @@ __device__ __forceinline__ void apply_hit(const Ctx &c, WaveState &w, int q, int
     const uint32_t sabs = c.abs0 + (uint32_t)s;
     if (st & kPending) {
         const uint32_t k = atomicAdd(&w.cnt[1], 1u);
-        w.defer[k] = make_int2(q, s);
+        w.defer[k] = (uint32_t)q | ((uint32_t)s << 16);
         return;
     }
     if (st & kFlag) {
@@ -202,7 +201,7 @@ __device__ __forceinline__ void apply_hit(const Ctx &c, WaveState &w, int q, int
     }
     if (diff < (uint32_t)kWord) return;
     const uint32_t t = atomicAdd(&w.cnt[0], 1u);
-    w.task[t] = Task{(uint32_t)q | (slot << 16), s, (int)(st - 1u - c.abs0) + kWord};
+    w.task[t] = Task{(uint32_t)q | (slot << 16), (uint32_t)s | ((st - 1u - c.abs0 + kWord) << 16)};
     w.ring[slot] = kPending | t;
 }
 
@@ -239,14 +238,15 @@ __device__ void flush(const Ctx &c, WaveState &w, SP S, int slen, unsigned long 
                 const Task tk = w.task[t];
                 const int q = (int)(tk.q_slot & 0xFFFFu);
                 const uint32_t slot = tk.q_slot >> 16;
+                const int ts = (int)(tk.s_pack & 0xFFFFu);
                 int hq, hs, hl, rext, slast;
-                const int score = two_hit_walk(c, S, slen, tk.s_left, tk.s, q, &hq, &hs, &hl, &rext, &slast);
+                const int score = two_hit_walk(c, S, slen, (int)(tk.s_pack >> 16), ts, q, &hq, &hs, &hl, &rext, &slast);
                 n_ext++;
                 if (score >= c.cutoff) {
                     const uint32_t k = atomicAdd(&w.cnt[2], 1u);
                     if (k < (uint32_t)c.stage_cap) c.stage[k] = InitHit{hq, hs, hl, score};
                 }
-                w.ring[slot] = rext ? ((c.abs0 + (uint32_t)slast - 2u + 1u) | kFlag) : (c.abs0 + (uint32_t)tk.s + 1u);
+                w.ring[slot] = rext ? ((c.abs0 + (uint32_t)slast - 2u + 1u) | kFlag) : (c.abs0 + (uint32_t)ts + 1u);
             }
         }
         wave_sync();
@@ -258,9 +258,9 @@ __device__ void flush(const Ctx &c, WaveState &w, SP S, int slen, unsigned long 
             int q = 0, s = 0;
             const bool pend = i < nd;
             if (pend) {
-                const int2 d = w.defer[i];
-                q = d.x;
-                s = d.y;
+                const uint32_t d = w.defer[i];
+                q = (int)(d & 0xFFFFu);
+                s = (int)(d >> 16);
             }
             resolve_round(c, w, pend, q, s, i);
         }
@@ -301,7 +301,7 @@ __device__ void scan_subject(const Ctx &c, WaveState &w, const Tables &t, SP S, 
         uint32_t pa = 0, nh = 0;
         if (s + kWord <= slen) word_range<kTable>(t, S[s] * 784u + S[s + 1] * 28u + S[s + 2], pa, nh);
         const uint32_t inc = wave_inclusive_sum(nh);
-        const uint32_t H = __shfl(inc, 63, 64);
+        const uint32_t H = last_lane(inc);
         const uint32_t ex = inc - nh;
         n_hits += nh;
         if (H <= (uint32_t)kHitBuf) {
@@ -341,6 +341,104 @@ __device__ void scan_subject(const Ctx &c, WaveState &w, const Tables &t, SP S, 
         wave_sync();
     }
     flush(c, w, S, slen, n_ext);
+}
+
+// The subject's init hits (staged by this wave) go to the pool, one work item per query of
+// the item's pack that has any.  Query offsets become offsets in that query.
+__device__ void emit_init_hits(const SeedArgs &a, const QueryDev &qd, WaveState &w, const Ctx &c, int64_t sid,
+                               uint32_t nih) {
+    const int lane = lane_id();
+    const int nm = qd.n_members;
+    auto overflow = [&](int qi) {
+        if (lane == 0) {
+            const uint32_t k = atomicAdd(&a.ctr->overflow_count, 1u);
+            if (k < a.ovf_cap) a.ovf[k] = ((int64_t)qi << kOvfShift) | sid;
+        }
+    };
+    if (nih > (uint32_t)a.stage_cap) {
+        for (int m = 0; m < nm; m++) overflow((int)qd.member_idx[m]);
+        return;
+    }
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(&a.ctr->pool_used, (unsigned long long)nih);
+    base = first_lane64(base);
+    if (base + nih > a.pool_cap) {
+        for (int m = 0; m < nm; m++) overflow((int)qd.member_idx[m]);
+        return;
+    }
+    if (nm == 1) {
+        int best = 0;
+        for (uint32_t k = lane; k < nih; k += 64) {
+            const InitHit h = c.stage[k];
+            a.pool[base + k] = h;
+            best = h.score > best ? h.score : best;
+        }
+        best = (int)last_lane((uint32_t)wave_inclusive_max(best));
+        if (lane == 0) {
+            const int qi = (int)qd.member_idx[0];
+            const uint32_t wi = atomicAdd(&a.ctr->work_count, 1u);
+            if (wi < a.work_cap) {
+                const int bucket = cost_bucket(best);
+                a.work[wi] = WorkItem{sid, (uint64_t)base, (int32_t)nih, qi, bucket, 0};
+                atomicAdd(&a.ctr->bucket_count[bucket], 1u);
+                atomicAdd(&a.ctr->init_hits, (unsigned long long)nih);
+            } else {
+                const uint32_t k = atomicAdd(&a.ctr->overflow_count, 1u);
+                if (k < a.ovf_cap) a.ovf[k] = ((int64_t)qi << kOvfShift) | sid;
+            }
+        }
+        return;
+    }
+    // a pack of several queries: count, lay out and relabel the hits per query
+    uint32_t *cnt = w.p0;          // [kMaxMembers] counts, then cursors
+    uint32_t *bestm = w.excl;      // [kMaxMembers] best score per query
+    uint32_t *offm = w.excl + 32;  // [kMaxMembers] first slot per query
+    if (lane < nm) {
+        cnt[lane] = 0;
+        bestm[lane] = 0;
+    }
+    wave_sync();
+    auto member_of = [&](int q) {
+        int m = 0;
+        while (m + 1 < nm && (int)qd.member_start[m + 1] <= q) m++;
+        return m;
+    };
+    for (uint32_t k = lane; k < nih; k += 64) {
+        const InitHit h = c.stage[k];
+        const int m = member_of(h.q_start);
+        atomicAdd(&cnt[m], 1u);
+        atomicMax(&bestm[m], (uint32_t)h.score);
+    }
+    wave_sync();
+    const uint32_t mine = lane < nm ? cnt[lane] : 0u;
+    const uint32_t off = wave_inclusive_sum(mine) - mine;
+    wave_sync();
+    if (lane < nm) {
+        offm[lane] = off;
+        cnt[lane] = 0;
+    }
+    wave_sync();
+    for (uint32_t k = lane; k < nih; k += 64) {
+        InitHit h = c.stage[k];
+        const int m = member_of(h.q_start);
+        const uint32_t pos = offm[m] + atomicAdd(&cnt[m], 1u);
+        h.q_start -= (int)qd.member_start[m];
+        a.pool[base + pos] = h;
+    }
+    wave_sync();
+    if (lane < nm && mine > 0) {
+        const int qi = (int)qd.member_idx[lane];
+        const uint32_t wi = atomicAdd(&a.ctr->work_count, 1u);
+        if (wi < a.work_cap) {
+            const int bucket = cost_bucket((int)bestm[lane]);
+            a.work[wi] = WorkItem{sid, (uint64_t)(base + off), (int32_t)mine, qi, bucket, 0};
+            atomicAdd(&a.ctr->bucket_count[bucket], 1u);
+            atomicAdd(&a.ctr->init_hits, (unsigned long long)mine);
+        } else {
+            const uint32_t k = atomicAdd(&a.ctr->overflow_count, 1u);
+            if (k < a.ovf_cap) a.ovf[k] = ((int64_t)qi << kOvfShift) | sid;
+        }
+    }
 }
 
 }  // namespace
@@ -385,7 +483,7 @@ __global__ void __launch_bounds__(1024) k_seed2(SeedArgs a) {
     w.cnt = w.p0 + 64;
     w.hits = w.cnt + 4;
     w.task = reinterpret_cast<Task *>(w.hits + kHitBuf);
-    w.defer = reinterpret_cast<int2 *>(w.task + kTaskCap);
+    w.defer = reinterpret_cast<uint32_t *>(w.task + kTaskCap);
     w.subj = reinterpret_cast<uint8_t *>(w.defer + kDeferCap);
 
     for (int i = threadIdx.x; i < 28 * 28; i += blockDim.x) mat[i / 28][i % 28] = kBlosum62Dev[i / 28][i % 28];
@@ -446,7 +544,7 @@ __global__ void __launch_bounds__(1024) k_seed2(SeedArgs a) {
         for (;;) {
             int64_t sid = 0;
             if (lane == 0) sid = sub0 + (int64_t)atomicAdd(reinterpret_cast<uint32_t *>(&blk[1]), 1u);
-            sid = __shfl(sid, 0, 64);
+            sid = (int64_t)first_lane64((uint64_t)sid);
             if (sid >= sub1) break;
             const uint64_t o0 = a.offs[sid];
             const int slen = (int)(a.offs[sid + 1] - o0);
@@ -479,47 +577,7 @@ __global__ void __launch_bounds__(1024) k_seed2(SeedArgs a) {
             const uint32_t nih = w.cnt[2];
             wave_sync();
             if (nih > 0) {
-                const int64_t tag = ((int64_t)(a.qi_base + qi) << kOvfShift) | sid;
-                if (nih > (uint32_t)a.stage_cap) {
-                    if (lane == 0) {
-                        const uint32_t k = atomicAdd(&a.ctr->overflow_count, 1u);
-                        if (k < a.ovf_cap) a.ovf[k] = tag;
-                    }
-                } else {
-                    unsigned long long base = 0;
-                    if (lane == 0) base = atomicAdd(&a.ctr->pool_used, (unsigned long long)nih);
-                    base = __shfl(base, 0, 64);
-                    if (base + nih > a.pool_cap) {
-                        if (lane == 0) {
-                            const uint32_t k = atomicAdd(&a.ctr->overflow_count, 1u);
-                            if (k < a.ovf_cap) a.ovf[k] = tag;
-                        }
-                    } else {
-                        int best = 0;
-                        for (uint32_t k = lane; k < nih; k += 64) {
-                            const InitHit h = c.stage[k];
-                            a.pool[base + k] = h;
-                            best = h.score > best ? h.score : best;
-                        }
-#pragma unroll
-                        for (int o = 32; o > 0; o >>= 1) {
-                            const int tt = __shfl_xor(best, o, 64);
-                            best = tt > best ? tt : best;
-                        }
-                        if (lane == 0) {
-                            const uint32_t wi = atomicAdd(&a.ctr->work_count, 1u);
-                            if (wi < a.work_cap) {
-                                const int bucket = cost_bucket(best);
-                                a.work[wi] = WorkItem{sid, (uint64_t)base, (int32_t)nih, a.qi_base + qi, bucket, 0};
-                                atomicAdd(&a.ctr->bucket_count[bucket], 1u);
-                                atomicAdd(&a.ctr->init_hits, (unsigned long long)nih);
-                            } else {
-                                const uint32_t k = atomicAdd(&a.ctr->overflow_count, 1u);
-                                if (k < a.ovf_cap) a.ovf[k] = tag;
-                            }
-                        }
-                    }
-                }
+                emit_init_hits(a, qd, w, c, sid, nih);
                 wave_sync();
                 if (lane == 0) w.cnt[2] = 0;
                 wave_sync();

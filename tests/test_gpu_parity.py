@@ -163,3 +163,45 @@ def test_batch_matches_single_queries(engine):
         assert sorted(want) == sorted(have)
         for oid in want:
             assert np.array_equal(want[oid], have[oid]), (len(q), oid)
+
+
+def test_packed_batch(engine):
+    """Many short queries share seed scans (merged lookup tables over their concatenation);
+    each query's lists and the summed counters equal the oracle's, with packing on or off, and a
+    query with other parameters goes to a pack of its own."""
+    if engine.get_tuning()["seed_variant"] == 1:
+        pytest.skip("variant 1 runs one query at a time")
+    lens = [3, 5, 11, 30, 47, 64, 90, 120, 150, 200, 230, 260, 300, 350, 380, 420, 40, 75, 9, 180]
+    qs = [blz.synth_query(300 + i, L) for i, L in enumerate(lens)]
+    spec = blz.synth_spec(seed=31, n_subjects=1200, homolog_frac=0.08, ident_min=0.35, ident_max=0.8)
+    res, offs, _ = blz.synth_db(spec, qs)
+    engine.upload_db(res, offs)
+    other = blz.default_params(word_threshold=12)
+    p_other = oracle.params(word_threshold=12)
+    params = [other if i == 7 else None for i in range(len(qs))]
+    qhs = [engine.prepare(q, p) for q, p in zip(qs, params)]
+    tune = engine.get_tuning()
+    try:
+        runs = {}
+        for packing in (0, 1):
+            engine.set_tuning(**{**tune, "query_packing": packing})
+            runs[packing] = engine.run_batch(qhs)
+            print(f"[case] packing {packing}: {runs[packing][1]['seed_launches']} seed launches", flush=True)
+    finally:
+        engine.set_tuning(**tune)
+        for h in qhs:
+            engine.free_query(h)
+    totals = {k: 0 for k in ("word_hits", "ungapped_ext", "init_hits", "gapped_ext", "dp_cells")}
+    for i, q in enumerate(qs):
+        counts, hsps, ost = oracle.prelim_db(q, res, offs, p_other if i == 7 else None, nthreads=NTHREADS)
+        for k in totals:
+            totals[k] += ost[k]
+        want = oracle.as_lists(counts, hsps)
+        for packing, (outs, _) in runs.items():
+            have = outs[i].as_dict()
+            assert sorted(want) == sorted(have), (packing, len(q))
+            for oid in want:
+                assert np.array_equal(want[oid], have[oid]), (packing, len(q), oid)
+    for packing, (_, stats) in runs.items():
+        for k, v in totals.items():
+            assert stats[k] == v, (packing, k)
