@@ -16,6 +16,10 @@ struct QueryDev {
     uint32_t n_entries;
     int32_t qlen;
     uint32_t ring_mask, claim_mask;
+    // seed kernel 2: diagonal (q - s) sits in ring slot (q + ring_base - s) mod ring_len, with
+    // ring_base a multiple of ring_len above every subject offset; ring_magic = ceil(2^32 /
+    // ring_len) makes the modulo a multiply-high (exact below 2^32 / ring_len)
+    uint32_t ring_len, ring_base, ring_magic;
     int32_t window, x_ungapped, ungapped_cutoff;
     int32_t x_gapped, gap_open, gap_extend, gapped_cutoff;
     // a pack of queries scanned together (seed stage): query offsets of the concatenation
@@ -27,6 +31,10 @@ struct QueryDev {
 };
 constexpr int kMaxMembers = 32;
 
+// Longest subject the seed kernel's LDS diagonal ring (16-bit entries) takes; longer subjects
+// use SeedArgs::long_ring.
+constexpr int kSeedNarrowMax = 0x3FFF;
+
 struct SeedArgs {
     const uint8_t *res;        // concatenated subject residues (device)
     const uint64_t *offs;      // subject offsets [nsubj+1] (device)
@@ -36,7 +44,8 @@ struct SeedArgs {
     const uint32_t *prefix;    // lookup prefix [kWords+1]
     const uint16_t *entries;   // query offsets per word, ascending
     int32_t window, x_ungapped, ungapped_cutoff;
-    uint32_t ring_mask;        // diagonal ring size - 1 (power of two >= qlen + 64 + window)
+    uint32_t ring_mask;        // seed kernel 1: diagonal ring size - 1 (power of two >= qlen + 64 + window)
+    uint32_t ring_len;         // seed kernel 2: largest diagonal ring of the launch's packs
     uint32_t claim_mask;       // same-diagonal arbitration table size - 1 (power of two)
     const uint16_t *rel16;     // compact prefix: offset of word w inside its 64-word block
     const uint32_t *blockbase; // compact prefix: first entry of each 64-word block
@@ -54,8 +63,10 @@ struct SeedArgs {
     const QueryDev *qd;
     int32_t nq;                // packs of this launch
     uint32_t *item_ctr;        // this launch's work-item counter
-    uint32_t items_per_query;
-    int64_t subj_per_item;
+    uint32_t *long_ring;       // long-subject launch: diagonal rings, ring_len words per wave
+    const int64_t *long_ids;   // long-subject launch: the subjects longer than kSeedNarrowMax
+    const int64_t *slice;      // subject slices of about equal residues: [slice[j], slice[j+1])
+    uint32_t n_slices;         // work items = packs x slices, pack-major, last slice first
 };
 
 struct GappedArgs {
@@ -86,7 +97,9 @@ struct GappedArgs {
 
 size_t seed_lds_bytes(uint32_t ring_words, int waves_per_block);
 size_t seed2_lds_bytes(const SeedArgs &a, int table, int waves);
-hipError_t launch_seed2(const SeedArgs &a, int table, int blocks, int waves, hipStream_t st);
+hipError_t launch_seed2(const SeedArgs &a, int table, int blocks, int waves, bool long_subjects, hipStream_t st);
+// Resident workgroups per CU of that launch shape (registers and LDS), 0 if it cannot run.
+int seed2_blocks_per_cu(const SeedArgs &a, int table, int waves, bool long_subjects);
 hipError_t launch_seed(const SeedArgs &a, int blocks, int waves_per_block, hipStream_t st);
 // cost-ordered work list: order[] receives the work indices, heaviest class first
 hipError_t launch_order(const WorkItem *work, DevCounters *ctr, uint32_t *order, uint32_t work_cap, int blocks,

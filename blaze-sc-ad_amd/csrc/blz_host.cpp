@@ -95,6 +95,10 @@ struct blz_ctx {
     size_t redo_cap = 0;
     uint32_t *pack_meta = nullptr; // member starts and indices of the run's packs
     size_t pack_meta_cap = 0;
+    int64_t *slices = nullptr;     // subject slice bounds of the run's seed launches
+    size_t slices_cap = 0;
+    uint32_t *long_ring = nullptr; // seed rings for subjects the LDS ring does not take
+    size_t long_ring_cap = 0;
     // packs of the last batch, reused while the same queries come again
     std::vector<const blz_query *> pack_key;
     std::vector<Pack> packs;
@@ -106,6 +110,9 @@ struct blz_db {
     uint64_t *offs = nullptr;   // device
     int64_t nsubj = 0;
     uint64_t nres = 0;
+    std::vector<uint64_t> h_offs;  // host copy of the offsets, for slicing by residues
+    int64_t *long_ids = nullptr;   // device: subjects longer than the seed kernel's LDS ring takes
+    int64_t n_long = 0;
 };
 
 struct blz_query {
@@ -188,6 +195,17 @@ const Neighbourhoods &neighbourhoods(int T) {
     std::lock_guard<std::mutex> lk(g_nb_mutex);
     if (g_nb.threshold != T) build_neighbourhoods(T, g_nb);
     return g_nb;
+}
+
+// Seed kernel 2 ring: every diagonal a hit within window letters behind the current 64-letter
+// step can touch, plus a step of slack; a multiple of 8 entries (16-byte clears).
+constexpr uint32_t kRingLenMax = 30000;  // the multiply-high modulo is exact up to here
+uint32_t ring_len_for(int qlen, int window) { return ((uint32_t)qlen + (uint32_t)window + 128u + 7u) & ~7u; }
+
+void set_ring(QueryDev &d, int qlen, int window) {
+    d.ring_len = ring_len_for(qlen, window);
+    d.ring_base = ((65536u + d.ring_len - 1) / d.ring_len) * d.ring_len;
+    d.ring_magic = (uint32_t)((((uint64_t)1 << 32) + d.ring_len - 1) / d.ring_len);
 }
 
 uint32_t ring_words_for(int qlen, int window) {
@@ -304,7 +322,7 @@ void blz_ctx_destroy(blz_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    void *bufs[] = {ctx->ctr,   ctx->stage,   ctx->pool,    ctx->work, ctx->ovf,     ctx->dp,
+    void *bufs[] = {ctx->long_ring, ctx->slices, ctx->ctr,   ctx->stage,   ctx->pool,    ctx->work, ctx->ovf,     ctx->dp,
                     ctx->saved, ctx->hsps,    ctx->results, ctx->qdev, ctx->item_ctr,
                     ctx->order, ctx->redo, ctx->saved_wide, ctx->pack_meta};
     for (auto &pk : ctx->packs) pk.t.release();
@@ -379,6 +397,18 @@ blz_status blz_db_upload(blz_ctx *ctx, const uint8_t *residues, const uint64_t *
         blz_db_free(db);
         return BLZ_EDEVICE;
     }
+    db->h_offs.assign(offsets, offsets + nsubj + 1);
+    std::vector<int64_t> long_ids;
+    for (int64_t i = 0; i < nsubj; i++)
+        if (offsets[i + 1] - offsets[i] > (uint64_t)kSeedNarrowMax) long_ids.push_back(i);
+    db->n_long = (int64_t)long_ids.size();
+    if (!long_ids.empty() &&
+        (!check(ctx, hipMalloc((void **)&db->long_ids, sizeof(int64_t) * long_ids.size()), "long subjects") ||
+         !check(ctx, hipMemcpy(db->long_ids, long_ids.data(), sizeof(int64_t) * long_ids.size(), hipMemcpyHostToDevice),
+                "long subjects"))) {
+        blz_db_free(db);
+        return BLZ_EDEVICE;
+    }
     *out = db;
     return BLZ_OK;
 }
@@ -387,6 +417,7 @@ void blz_db_free(blz_db *db) {
     if (!db) return;
     if (db->res) (void)hipFree(db->res);
     if (db->offs) (void)hipFree(db->offs);
+    if (db->long_ids) (void)hipFree(db->long_ids);
     delete db;
 }
 
@@ -460,6 +491,7 @@ QueryDev query_dev(const blz_query *q) {
     d.qlen = q->qlen;
     const uint32_t ring = ring_words_for(q->qlen, q->params.window);
     d.ring_mask = ring - 1;
+    set_ring(d, q->qlen, q->params.window);
     d.claim_mask = std::min<uint32_t>(ring, 128) - 1;
     d.window = q->params.window;
     d.x_ungapped = q->params.x_ungapped;
@@ -533,41 +565,59 @@ struct SeedPlan {
     int table = 0;
     int waves = 0;
     int blocks = 0;
+    int first_pack = 0;     // index of the launch's first pack in the query table
+    size_t slice_off = 0;   // its slice bounds in the run's slice table
+    bool long_subjects = false;  // the launch over the database's long subjects
 };
 
+// Cuts subjects [0, nsubj) into at most n contiguous slices of about equal residues (at least
+// one subject each); appends the bounds (count + 1 values) to out.
+void slice_db(const blz_db *db, int64_t n, std::vector<int64_t> &out) {
+    const int64_t ns = db->nsubj;
+    n = std::max<int64_t>(1, std::min<int64_t>(n, ns));
+    const uint64_t R = db->nres;
+    out.push_back(0);
+    int64_t prev = 0;
+    for (int64_t k = 1; k < n; k++) {
+        const uint64_t target = (uint64_t)((unsigned __int128)R * (uint64_t)k / (uint64_t)n);
+        int64_t i = (int64_t)(std::lower_bound(db->h_offs.begin(), db->h_offs.begin() + ns, target) -
+                              db->h_offs.begin());
+        if (i > prev && i < ns) {
+            out.push_back(i);
+            prev = i;
+        }
+    }
+    out.push_back(ns);
+}
+
 bool plan_seed2(blz_ctx *ctx, const std::vector<QueryDev> &qs, const std::vector<int> &idx, int table,
-                SeedPlan &pl) {
+                SeedPlan &pl, bool long_subjects = false) {
     SeedArgs &sa = pl.sa;
     for (int i : idx) {
         sa.qlen = std::max(sa.qlen, qs[i].qlen);
         sa.ring_mask = std::max(sa.ring_mask, qs[i].ring_mask);
+        sa.ring_len = std::max(sa.ring_len, qs[i].ring_len);
         sa.claim_mask = std::max(sa.claim_mask, qs[i].claim_mask);
         sa.n_entries = std::max(sa.n_entries, qs[i].n_entries);
     }
     const size_t cap = ctx->lds_per_block;
-    int waves = ctx->tune.seed_waves;
-    if (!waves) {  // most resident waves per CU (LDS is per CU); fewer waves per group on ties
-        int best_total = 0;
-        for (int w = 2; w <= 16; w++) {
-            const size_t need = seed2_lds_bytes(sa, table, w);
-            if (need > cap) break;
-            const int total = std::min<int>(32, w * (int)(cap / need));
-            if (total > best_total) {
-                best_total = total;
-                waves = w;
-            }
+    // most resident waves per CU (registers and LDS); fewer waves per workgroup on ties
+    int waves = 0, per_cu = 0;
+    for (int w = 1; w <= 16; w++) {
+        if (ctx->tune.seed_waves && w != ctx->tune.seed_waves) continue;
+        if (seed2_lds_bytes(sa, table, w) > cap) break;
+        const int n = seed2_blocks_per_cu(sa, table, w, long_subjects);
+        if (n > 0 && w * n > waves * per_cu) {
+            waves = w;
+            per_cu = n;
         }
-        if (!waves) waves = 1;
     }
-    while (waves > 1 && seed2_lds_bytes(sa, table, waves) > cap) waves--;
-    if (seed2_lds_bytes(sa, table, waves) > cap) return false;
-    const size_t used = seed2_lds_bytes(sa, table, waves);
-    int per_cu = std::max<int>(1, (int)(cap / used));
-    per_cu = std::min(per_cu, std::max(1, 32 / waves));
+    if (!waves) return false;
     if (ctx->tune.seed_blocks_per_cu) per_cu = ctx->tune.seed_blocks_per_cu;
     pl.table = table;
     pl.waves = waves;
     pl.blocks = std::max(1, ctx->num_cus * per_cu);
+    pl.long_subjects = long_subjects;
     return true;
 }
 
@@ -575,12 +625,14 @@ bool plan_seed2(blz_ctx *ctx, const std::vector<QueryDev> &qs, const std::vector
 // for at least min_waves waves per workgroup.
 int pick_table(blz_ctx *ctx, const std::vector<QueryDev> &qs, int i, bool compact_ok) {
     const int want = ctx->tune.seed_table;  // 0 auto, else table + 1
-    for (int table = 2; table >= 0; table--) {
-        if (want && table != want - 1) continue;
-        if (table == 2 && !compact_ok) continue;
-        SeedPlan probe;
-        if (plan_seed2(ctx, qs, {i}, table, probe) && (probe.waves >= 4 || table == 0 || want)) return table;
+    if (want) {
+        if (want - 1 == 2 && !compact_ok) return 1;
+        return want - 1;
     }
+    // built-in choice: entries in LDS, prefix in global memory (L2-resident), as long as that
+    // leaves four waves per workgroup; the prefix in LDS costs more waves than it saves
+    SeedPlan probe;
+    if (plan_seed2(ctx, qs, {i}, 1, probe) && probe.waves >= 4) return 1;
     return 0;
 }
 
@@ -644,6 +696,11 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
             d.qlen = t.len;
             const uint32_t ring = ring_words_for(t.len, q0->params.window);
             d.ring_mask = ring - 1;
+            set_ring(d, t.len, q0->params.window);
+            if (d.ring_len > kRingLenMax) {
+                ctx->err = "query too long for the seed kernel's diagonal ring";
+                return BLZ_EINVAL;
+            }
             d.claim_mask = std::min<uint32_t>(ring, 128) - 1;
             d.n_members = (int32_t)pk.members.size();
             meta_off.push_back(meta.size());
@@ -665,9 +722,21 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
                 return BLZ_EINVAL;
             }
             pl.sa.nq = (int32_t)idx.size();
-            pl.sa.s_begin = nq + (int64_t)order.size();  // first pack of the bin in the query table
+            pl.first_pack = nq + (int)order.size();
             for (int k : idx) order.push_back(k);
             plans.push_back(pl);
+            if (db->n_long > 0) {  // the same packs over the long subjects, tables in global memory
+                SeedPlan lp;
+                if (!plan_seed2(ctx, pq, idx, 0, lp, true)) {
+                    ctx->err = "query too long for the diagonal ring in LDS";
+                    return BLZ_EINVAL;
+                }
+                lp.sa.nq = pl.sa.nq;
+                lp.first_pack = pl.first_pack;
+                const int64_t groups = (db->n_long + lp.waves - 1) / lp.waves * lp.sa.nq;
+                lp.blocks = (int)std::max<int64_t>(1, std::min<int64_t>(lp.blocks, groups));
+                plans.push_back(lp);
+            }
         }
         if (!ensure(ctx, ctx->pack_meta, ctx->pack_meta_cap, std::max<size_t>(meta.size(), 1), "pack table"))
             return BLZ_ENOMEM;
@@ -678,9 +747,21 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
             qs.push_back(d);
         }
     }
+    std::vector<int64_t> slices;
+    size_t long_ring = 0;
     for (auto &pl : plans) {
         max_waves = std::max(max_waves, pl.waves);
         max_blocks = std::max<size_t>(max_blocks, (size_t)pl.blocks * pl.waves);
+        if (variant == 1) continue;
+        pl.slice_off = slices.size();
+        if (pl.long_subjects) {  // one subject per wave and item
+            long_ring = std::max(long_ring, (size_t)pl.blocks * pl.waves * pl.sa.ring_len);
+            for (int64_t k = 0; k < db->n_long; k += pl.waves) slices.push_back(k);
+            slices.push_back(db->n_long);
+        } else {  // about eight items per workgroup
+            slice_db(db, ((int64_t)pl.blocks * 8 + pl.sa.nq - 1) / pl.sa.nq, slices);
+        }
+        pl.sa.n_slices = (uint32_t)(slices.size() - pl.slice_off - 1);
     }
     const size_t lds_lanes = (size_t)(ctx->tune.gapped_blocks ? ctx->tune.gapped_blocks : ctx->num_cus * 4) * 64;
     const size_t wide_lanes = (size_t)kGappedBlocks * 64;
@@ -698,7 +779,9 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
         !ensure(ctx, ctx->hsps, ctx->hsp_cap, std::max<size_t>(1 << 22, per_q), "hsps") ||
         !ensure(ctx, ctx->results, ctx->results_cap, work_cap, "results") ||
         !ensure(ctx, ctx->qdev, ctx->qdev_cap, qs.size(), "query table") ||
-        !ensure(ctx, ctx->item_ctr, ctx->item_ctr_cap, plans.size() + 1, "item counters"))
+        !ensure(ctx, ctx->item_ctr, ctx->item_ctr_cap, plans.size() + 1, "item counters") ||
+        !ensure(ctx, ctx->slices, ctx->slices_cap, std::max<size_t>(slices.size(), 1), "slices") ||
+        (long_ring && !ensure(ctx, ctx->long_ring, ctx->long_ring_cap, long_ring, "long-subject rings")))
         return BLZ_ENOMEM;
 
     hipStream_t st = ctx->stream;
@@ -706,6 +789,8 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
                "query table") ||
         (!meta.empty() && !check(ctx, hipMemcpyAsync(ctx->pack_meta, meta.data(), sizeof(uint32_t) * meta.size(),
                                                       hipMemcpyHostToDevice, st), "pack table")) ||
+        (!slices.empty() && !check(ctx, hipMemcpyAsync(ctx->slices, slices.data(), sizeof(int64_t) * slices.size(),
+                                                        hipMemcpyHostToDevice, st), "slices")) ||
         !check(ctx, hipMemsetAsync(ctx->ctr, 0, sizeof(DevCounters), st), "memset") ||
         !check(ctx, hipMemsetAsync(ctx->item_ctr, 0, sizeof(uint32_t) * (plans.size() + 1), st), "memset"))
         return BLZ_EDEVICE;
@@ -741,16 +826,14 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
             sa.ring_mask = d.ring_mask;
             e = launch_seed(sa, pl.blocks, pl.waves, st);
         } else {
-            const int first = (int)sa.s_begin;
-            sa.qd = ctx->qdev + first;
+            sa.qd = ctx->qdev + pl.first_pack;
             sa.s_begin = 0;
             sa.s_end = db->nsubj;
-            const int64_t target_items = (int64_t)pl.blocks * 8;
-            sa.subj_per_item = std::max<int64_t>(16, (db->nsubj * sa.nq + target_items - 1) / target_items);
-            sa.subj_per_item = std::min<int64_t>(sa.subj_per_item, std::max<int64_t>(db->nsubj, 1));
-            sa.items_per_query = (uint32_t)((db->nsubj + sa.subj_per_item - 1) / sa.subj_per_item);
+            sa.slice = ctx->slices + pl.slice_off;
+            sa.long_ring = ctx->long_ring;
+            sa.long_ids = db->long_ids;
             sa.item_ctr = ctx->item_ctr + pi;
-            e = launch_seed2(sa, pl.table, pl.blocks, pl.waves, st);
+            e = launch_seed2(sa, pl.table, pl.blocks, pl.waves, pl.long_subjects, st);
         }
         if (!check(ctx, e, "k_seed launch")) return BLZ_EDEVICE;
     }
@@ -765,7 +848,7 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
     ga.count_cap = (uint32_t)ctx->work_cap;
     ga.next = &ctr->next_work;
     ga.next_heavy = &ctr->next_heavy;
-    ga.heavy_classes = ctx->tune.gapped_wave_classes > 0 ? ctx->tune.gapped_wave_classes - 1 : 2;
+    ga.heavy_classes = ctx->tune.gapped_wave_classes > 0 ? ctx->tune.gapped_wave_classes - 1 : kBuckets;
     ga.redo = ctx->redo;
     ga.redo_cap = (uint32_t)ctx->redo_cap;
     ga.dp_scratch = ctx->dp;
@@ -795,7 +878,7 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
     if (!check(ctx, hipEventRecord(ctx->ev[1], st), "event") ||
         (nsubj > 0 && !check(ctx, launch_order(ctx->work, ctr, ctx->order, (uint32_t)ctx->work_cap, 512, st), "k_order launch")) ||
         (nsubj > 0 && ga.heavy_classes > 0 && !check(ctx, launch_gapped_wave(gh, heavy_blocks, st), "k_gapped_wave launch")) ||
-        (nsubj > 0 && !check(ctx, launch_gapped_lds(ga, lds_blocks, st), "k_gapped_lds launch")) ||
+        (nsubj > 0 && ga.heavy_classes < kBuckets && !check(ctx, launch_gapped_lds(ga, lds_blocks, st), "k_gapped_lds launch")) ||
         (nsubj > 0 && !check(ctx, launch_gapped_wide(gw, kGappedBlocks, 64, st), "k_gapped_wide launch")) ||
         !check(ctx, hipEventRecord(ctx->ev[2], st), "event"))
         return BLZ_EDEVICE;

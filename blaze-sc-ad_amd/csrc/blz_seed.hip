@@ -1,7 +1,7 @@
 // blz_seed.hip -- seed stage: word lookup, two-hit diagonal state, batched ungapped X-drop.
 //
-// Work items are (query, block of subjects), taken query-major from a device counter by
-// persistent workgroups.  A workgroup stages its item's query in LDS: letters, BLOSUM62 and,
+// Work items are (query pack, slice of subjects), taken pack-major from a device counter by
+// persistent workgroups; slices hold about equal residues.  A workgroup stages its item's query in LDS: letters, BLOSUM62 and,
 // depending on the table placement kTable, the lookup entries (u16 query offsets per word) and
 // the compact lookup prefix (u32 base per 64-word block + u16 offset per word):
 //   kTable 2: prefix and entries in LDS, 1: entries in LDS, prefix in global memory,
@@ -31,21 +31,49 @@ namespace blz {
 
 namespace {
 
-constexpr uint32_t kFlag = 0x80000000u;     // right extension done: low bits = barrier + 1
-constexpr uint32_t kPending = 0x40000000u;  // extension queued: low bits = task index
-constexpr uint32_t kPosMask = 0x3FFFFFFFu;
+// Diagonal ring entry: 0 empty; last hit s + 1; kFlag | (barrier + 1) after a right extension
+// (NCBI's flag and last_hit = s_last_off - 2); kPending | task while an extension is queued.
+// Positions are offsets in the current subject: the ring is cleared for every subject.  Rings
+// of 16-bit entries (LDS) take subjects of up to kNarrowMax letters, longer ones use a ring of
+// 32-bit entries in global memory.
+template <class R>
+struct RingFmt;
+template <>
+struct RingFmt<uint16_t> {
+    static constexpr uint32_t kFlag = 0x8000u, kPending = 0x4000u, kPosMask = 0x3FFFu;
+};
+template <>
+struct RingFmt<uint32_t> {
+    static constexpr uint32_t kFlag = 0x80000000u, kPending = 0x40000000u, kPosMask = 0x3FFFFFFFu;
+};
+constexpr int kNarrowMax = kSeedNarrowMax;  // entries hold at most slen - 1 below the flag bits
+
+// Orders the wave's ring accesses: LDS for the 16-bit ring; the 32-bit ring lives in global
+// memory, whose accesses need a workgroup-scope fence.
+template <class R>
+__device__ __forceinline__ void ring_sync() {
+    if constexpr (sizeof(R) == 4) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    } else {
+        wave_sync();
+    }
+}
 constexpr int kTaskCap = 128;
 constexpr int kDeferCap = 256;
 constexpr int kHitBuf = 128;                // hits of one step laid out in LDS
 constexpr int kSubjBuf = 1024;              // subject letters staged in LDS per wave
+static_assert(kSubjBuf <= kNarrowMax, "staged subjects use the LDS ring");
 
 struct Task {
     uint32_t q_slot;  // query offset | diagonal slot << 16
     uint32_t s_pack;  // subject offset of the second hit | end of the first hit's word << 16
 };
 
+template <class R>
 struct WaveState {
-    uint32_t *ring;
+    R *ring;
     uint32_t *claim;
     uint32_t *excl;
     uint32_t *p0;
@@ -59,7 +87,7 @@ struct WaveState {
 __host__ __device__ constexpr uint32_t al16(uint32_t b) { return (b + 15u) & ~15u; }
 
 __host__ __device__ inline uint32_t wave_lds_bytes(uint32_t M, uint32_t C) {
-    return al16(4 * (M + C + 68 + 64 + 4 + kHitBuf) + sizeof(Task) * kTaskCap + sizeof(uint32_t) * kDeferCap +
+    return al16(2 * M + 4 * (C + 68 + 64 + 4 + kHitBuf) + sizeof(Task) * kTaskCap + sizeof(uint32_t) * kDeferCap +
                 kSubjBuf);
 }
 
@@ -74,8 +102,7 @@ struct Ctx {
     const int16_t (*mat)[28];
     const uint8_t *Q;
     int qlen;
-    uint32_t abs0;
-    uint32_t ring_mask;
+    uint32_t ring_len, ring_base, ring_magic;
     uint32_t claim_mask;
     int window;
     int xdrop;
@@ -83,6 +110,12 @@ struct Ctx {
     InitHit *stage;
     int stage_cap;
 };
+
+// Ring slot of diagonal q - s: (q + ring_base - s) mod ring_len by multiply-high.
+__device__ __forceinline__ uint32_t ring_slot(const Ctx &c, int q, int s) {
+    const uint32_t x = (uint32_t)q + c.ring_base - (uint32_t)s;
+    return x - __umulhi(x, c.ring_magic) * c.ring_len;
+}
 
 // Ungapped walks over letters held in one address space (LDS or global), four letters per
 // step: the loads of a step are independent, so one memory latency covers four positions.
@@ -180,55 +213,64 @@ __device__ __forceinline__ int two_hit_walk(const Ctx &c, SP S, int slen, int s_
     return left_score > right_score ? left_score : right_score;
 }
 
-// Two-hit state machine for one hit (q, s).
-__device__ __forceinline__ void apply_hit(const Ctx &c, WaveState &w, int q, int s) {
-    const uint32_t slot = (uint32_t)(q - s) & c.ring_mask;
-    const uint32_t st = w.ring[slot];
-    const uint32_t sabs = c.abs0 + (uint32_t)s;
+// Two-hit state machine for one hit (q, s) whose diagonal slot holds st.
+template <class R>
+__device__ __forceinline__ void apply_hit(const Ctx &c, WaveState<R> &w, int q, int s, uint32_t slot, uint32_t st) {
+    constexpr uint32_t kFlag = RingFmt<R>::kFlag, kPending = RingFmt<R>::kPending, kPosMask = RingFmt<R>::kPosMask;
+    const uint32_t sabs = (uint32_t)s;
     if (st & kPending) {
         const uint32_t k = atomicAdd(&w.cnt[1], 1u);
         w.defer[k] = (uint32_t)q | ((uint32_t)s << 16);
         return;
     }
     if (st & kFlag) {
-        if (sabs >= (st & kPosMask) - 1u) w.ring[slot] = sabs + 1u;
+        if (sabs >= (st & kPosMask) - 1u) w.ring[slot] = (R)(sabs + 1u);
         return;
     }
     const uint32_t diff = st ? sabs - (st - 1u) : 0x7FFFFFFFu;
     if (diff >= (uint32_t)c.window) {
-        w.ring[slot] = sabs + 1u;
+        w.ring[slot] = (R)(sabs + 1u);
         return;
     }
     if (diff < (uint32_t)kWord) return;
     const uint32_t t = atomicAdd(&w.cnt[0], 1u);
-    w.task[t] = Task{(uint32_t)q | (slot << 16), (uint32_t)s | ((st - 1u - c.abs0 + kWord) << 16)};
-    w.ring[slot] = kPending | t;
+    w.task[t] = Task{(uint32_t)q | (slot << 16), (uint32_t)s | ((st - 1u + kWord) << 16)};
+    w.ring[slot] = (R)(kPending | t);
 }
 
 // Applies up to one hit per lane; same-diagonal hits go earliest-rank first.
-__device__ __forceinline__ void resolve_round(const Ctx &c, WaveState &w, bool pend, int q, int s, uint32_t rank) {
-    const uint32_t h = (uint32_t)(q - s) & c.claim_mask;
+// The claim and the diagonal's state are read in the same pass: a winner's state is current,
+// since no lane writes a ring slot before the claims are settled, and a loser reads again.
+template <class R>
+__device__ __forceinline__ void resolve_round(const Ctx &c, WaveState<R> &w, bool pend, int q, int s, uint32_t rank) {
+    const uint32_t slot = ring_slot(c, q, s);
+    const uint32_t h = slot & c.claim_mask;
     while (__ballot(pend)) {
-        if (pend) atomicMin(&w.claim[h], rank);
-        wave_sync();
+        uint32_t st = 0;
+        if (pend) {
+            atomicMin(&w.claim[h], rank);
+            st = w.ring[slot];
+        }
+        ring_sync<R>();
         const bool win = pend && w.claim[h] == rank;
-        wave_sync();
+        ring_sync<R>();
         if (win) {
             w.claim[h] = 0xFFFFFFFFu;
             pend = false;
-            apply_hit(c, w, q, s);
+            apply_hit(c, w, q, s, slot, st);
         }
-        wave_sync();
+        ring_sync<R>();
     }
 }
 
 // Extends every queued pair (one lane each), writes the outcome to its diagonal, then replays
 // the hits queued behind pending diagonals; repeats until nothing is pending.
-template <class SP>
-__device__ void flush(const Ctx &c, WaveState &w, SP S, int slen, unsigned long long &n_ext) {
+template <class R, class SP>
+__device__ void flush(const Ctx &c, WaveState<R> &w, SP S, int slen, unsigned long long &n_ext) {
+    constexpr uint32_t kFlag = RingFmt<R>::kFlag;
     const int lane = lane_id();
     for (;;) {
-        wave_sync();
+        ring_sync<R>();
         const uint32_t nt = w.cnt[0];
         const uint32_t nd = w.cnt[1];
         if (nt == 0 && nd == 0) break;
@@ -246,12 +288,12 @@ __device__ void flush(const Ctx &c, WaveState &w, SP S, int slen, unsigned long 
                     const uint32_t k = atomicAdd(&w.cnt[2], 1u);
                     if (k < (uint32_t)c.stage_cap) c.stage[k] = InitHit{hq, hs, hl, score};
                 }
-                w.ring[slot] = rext ? ((c.abs0 + (uint32_t)slast - 2u + 1u) | kFlag) : (c.abs0 + (uint32_t)ts + 1u);
+                w.ring[slot] = (R)(rext ? (((uint32_t)slast - 2u + 1u) | kFlag) : ((uint32_t)ts + 1u));
             }
         }
-        wave_sync();
+        ring_sync<R>();
         if (lane == 0) w.cnt[0] = 0;
-        wave_sync();
+        ring_sync<R>();
         // replay the queued hits [0, nd) in order; new deferrals land at nd and beyond
         for (uint32_t r = 0; r < nd; r += 64) {
             const uint32_t i = r + lane;
@@ -264,10 +306,10 @@ __device__ void flush(const Ctx &c, WaveState &w, SP S, int slen, unsigned long 
             }
             resolve_round(c, w, pend, q, s, i);
         }
-        wave_sync();
+        ring_sync<R>();
         const uint32_t nd2 = w.cnt[1];
         for (uint32_t i = nd + lane; i < nd2; i += 64) w.defer[i - nd] = w.defer[i];
-        wave_sync();
+        ring_sync<R>();
         if (lane == 0) w.cnt[1] = nd2 - nd;
     }
 }
@@ -292,8 +334,8 @@ __device__ __forceinline__ void word_range(const Tables &t, uint32_t wd, uint32_
 }
 
 // Seed scan of one subject whose letters are read through S (LDS buffer or global memory).
-template <int kTable, class SP>
-__device__ void scan_subject(const Ctx &c, WaveState &w, const Tables &t, SP S, int slen, unsigned long long &n_hits,
+template <int kTable, class R, class SP>
+__device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP S, int slen, unsigned long long &n_hits,
                              unsigned long long &n_ext) {
     const int lane = lane_id();
     for (int c0 = 0; c0 + kWord <= slen; c0 += 64) {
@@ -345,7 +387,8 @@ __device__ void scan_subject(const Ctx &c, WaveState &w, const Tables &t, SP S, 
 
 // The subject's init hits (staged by this wave) go to the pool, one work item per query of
 // the item's pack that has any.  Query offsets become offsets in that query.
-__device__ void emit_init_hits(const SeedArgs &a, const QueryDev &qd, WaveState &w, const Ctx &c, int64_t sid,
+template <class R>
+__device__ void emit_init_hits(const SeedArgs &a, const QueryDev &qd, WaveState<R> &w, const Ctx &c, int64_t sid,
                                uint32_t nih) {
     const int lane = lane_id();
     const int nm = qd.n_members;
@@ -443,15 +486,19 @@ __device__ void emit_init_hits(const SeedArgs &a, const QueryDev &qd, WaveState 
 
 }  // namespace
 
-template <int kTable>
-__global__ void __launch_bounds__(1024) k_seed2(SeedArgs a) {
+// kLong: the launch over the subjects longer than kNarrowMax (a.long_ids), with diagonal rings
+// of 32-bit entries in global memory; the other launches skip those subjects.
+// At least six waves per SIMD (80 VGPRs): the kernel is bound by LDS latency, so resident
+// waves are what hides it.
+template <int kTable, bool kLong>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8))) k_seed2(SeedArgs a) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;
     const int nwaves = blockDim.x >> 6;
-    // LDS is sized for the largest query of the launch (a.qlen, a.ring_mask, a.claim_mask,
+    // LDS is sized for the largest query of the launch (a.qlen, a.ring_len, a.claim_mask,
     // a.n_entries); each item uses its own query's sizes
-    const uint32_t M = a.ring_mask + 1;
+    const uint32_t M = a.ring_len;
     const uint32_t C = a.claim_mask + 1;
 
     uint32_t off = 0;
@@ -475,9 +522,9 @@ __global__ void __launch_bounds__(1024) k_seed2(SeedArgs a) {
         off += al16(kRelBlocks * 4);
     }
     uint8_t *wb = smem + off + (size_t)wave * wave_lds_bytes(M, C);
-    WaveState w;
-    w.ring = reinterpret_cast<uint32_t *>(wb);
-    w.claim = w.ring + M;
+    WaveState<uint16_t> w;  // LDS ring of 16-bit entries
+    w.ring = reinterpret_cast<uint16_t *>(wb);
+    w.claim = reinterpret_cast<uint32_t *>(w.ring + M);
     w.excl = w.claim + C;
     w.p0 = w.excl + 68;
     w.cnt = w.p0 + 64;
@@ -487,20 +534,28 @@ __global__ void __launch_bounds__(1024) k_seed2(SeedArgs a) {
     w.subj = reinterpret_cast<uint8_t *>(w.defer + kDeferCap);
 
     for (int i = threadIdx.x; i < 28 * 28; i += blockDim.x) mat[i / 28][i % 28] = kBlosum62Dev[i / 28][i % 28];
-    for (uint32_t i = lane; i < M; i += 64) w.ring[i] = 0;
     for (uint32_t i = lane; i < C; i += 64) w.claim[i] = 0xFFFFFFFFu;
     if (lane < 4) w.cnt[lane] = 0;
 
     const uint64_t gwave = (uint64_t)blockIdx.x * nwaves + wave;
+    WaveState<uint32_t> wl;  // the same lists with a ring of 32-bit entries in global memory
+    wl.ring = kLong ? a.long_ring + gwave * M : nullptr;
+    wl.claim = w.claim;
+    wl.excl = w.excl;
+    wl.p0 = w.p0;
+    wl.cnt = w.cnt;
+    wl.hits = w.hits;
+    wl.task = w.task;
+    wl.defer = w.defer;
+    wl.subj = w.subj;
     Ctx c;
     c.mat = mat;
     c.Q = Qs;
     c.stage = a.stage + gwave * (uint64_t)a.stage_cap;
     c.stage_cap = a.stage_cap;
-    uint32_t abs_base = 0;  // wave-relative position: increases over every subject the wave sees
     unsigned long long n_hits = 0, n_ext = 0;
     int cur_q = -1;
-    const uint32_t n_items = (uint32_t)a.nq * a.items_per_query;
+    const uint32_t n_items = (uint32_t)a.nq * a.n_slices;
     Tables t{};
 
     for (;;) {
@@ -512,9 +567,12 @@ __global__ void __launch_bounds__(1024) k_seed2(SeedArgs a) {
         __syncthreads();
         const uint32_t item = (uint32_t)blk[0];
         if (item >= n_items) break;
-        const int qi = (int)(item / a.items_per_query);
-        const int64_t sub0 = a.s_begin + (int64_t)(item % a.items_per_query) * a.subj_per_item;
-        const int64_t sub1 = sub0 + a.subj_per_item < a.s_end ? sub0 + a.subj_per_item : a.s_end;
+        const int qi = (int)(item / a.n_slices);
+        // a length-sorted database ends with its longest subjects: take those first, so the
+        // launch drains on short ones
+        const uint32_t j = a.n_slices - 1u - item % a.n_slices;
+        const int64_t sub0 = a.slice[j];
+        const int64_t sub1 = a.slice[j + 1];
         const QueryDev &qd = a.qd[qi];
         if (qi != cur_q) {  // stage this query
             for (int i = threadIdx.x; i < qd.qlen; i += blockDim.x) Qs[i] = qd.query[i];
@@ -531,7 +589,9 @@ __global__ void __launch_bounds__(1024) k_seed2(SeedArgs a) {
             cur_q = qi;
         }
         c.qlen = qd.qlen;
-        c.ring_mask = qd.ring_mask;
+        c.ring_len = qd.ring_len;
+        c.ring_base = qd.ring_base;
+        c.ring_magic = qd.ring_magic;
         c.claim_mask = qd.claim_mask;
         c.window = qd.window;
         c.xdrop = qd.x_ungapped;
@@ -542,34 +602,44 @@ __global__ void __launch_bounds__(1024) k_seed2(SeedArgs a) {
         t.ent = kTable >= 1 ? ent_l : qd.entries;
 
         for (;;) {
-            int64_t sid = 0;
-            if (lane == 0) sid = sub0 + (int64_t)atomicAdd(reinterpret_cast<uint32_t *>(&blk[1]), 1u);
-            sid = (int64_t)first_lane64((uint64_t)sid);
-            if (sid >= sub1) break;
+            int64_t k = 0;
+            if (lane == 0) k = sub0 + (int64_t)atomicAdd(reinterpret_cast<uint32_t *>(&blk[1]), 1u);
+            k = (int64_t)first_lane64((uint64_t)k);
+            if (k >= sub1) break;
+            const int64_t sid = kLong ? a.long_ids[k] : k;
             const uint64_t o0 = a.offs[sid];
             const int slen = (int)(a.offs[sid + 1] - o0);
             const uint8_t *Sg = a.res + o0;
-            c.abs0 = abs_base;
-            abs_base += (uint32_t)slen + 128u;
-            if (slen <= kSubjBuf) {
-                // stage the subject: 16-byte loads from a 16-byte aligned start, then realign
-                const uint64_t a0 = o0 & ~uint64_t(15);
-                const int lead = (int)(o0 - a0);
-                const int nvec = (lead + slen + 15) >> 4;
-                const uint4 *src = reinterpret_cast<const uint4 *>(a.res + a0);
-                for (int v = lane; v < nvec; v += 64) {
-                    const uint4 x = src[v];
-                    const uint32_t xw[4] = {x.x, x.y, x.z, x.w};
-#pragma unroll
-                    for (int k = 0; k < 16; k++) {
-                        const int p = v * 16 + k - lead;
-                        if (p >= 0 && p < slen) w.subj[p] = (uint8_t)(xw[k >> 2] >> (8 * (k & 3)));
-                    }
-                }
-                wave_sync();
-                scan_subject<kTable>(c, w, t, (const uint8_t *)w.subj, slen, n_hits, n_ext);
+            const uint32_t Mq = c.ring_len;
+            if constexpr (kLong) {
+                for (uint32_t i = lane; i < Mq; i += 64) wl.ring[i] = 0;
+                ring_sync<uint32_t>();
+                scan_subject<kTable>(c, wl, t, Sg, slen, n_hits, n_ext);
             } else {
-                scan_subject<kTable>(c, w, t, Sg, slen, n_hits, n_ext);
+                if (slen > kNarrowMax) continue;  // the long-subject launch takes it
+                uint4 *r4 = reinterpret_cast<uint4 *>(w.ring);
+                for (uint32_t i = lane; i < Mq / 8; i += 64) r4[i] = uint4{0, 0, 0, 0};
+                if (slen <= kSubjBuf) {
+                    // stage the subject: 16-byte loads from a 16-byte aligned start, then realign
+                    const uint64_t a0 = o0 & ~uint64_t(15);
+                    const int lead = (int)(o0 - a0);
+                    const int nvec = (lead + slen + 15) >> 4;
+                    const uint4 *src = reinterpret_cast<const uint4 *>(a.res + a0);
+                    for (int v = lane; v < nvec; v += 64) {
+                        const uint4 x = src[v];
+                        const uint32_t xw[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                        for (int k = 0; k < 16; k++) {
+                            const int p = v * 16 + k - lead;
+                            if (p >= 0 && p < slen) w.subj[p] = (uint8_t)(xw[k >> 2] >> (8 * (k & 3)));
+                        }
+                    }
+                    wave_sync();
+                    scan_subject<kTable>(c, w, t, (const uint8_t *)w.subj, slen, n_hits, n_ext);
+                } else {
+                    wave_sync();
+                    scan_subject<kTable>(c, w, t, Sg, slen, n_hits, n_ext);
+                }
             }
 
             // hand the subject's init hits to the gapped stage
@@ -598,17 +668,36 @@ __global__ void __launch_bounds__(1024) k_seed2(SeedArgs a) {
 
 size_t seed2_lds_bytes(const SeedArgs &a, int table, int waves) {
     return block_lds_bytes(table, (uint32_t)a.qlen, a.n_entries) +
-           (size_t)waves * wave_lds_bytes(a.ring_mask + 1, a.claim_mask + 1);
+           (size_t)waves * wave_lds_bytes(a.ring_len, a.claim_mask + 1);
 }
 
-hipError_t launch_seed2(const SeedArgs &a, int table, int blocks, int waves, hipStream_t st) {
+namespace {
+template <int kTable, bool kLong>
+int occupancy_of(int waves, size_t lds) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_seed2<kTable, kLong>, 64 * waves, lds) != hipSuccess) return 0;
+    return n;
+}
+}  // namespace
+
+int seed2_blocks_per_cu(const SeedArgs &a, int table, int waves, bool long_subjects) {
     const size_t lds = seed2_lds_bytes(a, table, waves);
-    if (table == 2)
-        hipLaunchKernelGGL(k_seed2<2>, dim3(blocks), dim3(64 * waves), lds, st, a);
+    if (long_subjects) return occupancy_of<0, true>(waves, lds);
+    if (table == 2) return occupancy_of<2, false>(waves, lds);
+    if (table == 1) return occupancy_of<1, false>(waves, lds);
+    return occupancy_of<0, false>(waves, lds);
+}
+
+hipError_t launch_seed2(const SeedArgs &a, int table, int blocks, int waves, bool long_subjects, hipStream_t st) {
+    const size_t lds = seed2_lds_bytes(a, table, waves);
+    if (long_subjects)
+        hipLaunchKernelGGL((k_seed2<0, true>), dim3(blocks), dim3(64 * waves), lds, st, a);
+    else if (table == 2)
+        hipLaunchKernelGGL((k_seed2<2, false>), dim3(blocks), dim3(64 * waves), lds, st, a);
     else if (table == 1)
-        hipLaunchKernelGGL(k_seed2<1>, dim3(blocks), dim3(64 * waves), lds, st, a);
+        hipLaunchKernelGGL((k_seed2<1, false>), dim3(blocks), dim3(64 * waves), lds, st, a);
     else
-        hipLaunchKernelGGL(k_seed2<0>, dim3(blocks), dim3(64 * waves), lds, st, a);
+        hipLaunchKernelGGL((k_seed2<0, false>), dim3(blocks), dim3(64 * waves), lds, st, a);
     return hipGetLastError();
 }
 

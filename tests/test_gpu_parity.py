@@ -111,6 +111,22 @@ def test_repeats_long_subjects(engine):
     compare(engine, q, res, offs)
 
 
+def test_long_subject_rings(engine):
+    """Subjects on both sides of the longest length the LDS diagonal ring takes (16,383 letters),
+    and far beyond it, with copies of the query near their starts, middles and ends."""
+    q = blz.synth_query(41, 700)
+    subs = []
+    for k, L in enumerate([16383, 16384, 16390, 40000, 65535]):
+        s = blz.synth_query(500 + k, L).copy()
+        for pos in (5, L // 2, L - len(q) - 1):
+            s[pos:pos + len(q)] = q
+        subs.append(s)
+    subs.append(blz.synth_query(600, 300))
+    offs = np.zeros(len(subs) + 1, dtype=np.uint64)
+    offs[1:] = np.cumsum([len(s) for s in subs])
+    compare(engine, q, np.concatenate(subs), offs)
+
+
 def test_empty_db(engine):
     q = blz.synth_query(1, 100)
     res = np.zeros(0, np.uint8)
