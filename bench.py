@@ -1,9 +1,19 @@
-"""Benchmark: BLASTP prelim stage, one 500-aa query against a 2,000,000-subject synthetic DB.
+"""Benchmark: BLASTP prelim stage over a resident synthetic database.  Prints one JSON line.
 
-A step is one full prelim search of the query over the resident database (seed scan, two-hit,
-ungapped X-drop, gapped stage, HSP lists back on the host).  Prints one JSON line.
+--workload scan (default): one 500-aa query against a 2,000,000-subject DB.  A step is one full
+  prelim search of the query (seed scan, two-hit, ungapped X-drop, gapped stage, HSP lists back
+  on the host).
+--workload batch: 256 queries (log-uniform 50-3,000 aa) against the same DB in one batched run
+  per step (packs of short queries, one persistent launch per query-size bin, replayed as a
+  hipGraph), against the same 256 queries run one query per launch.
+--workload stress: 64 queries of 1,000-2,000 aa against 500,000 long subjects (median 800, up
+  to 35,000) with 20% homologs at 30-45% identity and low-complexity repeat blocks: GCUPS of the
+  semi-gapped stage, and bit-exact lists on a sample against the CPU restatement.
+--workload serve: Poisson query streams (lengths log-normal, median 350, sigma 0.8, clipped
+  30-4,000) at several rates against a resident DB, through the serving loop with continuous
+  batching, and through the same loop with one query per launch: p50/p99 latency, residues/s.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--subjects N] [--no-cpu]
+    python bench.py [--workload scan|batch|stress|serve] [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -91,6 +101,11 @@ def copy_calibration(nbytes: int) -> int:
 
 def main() -> int:
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="scan", choices=["scan", "batch", "stress", "serve"])
+    ap.add_argument("--rates", default="20,100,200", help="serve: arrival rates, queries/s")
+    ap.add_argument("--duration", type=float, default=4.0, help="serve: seconds of arrivals per rate")
+    ap.add_argument("--autotune", action="store_true", help="batch: tune on a sample batch first")
+    ap.add_argument("--queries", type=int, default=0, help="batch/stress: number of queries (0: 256 / 64)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
@@ -110,7 +125,16 @@ def main() -> int:
         print("bench.py measures one GPU: every GPU box here has one, and the path shards by database "
               "subsets with no collective in the hot loop; run with --gpus 1", file=sys.stderr)
         return 2
+    if args.workload == "batch":
+        return run_batch(args)
+    if args.workload == "stress":
+        return run_stress(args)
+    if args.workload == "serve":
+        return run_serve(args)
+    return run_scan(args)
 
+
+def run_scan(args) -> int:
     t0 = time.perf_counter()
     q, res, offs, hom = build_workload(args.subjects, args.qlen, args.seed)
     n_sub, n_res = len(offs) - 1, int(offs[-1])
@@ -164,7 +188,7 @@ def main() -> int:
         "metric": METRIC, "value": value, "unit": "residues/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "int32", "data": "synthetic",
-        "config": {"workload": "config1: one 500-aa query vs 2,000,000-subject length-sorted DB "
+        "config": {"workload": "one 500-aa query vs 2,000,000-subject length-sorted DB "
                                "(log-normal median 300, sigma 0.8, clip 30-5000), 1% homologs at 35-70% "
                                "identity, seed 0; one launch per stage",
                    "subjects": n_sub, "residues": n_res, "query_len": args.qlen},
@@ -176,6 +200,240 @@ def main() -> int:
         "roofline": {"kernel": kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def _same_lists(a: "blz.Result", b: "blz.Result") -> bool:
+    return np.array_equal(a.oids, b.oids) and np.array_equal(a.counts, b.counts) and np.array_equal(a.hsps, b.hsps)
+
+
+def run_batch(args) -> int:
+    """256 queries, log-uniform 50-3,000 aa, against the scan workload's DB: batched and replayed
+    as a hipGraph versus one query per launch."""
+    nq = args.queries or 256
+    rng = np.random.default_rng(args.seed)
+    lens = np.exp(rng.uniform(np.log(50), np.log(3000), nq)).astype(int)
+    queries = [blz.synth_query(1000 + i, int(L)) for i, L in enumerate(lens)]
+    t0 = time.perf_counter()
+    q0, res, offs, hom = build_workload(args.subjects, args.qlen, args.seed)
+    n_sub, n_res = len(offs) - 1, int(offs[-1])
+    log(f"workload built: {nq} queries ({int(lens.sum())} letters), {n_sub} subjects, {n_res} residues "
+        f"({time.perf_counter() - t0:.1f} s)")
+    with blz.Engine(0) as eng:
+        if args.tune:
+            eng.set_tuning(**json.loads(args.tune))
+        eng.upload_db(res, offs)
+        t = time.perf_counter()
+        qhs = [eng.prepare(q) for q in queries]
+        prep_ms = 1000 * (time.perf_counter() - t)
+        tuned = None
+        if args.autotune:
+            import blz_tune
+            sample = qhs[:: max(1, nq // 32)]
+            t = time.perf_counter()
+            tuned, tuned_ms = blz_tune.tune(eng, sample, log=log)
+            log(f"autotune on {len(sample)} queries: {tuned} ({tuned_ms:.1f} ms; {time.perf_counter() - t:.1f} s)")
+        for i in range(args.warmup):  # the first run captures the graph
+            got, st = eng.run_batch(qhs)
+            log(f"warmup {i}: {st['ms_total']:.1f} ms device, {st['seed_launches']} seed launches")
+        lat = []
+        for i in range(args.steps):
+            t = time.perf_counter()
+            got, st = eng.run_batch(qhs)
+            lat.append(time.perf_counter() - t)
+            log(f"step {i}: {1000 * lat[-1]:.1f} ms ({st['ms_total']:.1f} ms device)")
+        tuning = eng.get_tuning()
+        # the plain form: one query per launch, no packing, no graph
+        eng.set_tuning(**{**tuning, "query_packing": 1, "graph": 1})
+        t = time.perf_counter()
+        plain, plain_dev = [], 0.0
+        for qh in qhs:
+            r, s1 = eng.run(qh)
+            plain.append(r)
+            plain_dev += s1["ms_total"]
+        plain_s = time.perf_counter() - t
+        exact = all(_same_lists(a, b) for a, b in zip(got, plain))
+        for qh in qhs:
+            eng.free_query(qh)
+    step_s = float(np.mean(lat))
+    value = n_res * nq / step_s
+    plain_value = n_res * nq / plain_s
+    log(f"batched {value:.3e} residues/s, plain {plain_value:.3e}; lists identical: {exact}")
+    out = {
+        "metric": METRIC, "value": value, "unit": "residues/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1000 * step_s, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"{nq} queries, log-uniform 50-3,000 aa (seed {args.seed}), vs the 2,000,000-subject "
+                               "DB of the scan workload; one batched run per step, replayed as a hipGraph",
+                   "queries": nq, "query_letters": int(lens.sum()), "subjects": n_sub, "residues": n_res},
+        "device_ms_per_step": st["ms_total"], "seed_launches": st["seed_launches"], "prepare_ms": prep_ms,
+        "plain": {"value": plain_value, "ms": 1000 * plain_s, "device_ms": plain_dev,
+                  "form": "one blz_prelim_run per query, no packing, no graph"},
+        "speedup_vs_plain": value / plain_value, "bit_exact_vs_plain": exact,
+        "latency_ms": {"p50": 1000 * float(np.percentile(lat, 50)), "p99": 1000 * float(np.percentile(lat, 99))},
+        "tuning": tuning, "autotuned": tuned is not None, "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def run_stress(args) -> int:
+    """64 queries of 1,000-2,000 aa against 500,000 long subjects with repeats: GCUPS of the
+    semi-gapped stage and bit-exact lists on sampled queries against the CPU restatement."""
+    nq = args.queries or 64
+    rng = np.random.default_rng(args.seed)
+    lens = rng.integers(1000, 2001, nq)
+    queries = [blz.synth_query(2000 + i, int(L)) for i, L in enumerate(lens)]
+    spec = blz.synth_spec(seed=args.seed, n_subjects=args.subjects if args.subjects != 2_000_000 else 500_000,
+                          len_median=800, len_sigma=0.8, len_min=30, len_max=35000, homolog_frac=0.2,
+                          ident_min=0.30, ident_max=0.45, repeat_frac=0.05)
+    t0 = time.perf_counter()
+    res, offs, hom = blz.synth_db(spec, queries)
+    n_sub, n_res = len(offs) - 1, int(offs[-1])
+    log(f"workload built: {nq} queries, {n_sub} subjects, {n_res} residues, {int((hom >= 0).sum())} homologs "
+        f"({time.perf_counter() - t0:.1f} s)")
+    with blz.Engine(0) as eng:
+        if args.tune:
+            eng.set_tuning(**json.loads(args.tune))
+        eng.upload_db(res, offs)
+        qhs = [eng.prepare(q) for q in queries]
+        for i in range(args.warmup):
+            got, st = eng.run_batch(qhs, allow_overflow=True, fetch=False)
+        eng.set_tuning(**{**eng.get_tuning(), "graph": 1})  # separate stage times
+        lat, seeds, gapped, cells = [], [], [], []
+        for i in range(args.steps):
+            t = time.perf_counter()
+            got, st = eng.run_batch(qhs, allow_overflow=True)
+            lat.append(time.perf_counter() - t)
+            seeds.append(st["ms_seed"])
+            gapped.append(st["ms_gapped"])
+            cells.append(st["dp_cells"])
+            log(f"step {i}: seed {st['ms_seed']:.1f} ms gapped {st['ms_gapped']:.1f} ms, {st['dp_cells']} cells, "
+                f"{st['overflow_subjects']} overflow pairs")
+        overflow = eng.overflow_pairs()
+        tuning = eng.get_tuning()
+        for qh in qhs:
+            eng.free_query(qh)
+    exact, cpu = None, None
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle
+        threads = min(16, os.cpu_count() or 1)
+        sample = [0, nq // 2]
+        t = time.perf_counter()
+        exact = True
+        done_res = 0
+        for qi in sample:
+            counts, hsps, _ = oracle.prelim_db(queries[qi], res, offs, nthreads=threads)
+            want = oracle.as_lists(counts, hsps)
+            have = got[qi].as_dict()
+            ovf = {s for (q, s) in overflow if q == qi}
+            exact &= sorted(set(want) - ovf) == sorted(set(have) - ovf) and all(
+                np.array_equal(want[k], have[k]) for k in want if k not in ovf)
+            done_res += n_res
+        dt = time.perf_counter() - t
+        cpu = {"value": done_res / dt, "unit": "residues/s", "cores": threads, "kind": "port",
+               "sample": f"queries {sample} of the batch against the whole DB ({done_res} query-residue pairs, "
+                         f"{dt:.1f} s), CPU restatement of the NCBI prelim stage (oracle/) on {threads} threads"}
+        log(f"cpu {cpu['value']:.3e} residues/s; GPU lists of queries {sample} bit-exact: {exact}")
+    step_s = float(np.mean(lat))
+    gapped_ms = float(np.mean(gapped))
+    out = {
+        "metric": METRIC, "value": n_res * nq / step_s, "unit": "residues/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1000 * step_s, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"{nq} queries of 1,000-2,000 aa vs {n_sub} subjects (log-normal median 800, sigma 0.8, "
+                               "clip 30-35,000), 20% homologs at 30-45% identity, 5% of residues in repeat blocks "
+                               "of period 3-10; one batched run per step",
+                   "queries": nq, "subjects": n_sub, "residues": n_res},
+        "gcups": float(np.mean(cells)) / (gapped_ms * 1e-3) / 1e9 if gapped_ms > 0 else None,
+        "dp_cells": int(np.mean(cells)), "kernel_ms": {"k_seed": float(np.mean(seeds)), "k_gapped": gapped_ms},
+        "overflow_pairs": len(overflow), "bit_exact_sample": exact, "tuning": tuning,
+        "roofline": None, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def _play(srv, queries, arrivals):
+    """Submits queries[i] at arrivals[i] seconds from now (from a client thread) and waits for all;
+    returns per-request info dicts and the makespan."""
+    import threading
+
+    tickets = [None] * len(queries)
+    t0 = time.perf_counter()
+
+    def client():
+        for i, a in enumerate(arrivals):
+            d = t0 + a - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets[i] = srv.submit(queries[i])
+
+    th = threading.Thread(target=client)
+    th.start()
+    th.join()
+    infos = []
+    for tk in tickets:
+        infos.append(srv.wait(tk))
+        srv.release(tk)
+    return infos, time.perf_counter() - t0
+
+
+def run_serve(args) -> int:
+    """Poisson arrivals at each rate, served with continuous batching and with one query per
+    launch, on the same trace."""
+    rates = [float(r) for r in args.rates.split(",")]
+    n_subjects = args.subjects if args.subjects != 2_000_000 else 1_000_000
+    rng = np.random.default_rng(args.seed)
+    spec = blz.synth_spec(seed=args.seed, n_subjects=n_subjects, len_median=300, len_sigma=0.8, len_min=30,
+                          len_max=5000, homolog_frac=0.01, ident_min=0.35, ident_max=0.70)
+    traces = []
+    for k, rate in enumerate(rates):
+        n = max(1, int(rate * args.duration))
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+        lens = np.clip(np.exp(rng.normal(np.log(350), 0.8, n)), 30, 4000).astype(int)
+        traces.append((rate, arrivals, [blz.synth_query(5000 + 1000 * k + i, int(L)) for i, L in enumerate(lens)]))
+    t0 = time.perf_counter()
+    res, offs, hom = blz.synth_db(spec, traces[0][2][:50])
+    n_sub, n_res = len(offs) - 1, int(offs[-1])
+    log(f"DB built: {n_sub} subjects, {n_res} residues ({time.perf_counter() - t0:.1f} s)")
+    forms = {"batched": dict(slots=3, max_batch=64, max_batch_letters=32768),
+             "plain": dict(slots=1, max_batch=1)}
+    results = []
+    with blz.Engine(0) as eng:
+        tune = json.loads(args.tune) if args.tune else None
+        eng.upload_db(res, offs)
+        for rate, arrivals, queries in traces:
+            row = {"rate_qps": rate, "queries": len(queries), "letters": int(sum(len(q) for q in queries))}
+            for form, cfg in forms.items():
+                with blz.Server(eng, tuning=tune, **cfg) as srv:
+                    warm = (queries * 200)[: 3 * 64]  # three full batches: every slot sizes its buffers
+                    _play(srv, warm, np.zeros(len(warm)))
+                    infos, span = _play(srv, queries, arrivals)
+                    st = srv.stats()
+                lat = np.array([i["latency_us"] for i in infos]) / 1000.0
+                row[form] = {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                             "residues_per_s": len(queries) * n_res / span, "makespan_s": span,
+                             "mean_batch": float(np.mean([i["batch_queries"] for i in infos])),
+                             "device_ms": st["device_ms"], "failed": int(sum(i["status"] != 0 for i in infos))}
+                log(f"rate {rate:g} q/s {form}: p50 {row[form]['p50_ms']:.1f} ms p99 {row[form]['p99_ms']:.1f} ms, "
+                    f"{row[form]['residues_per_s']:.3e} residues/s, mean batch {row[form]['mean_batch']:.1f}")
+            results.append(row)
+    top = results[-1]
+    out = {
+        "metric": METRIC, "value": top["batched"]["residues_per_s"], "unit": "residues/s", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "ms_per_step": 1000 * top["batched"]["makespan_s"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"Poisson query streams at {args.rates} q/s for {args.duration:g} s each (lengths "
+                               f"log-normal median 350, sigma 0.8, clip 30-4,000) vs a resident {n_sub}-subject DB "
+                               f"({n_res} residues); serving loop with continuous batching (3 slots, <= 64 queries) "
+                               "against the same loop with one query per launch",
+                   "subjects": n_sub, "residues": n_res},
+        "latency_ms": {"p50": top["batched"]["p50_ms"], "p99": top["batched"]["p99_ms"]},
+        "rates": results, "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
     return 0

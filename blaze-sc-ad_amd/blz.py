@@ -55,7 +55,23 @@ class RunStats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("seed_variant", C.c_int32), ("seed_waves", C.c_int32), ("seed_blocks_per_cu", C.c_int32),
                 ("gapped_blocks", C.c_int32), ("gapped_threads", C.c_int32), ("seed_table", C.c_int32),
-                ("gapped_wave_classes", C.c_int32), ("query_packing", C.c_int32)]
+                ("gapped_wave_classes", C.c_int32), ("query_packing", C.c_int32), ("pack_len", C.c_int32),
+                ("bin_ring", C.c_int32), ("graph", C.c_int32)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("max_batch", C.c_int32), ("max_batch_letters", C.c_int64),
+                ("reserved", C.c_int32)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("status", C.c_int), ("batch_queries", C.c_int32), ("latency_us", C.c_double),
+                ("queue_us", C.c_double), ("n_subjects", C.c_int64), ("n_hsps", C.c_int64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("batches", C.c_uint64),
+                ("residues", C.c_uint64), ("device_ms", C.c_double)]
 
 
 class Sink(C.Structure):
@@ -79,7 +95,9 @@ EXPORTS = [
     "blz_db_upload", "blz_db_free", "blz_db_num_subjects", "blz_db_num_residues",
     "blz_query_prepare", "blz_query_free", "blz_prelim_run", "blz_prelim_run_batch",
     "blz_synth_default", "blz_synth_query", "blz_synth_db_size", "blz_synth_db_fill",
-    "blz_ctx_set_tuning", "blz_ctx_get_tuning",
+    "blz_ctx_set_tuning", "blz_ctx_get_tuning", "blz_result_sizes", "blz_result_copy", "blz_overflow_drain",
+    "blz_server_default_config", "blz_server_create", "blz_server_destroy", "blz_server_submit",
+    "blz_server_wait", "blz_server_copy", "blz_server_release", "blz_server_get_stats",
 ]
 
 _lib = None
@@ -121,6 +139,26 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.blz_ctx_set_tuning.restype = C.c_int
     lib.blz_ctx_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
     lib.blz_ctx_get_tuning.restype = C.c_int
+    lib.blz_result_sizes.argtypes = [vp, i32, C.POINTER(i64), C.POINTER(i64)]
+    lib.blz_result_sizes.restype = C.c_int
+    lib.blz_result_copy.argtypes = [vp, i32, C.POINTER(Sink)]
+    lib.blz_result_copy.restype = C.c_int
+    lib.blz_overflow_drain.argtypes = [vp, vp, i64, C.POINTER(i64)]
+    lib.blz_overflow_drain.restype = C.c_int
+    lib.blz_server_default_config.argtypes = [C.POINTER(ServerConfig)]
+    lib.blz_server_create.argtypes = [i32, vp, C.POINTER(ServerConfig), C.POINTER(Tuning), C.POINTER(vp)]
+    lib.blz_server_create.restype = C.c_int
+    lib.blz_server_destroy.argtypes = [vp]
+    lib.blz_server_submit.argtypes = [vp, vp, i32, C.POINTER(Params), C.POINTER(u64)]
+    lib.blz_server_submit.restype = C.c_int
+    lib.blz_server_wait.argtypes = [vp, u64, i32, C.POINTER(RequestInfo)]
+    lib.blz_server_wait.restype = C.c_int
+    lib.blz_server_copy.argtypes = [vp, u64, C.POINTER(Sink)]
+    lib.blz_server_copy.restype = C.c_int
+    lib.blz_server_release.argtypes = [vp, u64]
+    lib.blz_server_release.restype = C.c_int
+    lib.blz_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
+    lib.blz_server_get_stats.restype = C.c_int
     lib.blz_synth_default.argtypes = [C.POINTER(SynthSpec)]
     lib.blz_synth_query.argtypes = [u64, i32, vp]
     lib.blz_synth_db_size.argtypes = [C.POINTER(SynthSpec), C.POINTER(i64), C.POINTER(u64)]
@@ -207,6 +245,7 @@ class Engine:
         if st:
             raise BlzError(st, "blz_ctx_create: no usable gfx950 device")
         self.ctx = h
+        self.device = device
         self.db = None
         self.db_n = 0
 
@@ -281,37 +320,113 @@ class Engine:
                  n_subj_cap, hsps.ctypes.data, hsp_cap, 0, 0)
         return s, (oids, counts, hsps)
 
-    def run(self, qh, allow_overflow: bool = False, hsp_cap: int = 1 << 20):
+    def _fetch(self, qi: int) -> "Result":
+        ns, nh = C.c_int64(), C.c_int64()
+        st = self.lib.blz_result_sizes(self.ctx, qi, C.byref(ns), C.byref(nh))
+        if st:
+            self._err(st, "blz_result_sizes")
+        sink, (oids, counts, hsps) = self._sink(ns.value, nh.value)
+        st = self.lib.blz_result_copy(self.ctx, qi, C.byref(sink))
+        if st:
+            self._err(st, "blz_result_copy")
+        return Result(oids[: ns.value], counts[: ns.value], hsps[: nh.value])
+
+    def overflow_pairs(self) -> list[tuple[int, int]]:
+        """(query index, subject) pairs the last run could not finish within the device caps."""
+        n = C.c_int64()
+        self.lib.blz_overflow_drain(self.ctx, None, 0, C.byref(n))
+        buf = np.empty(max(1, n.value), dtype=np.int64)
+        st = self.lib.blz_overflow_drain(self.ctx, buf.ctypes.data, n.value, C.byref(n))
+        if st:
+            self._err(st, "blz_overflow_drain")
+        return [(int(v >> 40), int(v & ((1 << 40) - 1))) for v in buf[: n.value]]
+
+    def run(self, qh, allow_overflow: bool = False):
         """Prelim stage of one prepared query over the resident database -> (Result, stats)."""
         stats = RunStats()
-        sink, keep = self._sink(self.db_n, hsp_cap)
-        st = self.lib.blz_prelim_run(self.ctx, qh, self.db, C.byref(sink), C.byref(stats))
-        if st == 4:  # capacity: grow and run again
-            sink, keep = self._sink(self.db_n, int(sink.n_hsps))
-            stats = RunStats()
-            st = self.lib.blz_prelim_run(self.ctx, qh, self.db, C.byref(sink), C.byref(stats))
+        st = self.lib.blz_prelim_run(self.ctx, qh, self.db, None, C.byref(stats))
         if st and not (st == 5 and allow_overflow):
             self._err(st, "blz_prelim_run")
-        oids, counts, hsps = keep
-        ns, nh = int(sink.n_subjects), int(sink.n_hsps)
-        return Result(oids[:ns].copy(), counts[:ns].copy(), hsps[:nh].copy()), stats.as_dict()
+        return self._fetch(0), stats.as_dict()
 
-    def run_batch(self, qhs, allow_overflow: bool = False, hsp_cap: int = 1 << 20):
+    def run_batch(self, qhs, allow_overflow: bool = False, fetch: bool = True):
+        """Prelim stage of several prepared queries in one batched run -> ([Result], stats)."""
         n = len(qhs)
         stats = RunStats()
-        sinks = (Sink * n)()
-        keeps = []
-        for i in range(n):
-            s, k = self._sink(self.db_n, hsp_cap)
-            sinks[i] = s
-            keeps.append(k)
         arr = (C.c_void_p * n)(*[q.value if isinstance(q, C.c_void_p) else q for q in qhs])
-        st = self.lib.blz_prelim_run_batch(self.ctx, arr, n, self.db, sinks, C.byref(stats))
+        st = self.lib.blz_prelim_run_batch(self.ctx, arr, n, self.db, None, C.byref(stats))
         if st and not (st == 5 and allow_overflow):
             self._err(st, "blz_prelim_run_batch")
-        out = []
-        for i in range(n):
-            oids, counts, hsps = keeps[i]
-            ns, nh = int(sinks[i].n_subjects), int(sinks[i].n_hsps)
-            out.append(Result(oids[:ns].copy(), counts[:ns].copy(), hsps[:nh].copy()))
-        return out, stats.as_dict()
+        return ([self._fetch(i) for i in range(n)] if fetch else None), stats.as_dict()
+
+
+class Server:
+    """Request loop over an engine's resident database (blz_server_*): submit queries from any
+    thread, wait for each, fetch its lists."""
+
+    def __init__(self, engine: "Engine", slots: int = 0, max_batch: int = 0, max_batch_letters: int = 0,
+                 tuning: dict | None = None):
+        self.lib = engine.lib
+        cfg = ServerConfig()
+        self.lib.blz_server_default_config(C.byref(cfg))
+        if slots:
+            cfg.slots = slots
+        if max_batch:
+            cfg.max_batch = max_batch
+        if max_batch_letters:
+            cfg.max_batch_letters = max_batch_letters
+        t = None
+        if tuning:
+            t = Tuning()
+            engine.lib.blz_ctx_get_tuning(engine.ctx, C.byref(t))
+            for k, v in tuning.items():
+                setattr(t, k, v)
+        h = C.c_void_p()
+        st = self.lib.blz_server_create(engine.device, engine.db, C.byref(cfg), C.byref(t) if t else None,
+                                        C.byref(h))
+        if st:
+            raise BlzError(st, "blz_server_create")
+        self.s = h
+
+    def submit(self, query: np.ndarray, params: Params | None = None) -> int:
+        query = np.ascontiguousarray(query, dtype=np.uint8)
+        tk = C.c_uint64()
+        st = self.lib.blz_server_submit(self.s, query.ctypes.data, len(query),
+                                        C.byref(params) if params is not None else None, C.byref(tk))
+        if st:
+            raise BlzError(st, "blz_server_submit")
+        return tk.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1) -> dict:
+        info = RequestInfo()
+        st = self.lib.blz_server_wait(self.s, ticket, timeout_ms, C.byref(info))
+        if st:
+            raise BlzError(st, "blz_server_wait")
+        return {k: getattr(info, k) for k, _ in RequestInfo._fields_}
+
+    def result(self, ticket: int) -> "Result":
+        info = self.wait(ticket)
+        sink, (oids, counts, hsps) = Engine._sink(info["n_subjects"], info["n_hsps"])
+        st = self.lib.blz_server_copy(self.s, ticket, C.byref(sink))
+        if st:
+            raise BlzError(st, "blz_server_copy")
+        return Result(oids[: info["n_subjects"]], counts[: info["n_subjects"]], hsps[: info["n_hsps"]])
+
+    def release(self, ticket: int) -> None:
+        self.lib.blz_server_release(self.s, ticket)
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        self.lib.blz_server_get_stats(self.s, C.byref(s))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def close(self) -> None:
+        if self.s:
+            self.lib.blz_server_destroy(self.s)
+            self.s = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

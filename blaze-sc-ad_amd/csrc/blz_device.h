@@ -101,6 +101,7 @@ hipError_t launch_seed2(const SeedArgs &a, int table, int blocks, int waves, boo
 // Resident workgroups per CU of that launch shape (registers and LDS), 0 if it cannot run.
 int seed2_blocks_per_cu(const SeedArgs &a, int table, int waves, bool long_subjects);
 hipError_t launch_seed(const SeedArgs &a, int blocks, int waves_per_block, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t st);
 // cost-ordered work list: order[] receives the work indices, heaviest class first
 hipError_t launch_order(const WorkItem *work, DevCounters *ctr, uint32_t *order, uint32_t work_cap, int blocks,
                         hipStream_t st);

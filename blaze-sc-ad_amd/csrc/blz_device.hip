@@ -12,6 +12,8 @@
 // (The gapped stage is in blz_gapped.hip; the batched seed kernel in blz_seed.hip.)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "blz_common.h"
 #include "blz_device.h"
 #include "blz_dev_common.h"
@@ -208,6 +210,19 @@ size_t seed_lds_bytes(uint32_t ring_words, int waves_per_block) {
 hipError_t launch_seed(const SeedArgs &a, int blocks, int waves_per_block, hipStream_t st) {
     const size_t lds = seed_lds_bytes(a.ring_mask + 1, waves_per_block);
     hipLaunchKernelGGL(k_seed, dim3(blocks), dim3(64 * waves_per_block), lds, st, a);
+    return hipGetLastError();
+}
+
+// Clears counters with a kernel rather than hipMemsetAsync, so the same sequence can be captured
+// into a hipGraph.
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 64);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, st, p, (uint32_t)n, v);
     return hipGetLastError();
 }
 

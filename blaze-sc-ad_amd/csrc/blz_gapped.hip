@@ -477,9 +477,19 @@ __global__ void __launch_bounds__(256) k_order(const WorkItem *work, DevCounters
         base[b] = acc;
         acc += ctr->bucket_count[b];
     }
+    const int lane = lane_id();
+    const uint64_t below = (1ull << lane) - 1;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int b = work[i].bucket;
-        order[base[b] + atomicAdd(&ctr->bucket_cursor[b], 1u)] = i;
+        for (int k = 0; k < kBuckets; k++) {  // one counter update per class and wave
+            const uint64_t m = __ballot(b == k);
+            if (!m) continue;
+            const int leader = __ffsll((long long)m) - 1;
+            uint32_t at = 0;
+            if (lane == leader) at = atomicAdd(&ctr->bucket_cursor[k], (uint32_t)__popcll(m));
+            at = (uint32_t)__shfl((int)at, leader, 64);
+            if (b == k) order[base[k] + at + (uint32_t)__popcll(m & below)] = i;
+        }
     }
 }
 

@@ -7,8 +7,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -39,10 +41,11 @@ struct DevTables {
     uint16_t *d_rel16 = nullptr;
     uint32_t *d_blockbase = nullptr;
     bool compact_ok = false;
+    hipStream_t st = nullptr;  // stream-ordered allocations: freed without a device-wide sync
     void release() {
         void *bufs[] = {d_query, d_prefix, d_entries, d_rel16, d_blockbase};
         for (void *b : bufs)
-            if (b) (void)hipFree(b);
+            if (b) (void)(st ? hipFreeAsync(b, st) : hipFree(b));
         *this = DevTables{};
     }
 };
@@ -63,6 +66,7 @@ struct blz_ctx {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int num_cus = 0;
     size_t lds_per_block = 0;
+    bool async_alloc = false;  // stream-ordered allocator available
     blz_tuning tune{};
     std::string err;
     // reusable device buffers
@@ -99,13 +103,36 @@ struct blz_ctx {
     size_t slices_cap = 0;
     uint32_t *long_ring = nullptr; // seed rings for subjects the LDS ring does not take
     size_t long_ring_cap = 0;
-    // packs of the last batch, reused while the same queries come again
-    std::vector<const blz_query *> pack_key;
+    // packs of the last batch (keyed by query ids), reused while the same queries come again
+    std::vector<uint64_t> pack_key;
     std::vector<Pack> packs;
+    uint64_t gen = 0;              // bumped whenever a work buffer is reallocated
+    // lists of the last run, query-major and subjects ascending, for blz_result_*
+    std::vector<ResultItem> last_res;
+    std::vector<Hsp> last_hs;
+    std::vector<size_t> last_first;      // [nq + 1] first result item of each query
+    std::vector<int64_t> last_nh;        // [nq] HSPs of each query
+    std::vector<int64_t> last_ovf;       // overflow pairs: query << 40 | subject
+    // launch sequence of the last batch, captured; valid while the same batch comes again and
+    // no other run has rewritten the device tables in between
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uint64_t> graph_key;  // query ids, then the database id
+    blz_tuning graph_tune{};
+    uint64_t graph_gen = 0;
+    int graph_launches = 0;
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        graph_key.clear();
+    }
 };
 
 struct blz_db {
     blz_ctx *ctx = nullptr;
+    uint64_t uid = 0;           // never reused, unlike addresses
     uint8_t *res = nullptr;     // device
     uint64_t *offs = nullptr;   // device
     int64_t nsubj = 0;
@@ -117,6 +144,7 @@ struct blz_db {
 
 struct blz_query {
     blz_ctx *ctx = nullptr;
+    uint64_t uid = 0;           // never reused, unlike addresses
     int32_t qlen = 0;
     blz_params params{};
     std::vector<uint8_t> letters;
@@ -124,6 +152,8 @@ struct blz_query {
 };
 
 namespace {
+
+std::atomic<uint64_t> g_next_uid{1};
 
 constexpr int kStageCap = 4096;   // init hits per subject kept on the device
 constexpr int kDpRing = 4096;     // wide gapped pass: DP band ring per lane (power of two)
@@ -147,6 +177,7 @@ bool ensure(blz_ctx *ctx, T *&ptr, size_t &cap, size_t need, const char *what) {
     if (ptr) (void)hipFree(ptr);
     ptr = nullptr;
     cap = 0;
+    ctx->gen++;
     size_t n = std::max<size_t>(need, 1);
     if (!check(ctx, hipMalloc((void **)&ptr, n * sizeof(T)), what)) return false;
     cap = n;
@@ -248,24 +279,32 @@ blz_status build_tables(blz_ctx *ctx, const uint8_t *letters, int len, const std
     t.len = len;
     t.n_entries = prefix[kWords];
     t.compact_ok = compact_ok;
-    if (!check(ctx, hipMalloc((void **)&t.d_query, len + 64), "query") ||
-        !check(ctx, hipMalloc((void **)&t.d_prefix, sizeof(uint32_t) * (kWords + 1)), "prefix") ||
-        !check(ctx, hipMalloc((void **)&t.d_entries, sizeof(uint16_t) * entries.size()), "entries") ||
-        !check(ctx, hipMalloc((void **)&t.d_rel16, sizeof(uint16_t) * (kWords + 1)), "rel16") ||
-        !check(ctx, hipMalloc((void **)&t.d_blockbase, sizeof(uint32_t) * kRelBlocks), "blockbase")) {
+    // Stream-ordered allocations on the context's stream: releasing them later (hipFreeAsync)
+    // does not synchronise the device, so a serving loop can drop queries while other
+    // contexts' batches run.
+    hipStream_t st = ctx->stream;
+    t.st = ctx->async_alloc ? st : nullptr;
+    auto alloc = [&](void **p, size_t n) { return ctx->async_alloc ? hipMallocAsync(p, n, st) : hipMalloc(p, n); };
+    if (!check(ctx, alloc((void **)&t.d_query, len + 64), "query") ||
+        !check(ctx, alloc((void **)&t.d_prefix, sizeof(uint32_t) * (kWords + 1)), "prefix") ||
+        !check(ctx, alloc((void **)&t.d_entries, sizeof(uint16_t) * entries.size()), "entries") ||
+        !check(ctx, alloc((void **)&t.d_rel16, sizeof(uint16_t) * (kWords + 1)), "rel16") ||
+        !check(ctx, alloc((void **)&t.d_blockbase, sizeof(uint32_t) * kRelBlocks), "blockbase")) {
         t.release();
         return BLZ_ENOMEM;
     }
-    if (!check(ctx, hipMemcpy(t.d_query, letters, len, hipMemcpyHostToDevice), "query copy") ||
-        !check(ctx, hipMemset(t.d_query + len, 0, 64), "query pad") ||
-        !check(ctx, hipMemcpy(t.d_prefix, prefix.data(), sizeof(uint32_t) * (kWords + 1), hipMemcpyHostToDevice),
-               "prefix copy") ||
-        !check(ctx, hipMemcpy(t.d_entries, entries.data(), sizeof(uint16_t) * entries.size(), hipMemcpyHostToDevice),
-               "entries copy") ||
-        !check(ctx, hipMemcpy(t.d_rel16, rel16.data(), sizeof(uint16_t) * (kWords + 1), hipMemcpyHostToDevice),
-               "rel16 copy") ||
-        !check(ctx, hipMemcpy(t.d_blockbase, blockbase.data(), sizeof(uint32_t) * kRelBlocks, hipMemcpyHostToDevice),
-               "blockbase copy")) {
+    std::vector<uint8_t> qpad(letters, letters + len);
+    qpad.resize((size_t)len + 64, 0);
+    if (!check(ctx, hipMemcpyAsync(t.d_query, qpad.data(), qpad.size(), hipMemcpyHostToDevice, st), "query copy") ||
+        !check(ctx, hipMemcpyAsync(t.d_prefix, prefix.data(), sizeof(uint32_t) * (kWords + 1), hipMemcpyHostToDevice,
+                                   st), "prefix copy") ||
+        !check(ctx, hipMemcpyAsync(t.d_entries, entries.data(), sizeof(uint16_t) * entries.size(),
+                                   hipMemcpyHostToDevice, st), "entries copy") ||
+        !check(ctx, hipMemcpyAsync(t.d_rel16, rel16.data(), sizeof(uint16_t) * (kWords + 1), hipMemcpyHostToDevice,
+                                   st), "rel16 copy") ||
+        !check(ctx, hipMemcpyAsync(t.d_blockbase, blockbase.data(), sizeof(uint32_t) * kRelBlocks,
+                                   hipMemcpyHostToDevice, st), "blockbase copy") ||
+        !check(ctx, hipStreamSynchronize(st), "table upload")) {
         t.release();
         return BLZ_EDEVICE;
     }
@@ -300,6 +339,9 @@ blz_status blz_ctx_create(int32_t device, blz_ctx **out) {
     ctx->device = device;
     ctx->num_cus = prop.multiProcessorCount;
     ctx->lds_per_block = prop.sharedMemPerBlock;
+    int pools = 0;
+    ctx->async_alloc = hipDeviceGetAttribute(&pools, hipDeviceAttributeMemoryPoolsSupported, device) == hipSuccess &&
+                       pools != 0;
     if (!check(ctx, hipSetDevice(device), "hipSetDevice") ||
         !check(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "stream")) {
         delete ctx;
@@ -325,6 +367,7 @@ void blz_ctx_destroy(blz_ctx *ctx) {
     void *bufs[] = {ctx->long_ring, ctx->slices, ctx->ctr,   ctx->stage,   ctx->pool,    ctx->work, ctx->ovf,     ctx->dp,
                     ctx->saved, ctx->hsps,    ctx->results, ctx->qdev, ctx->item_ctr,
                     ctx->order, ctx->redo, ctx->saved_wide, ctx->pack_meta};
+    ctx->drop_graph();
     for (auto &pk : ctx->packs) pk.t.release();
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -339,9 +382,11 @@ blz_status blz_ctx_set_tuning(blz_ctx *ctx, const blz_tuning *t) {
     if (t->seed_variant < 0 || t->seed_variant > 2 || t->seed_waves < 0 || t->seed_waves > 16 ||
         t->seed_blocks_per_cu < 0 || t->gapped_blocks < 0 || t->gapped_threads < 0 || t->gapped_threads > 1024 ||
         (t->gapped_threads % 64) != 0 || t->seed_table < 0 || t->seed_table > 3 || t->gapped_wave_classes < 0 ||
-        t->gapped_wave_classes > kBuckets + 1 || t->query_packing < 0 || t->query_packing > 2)
+        t->gapped_wave_classes > kBuckets + 1 || t->query_packing < 0 || t->query_packing > 2 || t->pack_len < 0 ||
+        t->pack_len > 16000 || t->bin_ring < 0 || t->graph < 0 || t->graph > 1)
         return BLZ_EINVAL;
-    if (t->query_packing != ctx->tune.query_packing) {  // cached packs were made under the old choice
+    if (t->query_packing != ctx->tune.query_packing || t->pack_len != ctx->tune.pack_len) {
+        // cached packs were made under the old choice
         for (auto &pk : ctx->packs) pk.t.release();
         ctx->packs.clear();
         ctx->pack_key.clear();
@@ -382,6 +427,7 @@ blz_status blz_db_upload(blz_ctx *ctx, const uint8_t *residues, const uint64_t *
     (void)hipSetDevice(ctx->device);
     blz_db *db = new blz_db();
     db->ctx = ctx;
+    db->uid = g_next_uid++;
     db->nsubj = nsubj;
     db->nres = nres;
     // 64 padding bytes so word reads near the end stay inside the allocation
@@ -415,6 +461,7 @@ blz_status blz_db_upload(blz_ctx *ctx, const uint8_t *residues, const uint64_t *
 
 void blz_db_free(blz_db *db) {
     if (!db) return;
+    if (db->ctx) db->ctx->drop_graph();  // a captured batch may read this database
     if (db->res) (void)hipFree(db->res);
     if (db->offs) (void)hipFree(db->offs);
     if (db->long_ids) (void)hipFree(db->long_ids);
@@ -448,6 +495,7 @@ blz_status blz_query_prepare(blz_ctx *ctx, const uint8_t *query, int32_t qlen, c
     (void)hipSetDevice(ctx->device);
     blz_query *qq = new blz_query();
     qq->ctx = ctx;
+    qq->uid = g_next_uid++;
     qq->qlen = qlen;
     qq->params = p;
     qq->letters.assign(query, query + qlen);
@@ -463,9 +511,10 @@ blz_status blz_query_prepare(blz_ctx *ctx, const uint8_t *query, int32_t qlen, c
 void blz_query_free(blz_query *q) {
     if (!q) return;
     if (q->ctx) {
+        q->ctx->drop_graph();  // a captured batch may read this query's tables
         // a cached pack may hold this query; drop the cache
-        for (const blz_query *k : q->ctx->pack_key)
-            if (k == q) {
+        for (uint64_t k : q->ctx->pack_key)
+            if (k == q->uid) {
                 for (auto &pk : q->ctx->packs) pk.t.release();
                 q->ctx->packs.clear();
                 q->ctx->pack_key.clear();
@@ -522,7 +571,7 @@ blz_status make_packs(blz_ctx *ctx, const blz_query *const *queries, int nq, std
         const blz_query *q0 = queries[idx[k]];
         Pack pk;
         const int gap = q0->params.window + 3;
-        const int max_len = pack_len(q0->params.window);
+        const int max_len = ctx->tune.pack_len ? ctx->tune.pack_len : pack_len(q0->params.window);
         int len = 0;
         while (k < idx.size()) {
             const blz_query *q = queries[idx[k]];
@@ -601,17 +650,20 @@ bool plan_seed2(blz_ctx *ctx, const std::vector<QueryDev> &qs, const std::vector
         sa.n_entries = std::max(sa.n_entries, qs[i].n_entries);
     }
     const size_t cap = ctx->lds_per_block;
-    // most resident waves per CU (registers and LDS); fewer waves per workgroup on ties
+    // most resident waves per CU (registers and LDS); more waves per workgroup on ties (fewer
+    // workgroups stage the query and wait at item boundaries).  A tuned wave count that does not
+    // fit falls back to this choice.
     int waves = 0, per_cu = 0;
-    for (int w = 1; w <= 16; w++) {
-        if (ctx->tune.seed_waves && w != ctx->tune.seed_waves) continue;
-        if (seed2_lds_bytes(sa, table, w) > cap) break;
-        const int n = seed2_blocks_per_cu(sa, table, w, long_subjects);
-        if (n > 0 && w * n > waves * per_cu) {
-            waves = w;
-            per_cu = n;
+    for (int pass = ctx->tune.seed_waves ? 0 : 1; pass < 2 && !waves; pass++)
+        for (int w = 1; w <= 16; w++) {
+            if (pass == 0 && w != ctx->tune.seed_waves) continue;
+            if (seed2_lds_bytes(sa, table, w) > cap) break;
+            const int n = seed2_blocks_per_cu(sa, table, w, long_subjects);
+            if (n > 0 && w * n >= waves * per_cu) {
+                waves = w;
+                per_cu = n;
+            }
         }
-    }
     if (!waves) return false;
     if (ctx->tune.seed_blocks_per_cu) per_cu = ctx->tune.seed_blocks_per_cu;
     pl.table = table;
@@ -621,29 +673,84 @@ bool plan_seed2(blz_ctx *ctx, const std::vector<QueryDev> &qs, const std::vector
     return true;
 }
 
-// Table placement for one query: the tuned one if it fits, else the richest that leaves room
-// for at least min_waves waves per workgroup.
+// Table placement for one pack: the tuned one, else the one with the most resident waves.
 int pick_table(blz_ctx *ctx, const std::vector<QueryDev> &qs, int i, bool compact_ok) {
     const int want = ctx->tune.seed_table;  // 0 auto, else table + 1
     if (want) {
         if (want - 1 == 2 && !compact_ok) return 1;
         return want - 1;
     }
-    // built-in choice: entries in LDS, prefix in global memory (L2-resident), as long as that
-    // leaves four waves per workgroup; the prefix in LDS costs more waves than it saves
-    SeedPlan probe;
-    if (plan_seed2(ctx, qs, {i}, 1, probe) && probe.waves >= 4) return 1;
+    // built-in choice: the placement with the most resident waves per CU (the kernel is bound by
+    // LDS latency, which waves hide); entries in LDS only when that costs no waves
+    SeedPlan g, l;
+    if (!plan_seed2(ctx, qs, {i}, 0, g)) return 0;
+    if (plan_seed2(ctx, qs, {i}, 1, l) && l.waves * l.blocks >= g.waves * g.blocks) return 1;
     return 0;
 }
 
-// Prelim stage of queries[0..nq) over the database: seed launches (one per query-size bin),
-// one gapped launch, results copied into the sinks.
+// Query qi's lists of the last run into a sink; ECAPACITY (sizes still reported) if it is too small.
+blz_status copy_result(const blz_ctx *ctx, int qi, blz_hsp_sink *sink) {
+    const size_t r0 = ctx->last_first[qi], r1 = ctx->last_first[qi + 1];
+    sink->n_subjects = (int64_t)(r1 - r0);
+    sink->n_hsps = ctx->last_nh[qi];
+    if (sink->n_subjects > sink->subject_cap || sink->n_hsps > sink->hsp_cap) return BLZ_ECAPACITY;
+    int64_t o = 0;
+    for (size_t i = r0; i < r1; i++) {
+        const ResultItem &r = ctx->last_res[i];
+        sink->oids[i - r0] = r.sid;
+        sink->counts[i - r0] = r.n;
+        std::memcpy(sink->hsps + o, ctx->last_hs.data() + r.off, sizeof(Hsp) * r.n);
+        o += r.n;
+    }
+    return BLZ_OK;
+}
+
+blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq, const blz_db *db, int variant,
+                           bool use_graph, const std::vector<uint64_t> &key, int &n_launches);
+blz_status collect(blz_ctx *ctx, int nq, const blz_db *db, blz_hsp_sink *sinks, blz_run_stats *stats,
+                   int n_launches);
+
+// Prelim stage of queries[0..nq) over the database: seed launches (one per bin), the gapped
+// stage, results copied into the sinks.  batch: the caller's batch entry point, whose launch
+// sequence may be captured and replayed as a hipGraph.
 blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, const blz_db *db,
-                       blz_hsp_sink *sinks, blz_run_stats *stats) {
+                       blz_hsp_sink *sinks, blz_run_stats *stats, bool batch) {
     for (int i = 0; i < nq; i++)
-        if (!queries[i] || queries[i]->ctx != ctx) return BLZ_EINVAL;
+        if (!queries[i] || queries[i]->ctx->device != ctx->device) return BLZ_EINVAL;
     const int variant = ctx->tune.seed_variant ? ctx->tune.seed_variant : 2;
     const size_t nsubj = (size_t)db->nsubj;
+    hipStream_t st = ctx->stream;
+    DevCounters *ctr = ctx->ctr;  // one counter block for the run; one item counter per seed launch
+    const bool use_graph = batch && variant != 1 && ctx->tune.graph != 1 && nsubj > 0;
+    std::vector<uint64_t> key;
+    if (use_graph) {
+        for (int i = 0; i < nq; i++) key.push_back(queries[i]->uid);
+        key.push_back(db->uid);
+    }
+    int n_launches = 0;
+    if (use_graph && ctx->graph_exec && key == ctx->graph_key && ctx->graph_gen == ctx->gen &&
+        std::memcmp(&ctx->graph_tune, &ctx->tune, sizeof(blz_tuning)) == 0) {
+        // the same batch again: the tables on the device are the ones the graph was captured with
+        n_launches = ctx->graph_launches;
+        if (!check(ctx, hipEventRecord(ctx->ev[0], st), "event") ||
+            !check(ctx, hipGraphLaunch(ctx->graph_exec, st), "graph launch") ||
+            !check(ctx, hipEventRecord(ctx->ev[1], st), "event") || !check(ctx, hipEventRecord(ctx->ev[2], st), "event"))
+            return BLZ_EDEVICE;
+    } else {
+        const blz_status rc = plan_and_launch(ctx, queries, nq, db, variant, use_graph, key, n_launches);
+        if (rc != BLZ_OK) return rc;
+    }
+    return collect(ctx, nq, db, sinks, stats, n_launches);
+}
+
+// Plans the run (packs, bins, launch shapes, slices), uploads the run's tables and enqueues the
+// launch sequence; with use_graph the sequence is captured into ctx->graph first.
+blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq, const blz_db *db, int variant,
+                           bool use_graph, const std::vector<uint64_t> &key, int &n_launches) {
+    ctx->drop_graph();
+    const size_t nsubj = (size_t)db->nsubj;
+    hipStream_t st = ctx->stream;
+    DevCounters *ctr = ctx->ctr;
     // query table: [0, nq) the caller's queries (gapped stage), then one entry per pack (seed)
     std::vector<QueryDev> qs(nq);
     for (int i = 0; i < nq; i++) qs[i] = query_dev(queries[i]);
@@ -670,7 +777,8 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
         pl.blocks = std::max(1, ctx->num_cus * 8 / waves);
         plans.push_back(pl);
     } else {
-        std::vector<const blz_query *> key(queries, queries + nq);
+        std::vector<uint64_t> key(nq);
+        for (int i = 0; i < nq; i++) key[i] = queries[i]->uid;
         if (key != ctx->pack_key) {
             for (auto &pk : ctx->packs) pk.t.release();
             ctx->packs.clear();
@@ -709,12 +817,19 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
             pq[k] = d;
             compact[k] = t.compact_ok;
         }
-        std::vector<int> bins[3];
-        for (size_t k = 0; k < packs.size(); k++) bins[pick_table(ctx, pq, (int)k, compact[k])].push_back((int)k);
+        // bins: table placement x query size (diagonal ring length: up to E, 2E, 4E, beyond)
+        const uint32_t E = ctx->tune.bin_ring ? (uint32_t)ctx->tune.bin_ring : 1152u;
+        std::vector<int> bins[3 * 4];
+        for (size_t k = 0; k < packs.size(); k++) {
+            const uint32_t r = pq[k].ring_len;
+            const int rc = r <= E ? 0 : r <= 2 * E ? 1 : r <= 4 * E ? 2 : 3;
+            bins[pick_table(ctx, pq, (int)k, compact[k]) * 4 + rc].push_back((int)k);
+        }
         // each bin's packs are made contiguous after the queries; the plan records its first
         std::vector<int> order;
-        for (int table = 2; table >= 0; table--) {
-            const std::vector<int> &idx = bins[table];
+        for (int bin = 3 * 4 - 1; bin >= 0; bin--) {
+            const int table = bin / 4;
+            const std::vector<int> &idx = bins[bin];
             if (idx.empty()) continue;
             SeedPlan pl;
             if (!plan_seed2(ctx, pq, idx, table, pl)) {
@@ -762,6 +877,12 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
             slice_db(db, ((int64_t)pl.blocks * 8 + pl.sa.nq - 1) / pl.sa.nq, slices);
         }
         pl.sa.n_slices = (uint32_t)(slices.size() - pl.slice_off - 1);
+        if (std::getenv("BLZ_LOG_PLAN"))  // launch shapes, for tuning by hand
+            std::fprintf(stderr,
+                         "[blz] seed launch: table %d%s, %d packs, qlen <= %d, ring %u, entries <= %u, %d waves x %d "
+                         "blocks, %u slices, lds %zu\n",
+                         pl.table, pl.long_subjects ? " (long subjects)" : "", pl.sa.nq, pl.sa.qlen, pl.sa.ring_len,
+                         pl.sa.n_entries, pl.waves, pl.blocks, pl.sa.n_slices, seed2_lds_bytes(pl.sa, pl.table, pl.waves));
     }
     const size_t lds_lanes = (size_t)(ctx->tune.gapped_blocks ? ctx->tune.gapped_blocks : ctx->num_cus * 4) * 64;
     const size_t wide_lanes = (size_t)kGappedBlocks * 64;
@@ -784,19 +905,14 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
         (long_ring && !ensure(ctx, ctx->long_ring, ctx->long_ring_cap, long_ring, "long-subject rings")))
         return BLZ_ENOMEM;
 
-    hipStream_t st = ctx->stream;
     if (!check(ctx, hipMemcpyAsync(ctx->qdev, qs.data(), sizeof(QueryDev) * qs.size(), hipMemcpyHostToDevice, st),
                "query table") ||
         (!meta.empty() && !check(ctx, hipMemcpyAsync(ctx->pack_meta, meta.data(), sizeof(uint32_t) * meta.size(),
                                                       hipMemcpyHostToDevice, st), "pack table")) ||
         (!slices.empty() && !check(ctx, hipMemcpyAsync(ctx->slices, slices.data(), sizeof(int64_t) * slices.size(),
-                                                        hipMemcpyHostToDevice, st), "slices")) ||
-        !check(ctx, hipMemsetAsync(ctx->ctr, 0, sizeof(DevCounters), st), "memset") ||
-        !check(ctx, hipMemsetAsync(ctx->item_ctr, 0, sizeof(uint32_t) * (plans.size() + 1), st), "memset"))
+                                                        hipMemcpyHostToDevice, st), "slices")))
         return BLZ_EDEVICE;
-    DevCounters *ctr = ctx->ctr;  // one counter block for the run; one item counter per seed launch
 
-    if (!check(ctx, hipEventRecord(ctx->ev[0], st), "event")) return BLZ_EDEVICE;
     for (size_t pi = 0; pi < plans.size() && nsubj > 0; pi++) {
         SeedPlan &pl = plans[pi];
         SeedArgs &sa = pl.sa;
@@ -811,7 +927,6 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
         sa.ovf = ctx->ovf;
         sa.ovf_cap = (uint32_t)ctx->ovf_cap;
         sa.ctr = ctr;
-        hipError_t e;
         if (variant == 1) {
             const QueryDev &d = qs[0];
             sa.s_begin = 0;
@@ -824,7 +939,6 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
             sa.x_ungapped = d.x_ungapped;
             sa.ungapped_cutoff = d.ungapped_cutoff;
             sa.ring_mask = d.ring_mask;
-            e = launch_seed(sa, pl.blocks, pl.waves, st);
         } else {
             sa.qd = ctx->qdev + pl.first_pack;
             sa.s_begin = 0;
@@ -833,9 +947,7 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
             sa.long_ring = ctx->long_ring;
             sa.long_ids = db->long_ids;
             sa.item_ctr = ctx->item_ctr + pi;
-            e = launch_seed2(sa, pl.table, pl.blocks, pl.waves, pl.long_subjects, st);
         }
-        if (!check(ctx, e, "k_seed launch")) return BLZ_EDEVICE;
     }
     GappedArgs ga{};
     ga.res = db->res;
@@ -875,13 +987,61 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
     const int heavy_blocks = ctx->num_cus * 8;
     GappedArgs gh = ga;  // wave pass: per-block saved list of kSavedCapWave
     gh.saved_cap = kSavedCapWave;
-    if (!check(ctx, hipEventRecord(ctx->ev[1], st), "event") ||
-        (nsubj > 0 && !check(ctx, launch_order(ctx->work, ctr, ctx->order, (uint32_t)ctx->work_cap, 512, st), "k_order launch")) ||
-        (nsubj > 0 && ga.heavy_classes > 0 && !check(ctx, launch_gapped_wave(gh, heavy_blocks, st), "k_gapped_wave launch")) ||
-        (nsubj > 0 && ga.heavy_classes < kBuckets && !check(ctx, launch_gapped_lds(ga, lds_blocks, st), "k_gapped_lds launch")) ||
-        (nsubj > 0 && !check(ctx, launch_gapped_wide(gw, kGappedBlocks, 64, st), "k_gapped_wide launch")) ||
-        !check(ctx, hipEventRecord(ctx->ev[2], st), "event"))
+    const size_t n_item_ctr = plans.size() + 1;
+    // the launch sequence; timed: with the stage events (not under capture)
+    auto enqueue = [&](bool timed) -> bool {
+        if (!check(ctx, launch_fill_u32(reinterpret_cast<uint32_t *>(ctr), sizeof(DevCounters) / 4, 0, st), "clear") ||
+            !check(ctx, launch_fill_u32(ctx->item_ctr, n_item_ctr, 0, st), "clear") ||
+            (timed && !check(ctx, hipEventRecord(ctx->ev[0], st), "event")))
+            return false;
+        for (size_t pi = 0; pi < plans.size() && nsubj > 0; pi++) {
+            const SeedPlan &pl = plans[pi];
+            const hipError_t e = variant == 1 ? launch_seed(pl.sa, pl.blocks, pl.waves, st)
+                                              : launch_seed2(pl.sa, pl.table, pl.blocks, pl.waves, pl.long_subjects, st);
+            if (!check(ctx, e, "k_seed launch")) return false;
+        }
+        return (!timed || check(ctx, hipEventRecord(ctx->ev[1], st), "event")) &&
+               (nsubj == 0 || check(ctx, launch_order(ctx->work, ctr, ctx->order, (uint32_t)ctx->work_cap, 512, st),
+                                    "k_order launch")) &&
+               (nsubj == 0 || ga.heavy_classes == 0 ||
+                check(ctx, launch_gapped_wave(gh, heavy_blocks, st), "k_gapped_wave launch")) &&
+               (nsubj == 0 || ga.heavy_classes >= kBuckets ||
+                check(ctx, launch_gapped_lds(ga, lds_blocks, st), "k_gapped_lds launch")) &&
+               (nsubj == 0 || check(ctx, launch_gapped_wide(gw, kGappedBlocks, 64, st), "k_gapped_wide launch")) &&
+               (!timed || check(ctx, hipEventRecord(ctx->ev[2], st), "event"));
+    };
+    n_launches = (int)plans.size();
+    if (!use_graph) return enqueue(true) ? BLZ_OK : BLZ_EDEVICE;
+    if (!check(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed), "capture")) return BLZ_EDEVICE;
+    const bool ok = enqueue(false);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &g);
+    if (!ok || !check(ctx, ec, "capture")) {
+        if (g) (void)hipGraphDestroy(g);
         return BLZ_EDEVICE;
+    }
+    ctx->graph = g;
+    if (!check(ctx, hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0), "graph instantiate")) {
+        ctx->drop_graph();
+        return BLZ_EDEVICE;
+    }
+    ctx->graph_key = key;
+    ctx->graph_tune = ctx->tune;
+    ctx->graph_gen = ctx->gen;
+    ctx->graph_launches = n_launches;
+    if (!check(ctx, hipEventRecord(ctx->ev[0], st), "event") ||
+        !check(ctx, hipGraphLaunch(ctx->graph_exec, st), "graph launch") ||
+        !check(ctx, hipEventRecord(ctx->ev[1], st), "event") || !check(ctx, hipEventRecord(ctx->ev[2], st), "event"))
+        return BLZ_EDEVICE;
+    return BLZ_OK;
+}
+
+// Counters and HSP lists of the run just enqueued, into the caller's sinks (caller order,
+// subjects ascending).
+blz_status collect(blz_ctx *ctx, int nq, const blz_db *db, blz_hsp_sink *sinks, blz_run_stats *stats,
+                   int n_launches) {
+    hipStream_t st = ctx->stream;
+    DevCounters *ctr = ctx->ctr;
 
     DevCounters c{};
     if (!check(ctx, hipMemcpyAsync(&c, ctr, sizeof(c), hipMemcpyDeviceToHost, st), "counters") ||
@@ -905,32 +1065,35 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
     std::sort(res.begin(), res.end(), [](const ResultItem &a, const ResultItem &b) {
         return a.qi != b.qi ? a.qi < b.qi : a.sid < b.sid;
     });
-    blz_status rc = BLZ_OK;
+    ctx->last_first.assign((size_t)nq + 1, 0);
+    ctx->last_nh.assign((size_t)nq, 0);
     int64_t total = 0;
     size_t r0 = 0;
     for (int qcaller = 0; qcaller < nq; qcaller++) {
         size_t r1 = r0;
         int64_t nh = 0;
         while (r1 < res.size() && res[r1].qi == qcaller) nh += res[r1++].n;
+        ctx->last_first[qcaller] = r0;
+        ctx->last_nh[qcaller] = nh;
         total += nh;
-        if (sinks) {
-            blz_hsp_sink *sink = &sinks[qcaller];
-            sink->n_subjects = (int64_t)(r1 - r0);
-            sink->n_hsps = nh;
-            if ((int64_t)(r1 - r0) > sink->subject_cap || nh > sink->hsp_cap) {
-                rc = BLZ_ECAPACITY;
-            } else {
-                int64_t o = 0;
-                for (size_t i = r0; i < r1; i++) {
-                    sink->oids[i - r0] = res[i].sid;
-                    sink->counts[i - r0] = res[i].n;
-                    std::memcpy(sink->hsps + o, hs.data() + res[i].off, sizeof(Hsp) * res[i].n);
-                    o += res[i].n;
-                }
-            }
-        }
         r0 = r1;
     }
+    ctx->last_first[nq] = r0;
+    ctx->last_res.swap(res);
+    ctx->last_hs.swap(hs);
+    ctx->last_ovf.clear();
+    if (c.overflow_count > 0) {
+        ctx->last_ovf.resize(std::min<size_t>(c.overflow_count, ctx->ovf_cap));
+        if (!check(ctx, hipMemcpy(ctx->last_ovf.data(), ctx->ovf, sizeof(int64_t) * ctx->last_ovf.size(),
+                                  hipMemcpyDeviceToHost), "overflow list"))
+            return BLZ_EDEVICE;
+    }
+    blz_status rc = BLZ_OK;
+    if (sinks)
+        for (int qcaller = 0; qcaller < nq; qcaller++) {
+            const blz_status r = copy_result(ctx, qcaller, &sinks[qcaller]);
+            if (r != BLZ_OK && rc == BLZ_OK) rc = r;
+        }
     if (stats) {
         stats->subjects += (uint64_t)db->nsubj * (uint64_t)nq;
         stats->residues += db->nres * (uint64_t)nq;
@@ -945,7 +1108,7 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
         stats->ms_seed += ms_seed;
         stats->ms_gapped += ms_gapped;
         stats->ms_total += ms_seed + ms_gapped;
-        stats->seed_launches += (int32_t)plans.size();
+        stats->seed_launches += (int32_t)n_launches;
         stats->gapped_launches += 1;
     }
     if (c.overflow_count > 0 && rc == BLZ_OK) {
@@ -961,26 +1124,46 @@ extern "C" {
 
 blz_status blz_prelim_run(blz_ctx *ctx, const blz_query *q, const blz_db *db, blz_hsp_sink *sink,
                           blz_run_stats *stats) {
-    if (!ctx || !q || !db || q->ctx != ctx || db->ctx != ctx) return BLZ_EINVAL;
+    if (!ctx || !q || !db || q->ctx->device != ctx->device || db->ctx->device != ctx->device) return BLZ_EINVAL;
     (void)hipSetDevice(ctx->device);
-    return run_queries(ctx, &q, 1, db, sink, stats);
+    return run_queries(ctx, &q, 1, db, sink, stats, false);
 }
 
 blz_status blz_prelim_run_batch(blz_ctx *ctx, const blz_query *const *queries, int32_t nq, const blz_db *db,
                                 blz_hsp_sink *sinks, blz_run_stats *stats) {
-    if (!ctx || !queries || nq < 0 || !db || db->ctx != ctx) return BLZ_EINVAL;
+    if (!ctx || !queries || nq < 0 || !db || db->ctx->device != ctx->device) return BLZ_EINVAL;
     if (nq == 0) return BLZ_OK;
     (void)hipSetDevice(ctx->device);
     if ((ctx->tune.seed_variant ? ctx->tune.seed_variant : 2) == 1) {
         blz_status worst = BLZ_OK;
         for (int32_t i = 0; i < nq; i++) {
-            blz_status rc = run_queries(ctx, &queries[i], 1, db, sinks ? &sinks[i] : nullptr, stats);
+            blz_status rc = run_queries(ctx, &queries[i], 1, db, sinks ? &sinks[i] : nullptr, stats, false);
             if (rc == BLZ_EDEVICE || rc == BLZ_ENOMEM || rc == BLZ_EINVAL) return rc;
             if (rc != BLZ_OK && worst == BLZ_OK) worst = rc;
         }
         return worst;
     }
-    return run_queries(ctx, queries, nq, db, sinks, stats);
+    return run_queries(ctx, queries, nq, db, sinks, stats, true);
+}
+
+blz_status blz_result_sizes(const blz_ctx *ctx, int32_t qi, int64_t *n_subjects, int64_t *n_hsps) {
+    if (!ctx || qi < 0 || (size_t)qi + 1 >= ctx->last_first.size()) return BLZ_EINVAL;
+    if (n_subjects) *n_subjects = (int64_t)(ctx->last_first[qi + 1] - ctx->last_first[qi]);
+    if (n_hsps) *n_hsps = ctx->last_nh[qi];
+    return BLZ_OK;
+}
+
+blz_status blz_result_copy(const blz_ctx *ctx, int32_t qi, blz_hsp_sink *sink) {
+    if (!ctx || !sink || qi < 0 || (size_t)qi + 1 >= ctx->last_first.size()) return BLZ_EINVAL;
+    return copy_result(ctx, qi, sink);
+}
+
+blz_status blz_overflow_drain(const blz_ctx *ctx, int64_t *pairs, int64_t cap, int64_t *n) {
+    if (!ctx || !n || cap < 0 || (cap > 0 && !pairs)) return BLZ_EINVAL;
+    *n = (int64_t)ctx->last_ovf.size();
+    if (*n > cap) return BLZ_ECAPACITY;
+    std::copy(ctx->last_ovf.begin(), ctx->last_ovf.end(), pairs);
+    return BLZ_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,340 @@
+// blz_serve.cpp -- request loop over a resident database: continuous batching of queries.
+//
+// Requests arrive from any thread (blz_server_submit).  A preparation thread builds each query's
+// lookup tables on the device.  A dispatcher hands prepared requests to free slots as batches:
+// whenever a slot is free it takes the oldest waiting request and, from the requests waiting
+// with it, those of the same query-size class, up to the batch caps (queries and query letters:
+// the LDS and HBM footprint of a batch grows with both).  Each slot owns a context (its own
+// stream and work buffers) and runs its batch with blz_prelim_run_batch, so the batches of
+// different slots overlap on the device; requests that arrive while batches run are admitted
+// into the next free slot.  A request's latency is measured from submit to its lists being on
+// the host.
+//
+// No reference counterpart: the reference is a per-search hook (blast_engine.c:2983-2996) with
+// no request loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "blz_prelim.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+    uint64_t ticket = 0;
+    std::vector<uint8_t> letters;
+    blz_params params{};
+    blz_query *q = nullptr;
+    Clock::time_point t_submit, t_ready, t_start, t_done;
+    int32_t batch_queries = 0;
+    blz_status status = BLZ_OK;
+    bool done = false;
+    std::vector<int64_t> oids;
+    std::vector<int32_t> counts;
+    std::vector<blz_hsp> hsps;
+};
+
+// Query-size class of a request: the batch former keeps a batch within one class, so the
+// batch's seed launches are sized for similar queries.
+int size_class(int32_t qlen) { return qlen <= 400 ? 0 : qlen <= 1000 ? 1 : qlen <= 2000 ? 2 : 3; }
+
+struct Slot {
+    blz_ctx *ctx = nullptr;
+    std::thread th;
+    std::vector<Request *> batch;  // assigned by the dispatcher, cleared by the slot
+    bool busy = false;
+};
+
+}  // namespace
+
+struct blz_server {
+    blz_server_config cfg{};
+    const blz_db *db = nullptr;
+    blz_ctx *prep_ctx = nullptr;
+    std::vector<Slot> slots;
+    std::mutex mu;
+    std::condition_variable cv;  // any state change
+    std::deque<Request *> to_prepare, waiting;
+    std::unordered_map<uint64_t, Request *> requests;
+    uint64_t next_ticket = 1;
+    bool stopping = false;
+    std::thread prep_th, dispatch_th;
+    blz_server_stats stats{};
+};
+
+namespace {
+
+void prepare_loop(blz_server *s) {
+    for (;;) {
+        Request *r;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv.wait(lk, [&] { return s->stopping || !s->to_prepare.empty(); });
+            if (s->stopping) return;
+            r = s->to_prepare.front();
+            s->to_prepare.pop_front();
+        }
+        const blz_status st =
+            blz_query_prepare(s->prep_ctx, r->letters.data(), (int32_t)r->letters.size(), &r->params, &r->q);
+        std::lock_guard<std::mutex> lk(s->mu);
+        r->t_ready = Clock::now();
+        if (st != BLZ_OK) {
+            r->status = st;
+            r->done = true;
+            r->t_done = r->t_ready;
+        } else {
+            s->waiting.push_back(r);
+        }
+        s->cv.notify_all();
+    }
+}
+
+// Oldest waiting request plus same-class requests behind it, within the caps.
+std::vector<Request *> form_batch(blz_server *s) {
+    std::vector<Request *> b;
+    if (s->waiting.empty()) return b;
+    const int cls = size_class((int32_t)s->waiting.front()->letters.size());
+    int64_t letters = 0;
+    for (auto it = s->waiting.begin(); it != s->waiting.end() && (int)b.size() < s->cfg.max_batch;) {
+        Request *r = *it;
+        const int64_t L = (int64_t)r->letters.size();
+        if (size_class((int32_t)L) == cls && (b.empty() || letters + L <= s->cfg.max_batch_letters)) {
+            b.push_back(r);
+            letters += L;
+            it = s->waiting.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    return b;
+}
+
+void dispatch_loop(blz_server *s) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        s->cv.wait(lk, [&] {
+            if (s->stopping) return true;
+            if (s->waiting.empty()) return false;
+            for (auto &sl : s->slots)
+                if (!sl.busy) return true;
+            return false;
+        });
+        if (s->stopping) return;
+        for (auto &sl : s->slots) {
+            if (sl.busy || s->waiting.empty()) continue;
+            sl.batch = form_batch(s);
+            sl.busy = true;
+            s->stats.batches++;
+        }
+        s->cv.notify_all();
+    }
+}
+
+void slot_loop(blz_server *s, Slot *sl) {
+    for (;;) {
+        std::vector<Request *> batch;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv.wait(lk, [&] { return s->stopping || (sl->busy && !sl->batch.empty()); });
+            if (s->stopping) return;
+            batch.swap(sl->batch);
+        }
+        const Clock::time_point t0 = Clock::now();
+        std::vector<const blz_query *> qs;
+        for (Request *r : batch) qs.push_back(r->q);
+        blz_run_stats st{};
+        const blz_status rc = blz_prelim_run_batch(sl->ctx, qs.data(), (int32_t)qs.size(), s->db, nullptr, &st);
+        for (size_t i = 0; i < batch.size(); i++) {
+            Request *r = batch[i];
+            r->status = rc;
+            if (rc == BLZ_OK || rc == BLZ_EOVERFLOW) {
+                int64_t ns = 0, nh = 0;
+                blz_result_sizes(sl->ctx, (int32_t)i, &ns, &nh);
+                r->oids.resize((size_t)std::max<int64_t>(ns, 1));
+                r->counts.resize((size_t)std::max<int64_t>(ns, 1));
+                r->hsps.resize((size_t)std::max<int64_t>(nh, 1));
+                blz_hsp_sink sink{r->oids.data(), r->counts.data(), ns, r->hsps.data(), nh, 0, 0};
+                const blz_status c = blz_result_copy(sl->ctx, (int32_t)i, &sink);
+                if (c != BLZ_OK) r->status = c;
+                r->oids.resize((size_t)ns);
+                r->counts.resize((size_t)ns);
+                r->hsps.resize((size_t)nh);
+            }
+            blz_query_free(r->q);
+            r->q = nullptr;
+        }
+        std::lock_guard<std::mutex> lk(s->mu);
+        const Clock::time_point t1 = Clock::now();
+        for (Request *r : batch) {
+            r->t_start = t0;
+            r->t_done = t1;
+            r->batch_queries = (int32_t)batch.size();
+            r->done = true;
+        }
+        s->stats.completed += batch.size();
+        s->stats.residues += st.residues;
+        s->stats.device_ms += st.ms_total;
+        sl->busy = false;
+        s->cv.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void blz_server_default_config(blz_server_config *c) {
+    c->slots = 2;
+    c->max_batch = 64;
+    c->max_batch_letters = 32768;
+    c->reserved = 0;
+}
+
+blz_status blz_server_create(int32_t device, const blz_db *db, const blz_server_config *cfg,
+                             const blz_tuning *tuning, blz_server **out) {
+    if (!db || !out) return BLZ_EINVAL;
+    *out = nullptr;
+    blz_server *s = new blz_server();
+    blz_server_default_config(&s->cfg);
+    if (cfg) {
+        if (cfg->slots > 0) s->cfg.slots = cfg->slots;
+        if (cfg->max_batch > 0) s->cfg.max_batch = cfg->max_batch;
+        if (cfg->max_batch_letters > 0) s->cfg.max_batch_letters = cfg->max_batch_letters;
+    }
+    if (s->cfg.slots > 16) s->cfg.slots = 16;
+    s->db = db;
+    blz_status st = blz_ctx_create(device, &s->prep_ctx);
+    s->slots.resize((size_t)s->cfg.slots);
+    for (auto &sl : s->slots) {
+        if (st != BLZ_OK) break;
+        st = blz_ctx_create(device, &sl.ctx);
+        if (st == BLZ_OK && tuning) st = blz_ctx_set_tuning(sl.ctx, tuning);
+        if (st == BLZ_OK) {  // batches differ from one to the next: no graph capture
+            blz_tuning t;
+            blz_ctx_get_tuning(sl.ctx, &t);
+            t.graph = 1;
+            st = blz_ctx_set_tuning(sl.ctx, &t);
+        }
+    }
+    if (st != BLZ_OK) {
+        for (auto &sl : s->slots) blz_ctx_destroy(sl.ctx);
+        blz_ctx_destroy(s->prep_ctx);
+        delete s;
+        return st;
+    }
+    s->prep_th = std::thread(prepare_loop, s);
+    s->dispatch_th = std::thread(dispatch_loop, s);
+    for (auto &sl : s->slots) sl.th = std::thread(slot_loop, s, &sl);
+    *out = s;
+    return BLZ_OK;
+}
+
+void blz_server_destroy(blz_server *s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stopping = true;
+        s->cv.notify_all();
+    }
+    s->prep_th.join();
+    s->dispatch_th.join();
+    for (auto &sl : s->slots) sl.th.join();
+    for (auto &kv : s->requests) {
+        if (kv.second->q) blz_query_free(kv.second->q);
+        delete kv.second;
+    }
+    for (auto &sl : s->slots) blz_ctx_destroy(sl.ctx);
+    blz_ctx_destroy(s->prep_ctx);
+    delete s;
+}
+
+blz_status blz_server_submit(blz_server *s, const uint8_t *query, int32_t qlen, const blz_params *params,
+                             uint64_t *ticket) {
+    if (!s || !query || qlen <= 0 || !ticket) return BLZ_EINVAL;
+    Request *r = new Request();
+    r->t_submit = Clock::now();
+    r->letters.assign(query, query + qlen);
+    if (params)
+        r->params = *params;
+    else
+        blz_default_params(&r->params);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stopping) {
+        delete r;
+        return BLZ_EINVAL;
+    }
+    r->ticket = s->next_ticket++;
+    s->requests[r->ticket] = r;
+    s->to_prepare.push_back(r);
+    s->stats.submitted++;
+    *ticket = r->ticket;
+    s->cv.notify_all();
+    return BLZ_OK;
+}
+
+blz_status blz_server_wait(blz_server *s, uint64_t ticket, int32_t timeout_ms, blz_request_info *info) {
+    if (!s) return BLZ_EINVAL;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->requests.find(ticket);
+    if (it == s->requests.end()) return BLZ_EINVAL;
+    Request *r = it->second;
+    auto ready = [&] { return r->done; };
+    if (timeout_ms < 0)
+        s->cv.wait(lk, ready);
+    else if (!s->cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
+        return BLZ_ETIMEOUT;
+    if (info) {
+        auto us = [](Clock::duration d) { return std::chrono::duration<double, std::micro>(d).count(); };
+        info->status = r->status;
+        info->batch_queries = r->batch_queries;
+        info->latency_us = us(r->t_done - r->t_submit);
+        info->queue_us = us(r->t_start - r->t_submit);
+        info->n_subjects = (int64_t)r->oids.size();
+        info->n_hsps = (int64_t)r->hsps.size();
+    }
+    return BLZ_OK;
+}
+
+blz_status blz_server_copy(blz_server *s, uint64_t ticket, blz_hsp_sink *sink) {
+    if (!s || !sink) return BLZ_EINVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->requests.find(ticket);
+    if (it == s->requests.end() || !it->second->done) return BLZ_EINVAL;
+    Request *r = it->second;
+    sink->n_subjects = (int64_t)r->oids.size();
+    sink->n_hsps = (int64_t)r->hsps.size();
+    if (sink->n_subjects > sink->subject_cap || sink->n_hsps > sink->hsp_cap) return BLZ_ECAPACITY;
+    std::copy(r->oids.begin(), r->oids.end(), sink->oids);
+    std::copy(r->counts.begin(), r->counts.end(), sink->counts);
+    std::copy(r->hsps.begin(), r->hsps.end(), sink->hsps);
+    return BLZ_OK;
+}
+
+blz_status blz_server_release(blz_server *s, uint64_t ticket) {
+    if (!s) return BLZ_EINVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->requests.find(ticket);
+    if (it == s->requests.end() || !it->second->done) return BLZ_EINVAL;
+    delete it->second;
+    s->requests.erase(it);
+    return BLZ_OK;
+}
+
+blz_status blz_server_get_stats(blz_server *s, blz_server_stats *out) {
+    if (!s || !out) return BLZ_EINVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->stats;
+    return BLZ_OK;
+}
+
+}  // extern "C"

@@ -33,7 +33,8 @@ typedef enum {
     BLZ_ENOMEM = 2,     /* host or device allocation failed */
     BLZ_EDEVICE = 3,    /* HIP runtime error or no usable gfx950 device */
     BLZ_ECAPACITY = 4,  /* caller's sink too small: counts are filled, *total says how many */
-    BLZ_EOVERFLOW = 5   /* some subjects exceeded the device caps and were not processed */
+    BLZ_EOVERFLOW = 5,  /* some subjects exceeded the device caps and were not processed */
+    BLZ_ETIMEOUT = 6    /* blz_server_wait: the request is not done yet */
 } blz_status;
 
 /* Search parameters.  The reference passes the two cutoffs at run time
@@ -122,6 +123,12 @@ typedef struct {
     int32_t gapped_wave_classes;/* cost classes 0..k-2 run one wave per subject, the rest one lane;
                                   k = 1..5 (0: all classes on waves) */
     int32_t query_packing;     /* batches: 1 scans each query on its own, else short queries share scans */
+    int32_t pack_len;          /* letters of a query pack (concatenation); 0: 1024 - 128 - window */
+    int32_t bin_ring;          /* query-size bins of a batch: diagonal-ring lengths up to bin_ring,
+                                  2x, 4x, beyond (one seed launch per bin and table placement); 0: 1152 */
+    int32_t graph;             /* batches: 1 launches on the stream every time; else the launch
+                                  sequence is captured in a hipGraph and replayed while the same
+                                  queries and database come again */
 } blz_tuning;
 
 blz_status blz_ctx_set_tuning(blz_ctx *ctx, const blz_tuning *t);
@@ -144,14 +151,76 @@ blz_status blz_query_prepare(blz_ctx *ctx, const uint8_t *query, int32_t qlen,
 void blz_query_free(blz_query *q);
 
 /* Prelim stage of one query over the whole database, one launch per stage.
- * Replaces BLAZE_GPU + BLAST_PreliminarySearchEngineGPU (blast_engine.c:2983-2996). */
+ * Replaces BLAZE_GPU + BLAST_PreliminarySearchEngineGPU (blast_engine.c:2983-2996).
+ * Queries and databases may come from any context on the same device. */
 blz_status blz_prelim_run(blz_ctx *ctx, const blz_query *q, const blz_db *db,
                           blz_hsp_sink *sink, blz_run_stats *stats);
 
 /* Prelim stage of nq queries over the database in one persistent work-stealing launch per
- * stage.  sinks[i] receives query i's lists; stats (optional) sums over the batch. */
+ * query-size bin and stage; short queries share scans (packs).  sinks[i] receives query i's
+ * lists; sinks may be NULL, then the lists are fetched afterwards with blz_result_*.  stats
+ * (optional) sums over the batch.  Unless tuning.graph is 1 the launch sequence is captured
+ * into a hipGraph and replayed while the same queries and database come again; under replay
+ * the stage times are not separated (ms_seed holds the whole device time). */
 blz_status blz_prelim_run_batch(blz_ctx *ctx, const blz_query *const *queries, int32_t nq,
                                 const blz_db *db, blz_hsp_sink *sinks, blz_run_stats *stats);
+
+/* Lists of query qi (its index in the last run's batch; 0 after blz_prelim_run) kept by the
+ * context until its next run: their sizes, and a copy into a caller-sized sink (BLZ_ECAPACITY,
+ * sizes still reported, if it is too small).  No reference counterpart: the reference hands
+ * back survivor OIDs only (gpuBLAZE.cu:1673-1691). */
+blz_status blz_result_sizes(const blz_ctx *ctx, int32_t qi, int64_t *n_subjects, int64_t *n_hsps);
+blz_status blz_result_copy(const blz_ctx *ctx, int32_t qi, blz_hsp_sink *sink);
+
+/* (query index << 40 | subject) of the pairs the last run could not finish within the device
+ * caps (the run returned BLZ_EOVERFLOW); *n receives their number.  The caller re-runs them on
+ * its own path, as BLAZE re-evaluates capped subjects on the CPU (README.md:36-37). */
+blz_status blz_overflow_drain(const blz_ctx *ctx, int64_t *pairs, int64_t cap, int64_t *n);
+
+/* ---- serving: a request loop over a resident database ------------------------------
+ * No reference counterpart (the reference is a per-search hook, blast_engine.c:2983-2996).
+ * Requests are submitted from any thread.  A preparation thread builds their lookup tables;
+ * whenever one of `slots` contexts (each its own stream and buffers) is free, the oldest waiting
+ * request and the same-size-class requests behind it (up to max_batch queries and
+ * max_batch_letters query letters) run there as one batch (blz_prelim_run_batch).  Batches of
+ * different slots overlap on the device, and requests arriving meanwhile join the next batch.
+ * A request's lists are exactly those blz_prelim_run gives for it alone. */
+typedef struct blz_server blz_server;
+
+typedef struct {
+    int32_t slots;             /* batches in flight; 0: 2 (at most 16) */
+    int32_t max_batch;         /* queries per batch; 0: 64 (1: one query per launch) */
+    int64_t max_batch_letters; /* query letters per batch; 0: 32768 */
+    int32_t reserved;
+} blz_server_config;
+
+typedef struct {
+    blz_status status;
+    int32_t batch_queries;     /* size of the batch the request ran in */
+    double latency_us;         /* submit -> lists on the host */
+    double queue_us;           /* submit -> its batch started */
+    int64_t n_subjects, n_hsps;
+} blz_request_info;
+
+typedef struct {
+    uint64_t submitted, completed, batches;
+    uint64_t residues;         /* database residues scanned, summed over requests */
+    double device_ms;          /* device time of the batches, summed over slots */
+} blz_server_stats;
+
+void blz_server_default_config(blz_server_config *cfg);
+/* The database (uploaded with any context on `device`) must outlive the server.  tuning
+ * (optional) is applied to every slot. */
+blz_status blz_server_create(int32_t device, const blz_db *db, const blz_server_config *cfg,
+                             const blz_tuning *tuning, blz_server **out);
+void blz_server_destroy(blz_server *s);
+blz_status blz_server_submit(blz_server *s, const uint8_t *query, int32_t qlen, const blz_params *params,
+                             uint64_t *ticket);
+/* Blocks until the request is done (timeout_ms < 0: no limit; BLZ_ETIMEOUT if it ran out). */
+blz_status blz_server_wait(blz_server *s, uint64_t ticket, int32_t timeout_ms, blz_request_info *info);
+blz_status blz_server_copy(blz_server *s, uint64_t ticket, blz_hsp_sink *sink);
+blz_status blz_server_release(blz_server *s, uint64_t ticket);
+blz_status blz_server_get_stats(blz_server *s, blz_server_stats *out);
 
 /* ---- synthetic data (NCBIstdaa, Robinson-Robinson background) ----------------------- */
 

@@ -221,3 +221,59 @@ def test_packed_batch(engine):
     for packing, (_, stats) in runs.items():
         for k, v in totals.items():
             assert stats[k] == v, (packing, k)
+
+
+def test_graph_replay_and_sinks(engine):
+    """A batch captured as a hipGraph and replayed gives the eager lists on every replay, with
+    another context's eager runs and host synchronisations between the replays; caller-sized
+    sinks receive the same lists, and a too-small sink reports BLZ_ECAPACITY with the sizes."""
+    if engine.get_tuning()["seed_variant"] == 1:
+        pytest.skip("variant 1 runs one query at a time")
+    qs = [blz.synth_query(700 + i, L) for i, L in enumerate([60, 200, 450, 900, 1300])]
+    spec = blz.synth_spec(seed=17, n_subjects=1500, homolog_frac=0.1, ident_min=0.35, ident_max=0.8)
+    res, offs, _ = blz.synth_db(spec, qs)
+    engine.upload_db(res, offs)
+    qhs = [engine.prepare(q) for q in qs]
+    tune = engine.get_tuning()
+    try:
+        engine.set_tuning(**{**tune, "graph": 1})
+        eager, st_e = engine.run_batch(qhs)
+        engine.set_tuning(**{**tune, "graph": 0})
+        with blz.Engine(0) as other:
+            other.upload_db(res, offs)
+            oq = other.prepare(qs[2])
+            want_other, _ = other.run(oq)
+            for rep in range(4):
+                got, st = engine.run_batch(qhs)
+                print(f"[case] replay {rep}: {st['seed_launches']} seed launches", flush=True)
+                for k in ("word_hits", "ungapped_ext", "init_hits", "gapped_ext", "dp_cells", "hsps"):
+                    assert st[k] == st_e[k], (rep, k)
+                for a, b in zip(eager, got):
+                    assert a.as_dict().keys() == b.as_dict().keys()
+                    assert np.array_equal(a.hsps, b.hsps) and np.array_equal(a.oids, b.oids), rep
+                o, _ = other.run(oq)  # eager work on another stream between replays
+                assert np.array_equal(o.hsps, want_other.hsps)
+            other.free_query(oq)
+        # caller-sized sinks through the C ABI
+        import ctypes as C
+        lib = blz.load_library()
+        n = len(qhs)
+        keeps, sinks = [], (blz.Sink * n)()
+        for i, r in enumerate(eager):
+            sk, kp = engine._sink(len(r.oids), len(r.hsps))
+            sinks[i] = sk
+            keeps.append(kp)
+        arr = (C.c_void_p * n)(*[q.value if isinstance(q, C.c_void_p) else q for q in qhs])
+        assert lib.blz_prelim_run_batch(engine.ctx, arr, n, engine.db, sinks, None) == 0
+        for i, r in enumerate(eager):
+            oids, counts, hsps = keeps[i]
+            assert sinks[i].n_subjects == len(r.oids) and sinks[i].n_hsps == len(r.hsps)
+            assert np.array_equal(oids[: len(r.oids)], r.oids) and np.array_equal(hsps[: len(r.hsps)], r.hsps)
+        big = max(range(n), key=lambda i: len(eager[i].hsps))
+        small, _ = engine._sink(len(eager[big].oids), max(0, len(eager[big].hsps) - 1))
+        assert lib.blz_result_copy(engine.ctx, big, C.byref(small)) == 4  # BLZ_ECAPACITY
+        assert small.n_hsps == len(eager[big].hsps)
+    finally:
+        engine.set_tuning(**tune)
+        for h in qhs:
+            engine.free_query(h)
