@@ -111,8 +111,10 @@ hipError_t launch_gapped_lds(const GappedArgs &a, int blocks, hipStream_t st);
 // gapped stage with the DP band in global memory (wide rings), for the redo list
 hipError_t launch_gapped_wide(const GappedArgs &a, int blocks, int threads, hipStream_t st);
 constexpr int kLdsRing = 128;   // LDS DP ring entries per lane (lane-serial pass)
-constexpr int kWaveRing = 1024; // LDS DP ring entries per wave (wave pass)
+constexpr int kWaveRing = 512;  // LDS DP ring entries per wave (wave pass); wider bands are redone
 // gapped stage with one wave per subject of the heavy cost classes
 hipError_t launch_gapped_wave(const GappedArgs &a, int blocks, hipStream_t st);
+// Resident k_gapped_wave workgroups (one wave each) per CU.
+int gapped_wave_blocks_per_cu();
 
 }  // namespace blz

@@ -597,6 +597,12 @@ __global__ void __launch_bounds__(64) k_gapped_wave(GappedArgs a) {
     }
 }
 
+int gapped_wave_blocks_per_cu() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_gapped_wave, 64, 0) != hipSuccess) return 8;
+    return n > 0 ? n : 8;
+}
+
 hipError_t launch_gapped_wave(const GappedArgs &a, int blocks, hipStream_t st) {
     hipLaunchKernelGGL(k_gapped_wave, dim3(blocks), dim3(64), 0, st, a);
     return hipGetLastError();

@@ -885,6 +885,8 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
                          pl.sa.n_entries, pl.waves, pl.blocks, pl.sa.n_slices, seed2_lds_bytes(pl.sa, pl.table, pl.waves));
     }
     const size_t lds_lanes = (size_t)(ctx->tune.gapped_blocks ? ctx->tune.gapped_blocks : ctx->num_cus * 4) * 64;
+    const int heavy_blocks = ctx->num_cus * gapped_wave_blocks_per_cu();  // every resident wave
+    const size_t saved_records = std::max(lds_lanes * kSavedCap, (size_t)heavy_blocks * kSavedCapWave);
     const size_t wide_lanes = (size_t)kGappedBlocks * 64;
     const size_t per_q = std::max<size_t>(nsubj, 1);
     const size_t work_cap = std::min(per_q * (size_t)nq, kWorkCapMax);
@@ -895,7 +897,7 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
         !ensure(ctx, ctx->redo, ctx->redo_cap, kRedoCap, "redo") ||
         !ensure(ctx, ctx->ovf, ctx->ovf_cap, 1 << 16, "ovf") ||
         !ensure(ctx, ctx->dp, ctx->dp_entries, wide_lanes * kDpRing, "dp") ||
-        !ensure(ctx, ctx->saved, ctx->saved_records, lds_lanes * kSavedCap, "saved") ||
+        !ensure(ctx, ctx->saved, ctx->saved_records, saved_records, "saved") ||
         !ensure(ctx, ctx->saved_wide, ctx->saved_wide_records, wide_lanes * kSavedCapWide, "saved wide") ||
         !ensure(ctx, ctx->hsps, ctx->hsp_cap, std::max<size_t>(1 << 22, per_q), "hsps") ||
         !ensure(ctx, ctx->results, ctx->results_cap, work_cap, "results") ||
@@ -984,7 +986,6 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
     gw.hsp_scratch = ctx->saved_wide;
     gw.saved_cap = kSavedCapWide;
     const int lds_blocks = ctx->tune.gapped_blocks ? ctx->tune.gapped_blocks : ctx->num_cus * 4;
-    const int heavy_blocks = ctx->num_cus * 8;
     GappedArgs gh = ga;  // wave pass: per-block saved list of kSavedCapWave
     gh.saved_cap = kSavedCapWave;
     const size_t n_item_ctr = plans.size() + 1;

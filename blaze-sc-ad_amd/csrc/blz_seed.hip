@@ -322,14 +322,15 @@ struct Tables {
     const uint16_t *ent;  // LDS (kTable >= 1) or global
 };
 
+// Lookup entries [pa, pb) of word wd.
 template <int kTable>
-__device__ __forceinline__ void word_range(const Tables &t, uint32_t wd, uint32_t &pa, uint32_t &nh) {
+__device__ __forceinline__ void word_range(const Tables &t, uint32_t wd, uint32_t &pa, uint32_t &pb) {
     if (kTable == 2) {
         pa = t.bb[wd >> 6] + t.rel[wd];
-        nh = t.bb[(wd + 1) >> 6] + t.rel[wd + 1] - pa;
+        pb = t.bb[(wd + 1) >> 6] + t.rel[wd + 1];
     } else {
         pa = t.gprefix[wd];
-        nh = t.gprefix[wd + 1] - pa;
+        pb = t.gprefix[wd + 1];
     }
 }
 
@@ -340,15 +341,24 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
     const int lane = lane_id();
     for (int c0 = 0; c0 + kWord <= slen; c0 += 64) {
         const int s = c0 + lane;
-        uint32_t pa = 0, nh = 0;
-        if (s + kWord <= slen) word_range<kTable>(t, S[s] * 784u + S[s + 1] * 28u + S[s + 2], pa, nh);
+        uint32_t pa = 0, pb = 0;
+        if (s + kWord <= slen) word_range<kTable>(t, S[s] * 784u + S[s + 1] * 28u + S[s + 2], pa, pb);
+        const uint32_t nh = pb - pa;
         const uint32_t inc = wave_inclusive_sum(nh);
         const uint32_t H = last_lane(inc);
         const uint32_t ex = inc - nh;
         n_hits += nh;
         if (H <= (uint32_t)kHitBuf) {
-            // lay the hits out: entry = query offset << 6 | lane of the subject position
-            for (uint32_t k = 0; k < nh; k++) w.hits[ex + k] = ((uint32_t)t.ent[pa + k] << 6) | (uint32_t)lane;
+            // lay the hits out: entry = query offset << 6 | lane of the subject position; four
+            // independent loads per pass (indices clamped to the word's last entry)
+            for (uint32_t k = 0; k < nh; k += 4) {
+                uint32_t e[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) e[j] = t.ent[pa + min(k + j, nh - 1)];
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if (k + j < nh) w.hits[ex + k + j] = (e[j] << 6) | (uint32_t)lane;
+            }
             wave_sync();
             for (uint32_t r = 0; r < H; r += 64) {
                 if (w.cnt[0] > (uint32_t)(kTaskCap - 64) || w.cnt[1] > 64u) flush(c, w, S, slen, n_ext);
