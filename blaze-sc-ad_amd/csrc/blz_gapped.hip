@@ -598,9 +598,12 @@ __global__ void __launch_bounds__(64) k_gapped_wave(GappedArgs a) {
 }
 
 int gapped_wave_blocks_per_cu() {
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_gapped_wave, 64, 0) != hipSuccess) return 8;
-    return n > 0 ? n : 8;
+    static const int n = [] {
+        int k = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, k_gapped_wave, 64, 0) != hipSuccess || k <= 0) k = 8;
+        return k;
+    }();
+    return n;
 }
 
 hipError_t launch_gapped_wave(const GappedArgs &a, int blocks, hipStream_t st) {

@@ -23,6 +23,10 @@
 // Reference counterpart: gpuBLAZE_full, BLAZE_gpu/gpuBLAZE.cu:885-1252.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "blz_common.h"
 #include "blz_dev_common.h"
 #include "blz_device.h"
@@ -684,8 +688,16 @@ size_t seed2_lds_bytes(const SeedArgs &a, int table, int waves) {
 namespace {
 template <int kTable, bool kLong>
 int occupancy_of(int waves, size_t lds) {
+    // the runtime query costs microseconds and a run asks it dozens of times: keep the answers
+    static std::mutex mu;
+    static std::map<std::pair<int, size_t>, int> known;
+    std::lock_guard<std::mutex> lk(mu);
+    const auto key = std::make_pair(waves, lds);
+    auto it = known.find(key);
+    if (it != known.end()) return it->second;
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_seed2<kTable, kLong>, 64 * waves, lds) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_seed2<kTable, kLong>, 64 * waves, lds) != hipSuccess) n = 0;
+    known[key] = n;
     return n;
 }
 }  // namespace
