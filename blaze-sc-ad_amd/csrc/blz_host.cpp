@@ -14,6 +14,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "blz_common.h"
@@ -99,6 +100,9 @@ struct blz_ctx {
     size_t redo_cap = 0;
     uint32_t *pack_meta = nullptr; // member starts and indices of the run's packs
     size_t pack_meta_cap = 0;
+    hipStream_t up_stream = nullptr;  // database uploads: side stream and pinned staging
+    uint8_t *up_buf[2] = {nullptr, nullptr};
+    hipEvent_t up_ev[2] = {nullptr, nullptr};
     int64_t *slices = nullptr;     // subject slice bounds of the run's seed launches
     size_t slices_cap = 0;
     uint32_t *long_ring = nullptr; // seed rings for subjects the LDS ring does not take
@@ -311,6 +315,56 @@ blz_status build_tables(blz_ctx *ctx, const uint8_t *letters, int len, const std
     return BLZ_OK;
 }
 
+// Residues to the device through two pinned staging buffers on a side stream: host threads
+// check and copy chunk k + 1 into one buffer while the DMA of chunk k runs from the other.
+// Replaces the per-chunk gather + pageable H2D of the reference (gpuBLAZE.cu:1587-1618).
+constexpr size_t kUploadChunk = 64u << 20;
+blz_status upload_residues(blz_ctx *ctx, uint8_t *dst, const uint8_t *src, uint64_t n) {
+    if (n == 0) return BLZ_OK;
+    if (!ctx->up_stream && !check(ctx, hipStreamCreateWithFlags(&ctx->up_stream, hipStreamNonBlocking), "upload stream"))
+        return BLZ_EDEVICE;
+    for (int b = 0; b < 2; b++) {
+        if (!ctx->up_buf[b] && !check(ctx, hipHostMalloc((void **)&ctx->up_buf[b], kUploadChunk, hipHostMallocDefault),
+                                      "pinned staging"))
+            return BLZ_ENOMEM;
+        if (!ctx->up_ev[b] && !check(ctx, hipEventCreateWithFlags(&ctx->up_ev[b], hipEventDisableTiming), "event"))
+            return BLZ_EDEVICE;
+    }
+    const int nthreads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    bool bad = false;
+    for (uint64_t off = 0, k = 0; off < n; off += kUploadChunk, k++) {
+        const size_t len = (size_t)std::min<uint64_t>(kUploadChunk, n - off);
+        uint8_t *buf = ctx->up_buf[k & 1];
+        if (k >= 2 && !check(ctx, hipEventSynchronize(ctx->up_ev[k & 1]), "upload"))  // buffer free again
+            return BLZ_EDEVICE;
+        std::vector<std::thread> th;
+        std::vector<char> flag((size_t)nthreads, 0);
+        const size_t part = (len + nthreads - 1) / nthreads;
+        for (int t = 0; t < nthreads; t++)
+            th.emplace_back([&, t] {
+                const size_t a = std::min(len, t * part), e = std::min(len, a + part);
+                uint8_t any = 0;
+                for (size_t i = a; i < e; i++) {
+                    const uint8_t v = src[off + i];
+                    any |= (uint8_t)(v >= kAlpha);
+                    buf[i] = v;
+                }
+                flag[t] = (char)(any != 0);
+            });
+        for (auto &x : th) x.join();
+        for (char f : flag) bad |= f != 0;
+        if (bad) break;
+        if (!check(ctx, hipMemcpyAsync(dst + off, buf, len, hipMemcpyHostToDevice, ctx->up_stream), "db copy") ||
+            !check(ctx, hipEventRecord(ctx->up_ev[k & 1], ctx->up_stream), "event"))
+            return BLZ_EDEVICE;
+    }
+    if (!check(ctx, hipStreamSynchronize(ctx->up_stream), "db copy")) return BLZ_EDEVICE;
+    if (bad) {
+        ctx->err = "residue outside NCBIstdaa (>= 28)";
+        return BLZ_EINVAL;
+    }
+    return BLZ_OK;
+}
 
 }  // namespace
 
@@ -373,6 +427,11 @@ void blz_ctx_destroy(blz_ctx *ctx) {
         if (b) (void)hipFree(b);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    for (int b = 0; b < 2; b++) {
+        if (ctx->up_buf[b]) (void)hipHostFree(ctx->up_buf[b]);
+        if (ctx->up_ev[b]) (void)hipEventDestroy(ctx->up_ev[b]);
+    }
+    if (ctx->up_stream) (void)hipStreamDestroy(ctx->up_stream);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -419,11 +478,6 @@ blz_status blz_db_upload(blz_ctx *ctx, const uint8_t *residues, const uint64_t *
             return BLZ_EINVAL;
         }
     const uint64_t nres = offsets[nsubj];
-    for (uint64_t k = 0; k < nres; k++)
-        if (residues[k] >= kAlpha) {
-            ctx->err = "residue outside NCBIstdaa (>= 28)";
-            return BLZ_EINVAL;
-        }
     (void)hipSetDevice(ctx->device);
     blz_db *db = new blz_db();
     db->ctx = ctx;
@@ -436,8 +490,12 @@ blz_status blz_db_upload(blz_ctx *ctx, const uint8_t *residues, const uint64_t *
         blz_db_free(db);
         return BLZ_ENOMEM;
     }
-    if (!check(ctx, hipMemcpy(db->res, residues, nres, hipMemcpyHostToDevice), "db copy") ||
-        !check(ctx, hipMemset(db->res + nres, 0, 64), "db pad") ||
+    const blz_status up = upload_residues(ctx, db->res, residues, nres);
+    if (up != BLZ_OK) {
+        blz_db_free(db);
+        return up;
+    }
+    if (!check(ctx, hipMemset(db->res + nres, 0, 64), "db pad") ||
         !check(ctx, hipMemcpy(db->offs, offsets, sizeof(uint64_t) * (nsubj + 1), hipMemcpyHostToDevice),
                "db copy")) {
         blz_db_free(db);

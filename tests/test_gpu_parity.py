@@ -139,6 +139,21 @@ def test_empty_db(engine):
     assert len(got.oids) == 0 and stats["word_hits"] == 0
 
 
+def test_upload_rejects_bad_input(engine):
+    """Letters outside NCBIstdaa (anywhere in the staged chunks) and subjects of 65,536 letters
+    or more are rejected with BLZ_EINVAL; nothing is left half-uploaded."""
+    res = np.ones(1000, np.uint8)
+    res[777] = 28
+    offs = np.array([0, 500, 1000], np.uint64)
+    with pytest.raises(blz.BlzError) as e:
+        engine.upload_db(res, offs)
+    assert e.value.status == 1
+    big = np.ones(70000, np.uint8)
+    with pytest.raises(blz.BlzError) as e:
+        engine.upload_db(big, np.array([0, 70000], np.uint64))
+    assert e.value.status == 1
+
+
 def test_params_variants(engine):
     """Non-default cutoffs, X-drops and gap costs are honoured identically."""
     q, res, offs, _ = make(21, 400, 600, homolog_frac=0.1, ident_min=0.4, ident_max=0.6)
