@@ -66,7 +66,8 @@ __device__ __forceinline__ void ring_sync() {
 }
 constexpr int kTaskCap = 128;
 constexpr int kDeferCap = 256;
-constexpr int kHitBuf = 128;                // hits of one step laid out in LDS
+constexpr int kHitBuf = 128;                // most hits of one step that go through the queue
+constexpr int kHitQueue = 192;              // queued hits: fewer than 64 waiting + one step
 constexpr int kSubjBuf = 1024;              // subject letters staged in LDS per wave
 static_assert(kSubjBuf <= kNarrowMax, "staged subjects use the LDS ring");
 
@@ -91,7 +92,7 @@ struct WaveState {
 __host__ __device__ constexpr uint32_t al16(uint32_t b) { return (b + 15u) & ~15u; }
 
 __host__ __device__ inline uint32_t wave_lds_bytes(uint32_t M, uint32_t C) {
-    return al16(2 * M + 4 * (C + 68 + 64 + 4 + kHitBuf) + sizeof(Task) * kTaskCap + sizeof(uint32_t) * kDeferCap +
+    return al16(2 * M + 4 * (C + 68 + 64 + 4 + kHitQueue) + sizeof(Task) * kTaskCap + sizeof(uint32_t) * kDeferCap +
                 kSubjBuf);
 }
 
@@ -343,6 +344,18 @@ template <int kTable, class R, class SP>
 __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP S, int slen, unsigned long long &n_hits,
                              unsigned long long &n_ext) {
     const int lane = lane_id();
+    // Hits wait in a queue (w.hits[head, tail), entries q << 16 | subject offset, subject order)
+    // and go through the state machine 64 at a time, so a round is full even when a step
+    // yields a few more than 64 hits.
+    uint32_t head = 0, tail = 0;
+    auto round = [&]() {
+        if (w.cnt[0] > (uint32_t)(kTaskCap - 64) || w.cnt[1] > 64u) flush(c, w, S, slen, n_ext);
+        const uint32_t n = tail - head < 64u ? tail - head : 64u;
+        const bool pend = (uint32_t)lane < n;
+        const uint32_t e = pend ? w.hits[head + lane] : 0u;
+        resolve_round(c, w, pend, (int)(e >> 16), (int)(e & 0xFFFFu), (uint32_t)lane);
+        head += n;
+    };
     for (int c0 = 0; c0 + kWord <= slen; c0 += 64) {
         const int s = c0 + lane;
         uint32_t pa = 0, pb = 0;
@@ -353,25 +366,29 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
         const uint32_t ex = inc - nh;
         n_hits += nh;
         if (H <= (uint32_t)kHitBuf) {
-            // lay the hits out: entry = query offset << 6 | lane of the subject position; four
-            // independent loads per pass (indices clamped to the word's last entry)
+            if (tail + H > (uint32_t)kHitQueue) {  // move the (< 64) waiting hits to the front
+                const uint32_t rest = tail - head;
+                const uint32_t v = (uint32_t)lane < rest ? w.hits[head + lane] : 0u;
+                wave_sync();
+                if ((uint32_t)lane < rest) w.hits[lane] = v;
+                head = 0;
+                tail = rest;
+            }
+            // append: four independent lookup loads per pass (indices clamped to the word's
+            // last entry)
             for (uint32_t k = 0; k < nh; k += 4) {
                 uint32_t e[4];
 #pragma unroll
                 for (int j = 0; j < 4; j++) e[j] = t.ent[pa + min(k + j, nh - 1)];
 #pragma unroll
                 for (int j = 0; j < 4; j++)
-                    if (k + j < nh) w.hits[ex + k + j] = (e[j] << 6) | (uint32_t)lane;
+                    if (k + j < nh) w.hits[tail + ex + k + j] = (e[j] << 16) | (uint32_t)s;
             }
             wave_sync();
-            for (uint32_t r = 0; r < H; r += 64) {
-                if (w.cnt[0] > (uint32_t)(kTaskCap - 64) || w.cnt[1] > 64u) flush(c, w, S, slen, n_ext);
-                const uint32_t g = r + lane;
-                const bool pend = g < H;
-                const uint32_t e = pend ? w.hits[g] : 0u;
-                resolve_round(c, w, pend, (int)(e >> 6), c0 + (int)(e & 63u), g);
-            }
+            tail += H;
+            while (tail - head >= 64u) round();
         } else {
+            while (tail > head) round();  // earlier hits first
             // dense step (low-complexity letters): find each hit's position by binary search
             w.excl[lane] = ex;
             w.p0[lane] = pa;
@@ -391,11 +408,13 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
                     src = lo;
                     qo = t.ent[w.p0[src] + (g - w.excl[src])];
                 }
-                resolve_round(c, w, pend, qo, c0 + src, g);
+                resolve_round(c, w, pend, qo, c0 + src, (uint32_t)lane);
             }
+            head = tail = 0;
         }
         wave_sync();
     }
+    while (tail > head) round();
     flush(c, w, S, slen, n_ext);
 }
 
@@ -543,7 +562,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
     w.p0 = w.excl + 68;
     w.cnt = w.p0 + 64;
     w.hits = w.cnt + 4;
-    w.task = reinterpret_cast<Task *>(w.hits + kHitBuf);
+    w.task = reinterpret_cast<Task *>(w.hits + kHitQueue);
     w.defer = reinterpret_cast<uint32_t *>(w.task + kTaskCap);
     w.subj = reinterpret_cast<uint8_t *>(w.defer + kDeferCap);
 
