@@ -299,16 +299,19 @@ __device__ int wave_semi_gapped(const int16_t (*mat)[28], const uint8_t *A, cons
             if (am) {
                 if (fa < 0) fa = c0 + (int)__builtin_ctzll(am);
                 la = c0 + 63 - (int)__builtin_clzll(am);
-                const int cmax = (int)last_lane((uint32_t)wave_inclusive_max(alive ? h : kMinIntDp));
-                if (cmax > rmax) {
-                    rmax = cmax;
-                    rmax_b = c0 + (int)__builtin_ctzll(__ballot(alive && h == cmax));
-                }
+            }
+            // The row's best live score: a dead cell lies below the running best, so only a chunk
+            // whose maximum tops the running best (carry_h) can raise it, and the first cell
+            // reaching that maximum is live (everything before it in row order is lower).
+            const int cm = (int)last_lane((uint32_t)hin);
+            if (cm > carry_h) {
+                rmax = cm;
+                rmax_b = c0 + (int)__builtin_ctzll(__ballot(valid && h == cm));
             }
             if (valid)
                 ring[b & wmask] = alive ? make_int2(h, max(h - goe, prev.y - ge)) : make_int2(kMinIntDp, kMinIntDp);
             carry_g = max(carry_g, (int)last_lane((uint32_t)gin));
-            carry_h = max(carry_h, (int)last_lane((uint32_t)hin));
+            carry_h = max(carry_h, cm);
             carry_hp = (int)last_lane((uint32_t)prev.x);
         }
         nc += (unsigned long long)(size - first);

@@ -100,6 +100,8 @@ struct blz_ctx {
     size_t redo_cap = 0;
     uint32_t *pack_meta = nullptr; // member starts and indices of the run's packs
     size_t pack_meta_cap = 0;
+    void *pin = nullptr;           // pinned host staging for the lists coming back
+    size_t pin_cap = 0;
     hipStream_t up_stream = nullptr;  // database uploads: side stream and pinned staging
     uint8_t *up_buf[2] = {nullptr, nullptr};
     hipEvent_t up_ev[2] = {nullptr, nullptr};
@@ -119,17 +121,19 @@ struct blz_ctx {
     std::vector<int64_t> last_ovf;       // overflow pairs: query << 40 | subject
     // launch sequence of the last batch, captured; valid while the same batch comes again and
     // no other run has rewritten the device tables in between
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph[2] = {nullptr, nullptr};          // seed stage, gapped stage
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
     std::vector<uint64_t> graph_key;  // query ids, then the database id
     blz_tuning graph_tune{};
     uint64_t graph_gen = 0;
     int graph_launches = 0;
     void drop_graph() {
-        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        graph_exec = nullptr;
-        graph = nullptr;
+        for (int k = 0; k < 2; k++) {
+            if (graph_exec[k]) (void)hipGraphExecDestroy(graph_exec[k]);
+            if (graph[k]) (void)hipGraphDestroy(graph[k]);
+            graph_exec[k] = nullptr;
+            graph[k] = nullptr;
+        }
         graph_key.clear();
     }
 };
@@ -427,6 +431,7 @@ void blz_ctx_destroy(blz_ctx *ctx) {
         if (b) (void)hipFree(b);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->pin) (void)hipHostFree(ctx->pin);
     for (int b = 0; b < 2; b++) {
         if (ctx->up_buf[b]) (void)hipHostFree(ctx->up_buf[b]);
         if (ctx->up_ev[b]) (void)hipEventDestroy(ctx->up_ev[b]);
@@ -746,6 +751,15 @@ int pick_table(blz_ctx *ctx, const std::vector<QueryDev> &qs, int i, bool compac
     return 0;
 }
 
+// Host-side phase times of a run on stderr when BLZ_LOG_TIME is set (tuning by hand).
+bool log_time() {
+    static const bool on = std::getenv("BLZ_LOG_TIME") != nullptr;
+    return on;
+}
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 // Query qi's lists of the last run into a sink; ECAPACITY (sizes still reported) if it is too small.
 blz_status copy_result(const blz_ctx *ctx, int qi, blz_hsp_sink *sink) {
     const size_t r0 = ctx->last_first[qi], r1 = ctx->last_first[qi + 1];
@@ -765,38 +779,51 @@ blz_status copy_result(const blz_ctx *ctx, int qi, blz_hsp_sink *sink) {
 
 blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq, const blz_db *db, int variant,
                            bool use_graph, const std::vector<uint64_t> &key, int &n_launches);
+
+// The captured stages, bracketed by the stage events.
+blz_status replay_graph(blz_ctx *ctx) {
+    hipStream_t st = ctx->stream;
+    return check(ctx, hipEventRecord(ctx->ev[0], st), "event") &&
+                   check(ctx, hipGraphLaunch(ctx->graph_exec[0], st), "graph launch") &&
+                   check(ctx, hipEventRecord(ctx->ev[1], st), "event") &&
+                   check(ctx, hipGraphLaunch(ctx->graph_exec[1], st), "graph launch") &&
+                   check(ctx, hipEventRecord(ctx->ev[2], st), "event")
+               ? BLZ_OK
+               : BLZ_EDEVICE;
+}
+
 blz_status collect(blz_ctx *ctx, int nq, const blz_db *db, blz_hsp_sink *sinks, blz_run_stats *stats,
                    int n_launches);
 
 // Prelim stage of queries[0..nq) over the database: seed launches (one per bin), the gapped
-// stage, results copied into the sinks.  batch: the caller's batch entry point, whose launch
-// sequence may be captured and replayed as a hipGraph.
+// stage, results copied into the sinks.  The launch sequence is captured as two hipGraphs (one
+// per stage) and replayed while the same queries and database come again.
 blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, const blz_db *db,
-                       blz_hsp_sink *sinks, blz_run_stats *stats, bool batch) {
+                       blz_hsp_sink *sinks, blz_run_stats *stats) {
     for (int i = 0; i < nq; i++)
         if (!queries[i] || queries[i]->ctx->device != ctx->device) return BLZ_EINVAL;
     const int variant = ctx->tune.seed_variant ? ctx->tune.seed_variant : 2;
     const size_t nsubj = (size_t)db->nsubj;
     hipStream_t st = ctx->stream;
     DevCounters *ctr = ctx->ctr;  // one counter block for the run; one item counter per seed launch
-    const bool use_graph = batch && variant != 1 && ctx->tune.graph != 1 && nsubj > 0;
+    const bool use_graph = variant != 1 && ctx->tune.graph != 1 && nsubj > 0;
     std::vector<uint64_t> key;
     if (use_graph) {
         for (int i = 0; i < nq; i++) key.push_back(queries[i]->uid);
         key.push_back(db->uid);
     }
     int n_launches = 0;
-    if (use_graph && ctx->graph_exec && key == ctx->graph_key && ctx->graph_gen == ctx->gen &&
+    if (use_graph && ctx->graph_exec[0] && key == ctx->graph_key && ctx->graph_gen == ctx->gen &&
         std::memcmp(&ctx->graph_tune, &ctx->tune, sizeof(blz_tuning)) == 0) {
-        // the same batch again: the tables on the device are the ones the graph was captured with
+        // the same queries again: the tables on the device are the ones the graphs were captured with
         n_launches = ctx->graph_launches;
-        if (!check(ctx, hipEventRecord(ctx->ev[0], st), "event") ||
-            !check(ctx, hipGraphLaunch(ctx->graph_exec, st), "graph launch") ||
-            !check(ctx, hipEventRecord(ctx->ev[1], st), "event") || !check(ctx, hipEventRecord(ctx->ev[2], st), "event"))
-            return BLZ_EDEVICE;
+        const blz_status rc = replay_graph(ctx);
+        if (rc != BLZ_OK) return rc;
     } else {
+        const double t0 = log_time() ? now_ms() : 0;
         const blz_status rc = plan_and_launch(ctx, queries, nq, db, variant, use_graph, key, n_launches);
         if (rc != BLZ_OK) return rc;
+        if (log_time()) std::fprintf(stderr, "[blz] plan + enqueue %.3f ms\n", now_ms() - t0);
     }
     return collect(ctx, nq, db, sinks, stats, n_launches);
 }
@@ -1047,11 +1074,10 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
     GappedArgs gh = ga;  // wave pass: per-block saved list of kSavedCapWave
     gh.saved_cap = kSavedCapWave;
     const size_t n_item_ctr = plans.size() + 1;
-    // the launch sequence; timed: with the stage events (not under capture)
-    auto enqueue = [&](bool timed) -> bool {
+    // the two stages' launch sequences
+    auto enqueue_seed = [&]() -> bool {
         if (!check(ctx, launch_fill_u32(reinterpret_cast<uint32_t *>(ctr), sizeof(DevCounters) / 4, 0, st), "clear") ||
-            !check(ctx, launch_fill_u32(ctx->item_ctr, n_item_ctr, 0, st), "clear") ||
-            (timed && !check(ctx, hipEventRecord(ctx->ev[0], st), "event")))
+            !check(ctx, launch_fill_u32(ctx->item_ctr, n_item_ctr, 0, st), "clear"))
             return false;
         for (size_t pi = 0; pi < plans.size() && nsubj > 0; pi++) {
             const SeedPlan &pl = plans[pi];
@@ -1059,40 +1085,46 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
                                               : launch_seed2(pl.sa, pl.table, pl.blocks, pl.waves, pl.long_subjects, st);
             if (!check(ctx, e, "k_seed launch")) return false;
         }
-        return (!timed || check(ctx, hipEventRecord(ctx->ev[1], st), "event")) &&
-               (nsubj == 0 || check(ctx, launch_order(ctx->work, ctr, ctx->order, (uint32_t)ctx->work_cap, 512, st),
+        return true;
+    };
+    auto enqueue_gapped = [&]() -> bool {
+        return (nsubj == 0 || check(ctx, launch_order(ctx->work, ctr, ctx->order, (uint32_t)ctx->work_cap, 512, st),
                                     "k_order launch")) &&
                (nsubj == 0 || ga.heavy_classes == 0 ||
                 check(ctx, launch_gapped_wave(gh, heavy_blocks, st), "k_gapped_wave launch")) &&
                (nsubj == 0 || ga.heavy_classes >= kBuckets ||
                 check(ctx, launch_gapped_lds(ga, lds_blocks, st), "k_gapped_lds launch")) &&
-               (nsubj == 0 || check(ctx, launch_gapped_wide(gw, kGappedBlocks, 64, st), "k_gapped_wide launch")) &&
-               (!timed || check(ctx, hipEventRecord(ctx->ev[2], st), "event"));
+               (nsubj == 0 || check(ctx, launch_gapped_wide(gw, kGappedBlocks, 64, st), "k_gapped_wide launch"));
     };
     n_launches = (int)plans.size();
-    if (!use_graph) return enqueue(true) ? BLZ_OK : BLZ_EDEVICE;
-    if (!check(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed), "capture")) return BLZ_EDEVICE;
-    const bool ok = enqueue(false);
-    hipGraph_t g = nullptr;
-    const hipError_t ec = hipStreamEndCapture(st, &g);
-    if (!ok || !check(ctx, ec, "capture")) {
-        if (g) (void)hipGraphDestroy(g);
-        return BLZ_EDEVICE;
-    }
-    ctx->graph = g;
-    if (!check(ctx, hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0), "graph instantiate")) {
-        ctx->drop_graph();
-        return BLZ_EDEVICE;
+    if (!use_graph)
+        return check(ctx, hipEventRecord(ctx->ev[0], st), "event") && enqueue_seed() &&
+                       check(ctx, hipEventRecord(ctx->ev[1], st), "event") && enqueue_gapped() &&
+                       check(ctx, hipEventRecord(ctx->ev[2], st), "event")
+                   ? BLZ_OK
+                   : BLZ_EDEVICE;
+    // one graph per stage, so the stage events can sit between them on the stream
+    for (int k = 0; k < 2; k++) {
+        if (!check(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed), "capture")) return BLZ_EDEVICE;
+        const bool ok = k == 0 ? enqueue_seed() : enqueue_gapped();
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(st, &g);
+        if (!ok || !check(ctx, ec, "capture")) {
+            if (g) (void)hipGraphDestroy(g);
+            ctx->drop_graph();
+            return BLZ_EDEVICE;
+        }
+        ctx->graph[k] = g;
+        if (!check(ctx, hipGraphInstantiate(&ctx->graph_exec[k], g, nullptr, nullptr, 0), "graph instantiate")) {
+            ctx->drop_graph();
+            return BLZ_EDEVICE;
+        }
     }
     ctx->graph_key = key;
     ctx->graph_tune = ctx->tune;
     ctx->graph_gen = ctx->gen;
     ctx->graph_launches = n_launches;
-    if (!check(ctx, hipEventRecord(ctx->ev[0], st), "event") ||
-        !check(ctx, hipGraphLaunch(ctx->graph_exec, st), "graph launch") ||
-        !check(ctx, hipEventRecord(ctx->ev[1], st), "event") || !check(ctx, hipEventRecord(ctx->ev[2], st), "event"))
-        return BLZ_EDEVICE;
-    return BLZ_OK;
+    return replay_graph(ctx);
 }
 
 // Counters and HSP lists of the run just enqueued, into the caller's sinks (caller order,
@@ -1103,27 +1135,47 @@ blz_status collect(blz_ctx *ctx, int nq, const blz_db *db, blz_hsp_sink *sinks, 
     DevCounters *ctr = ctx->ctr;
 
     DevCounters c{};
+    const double t0 = log_time() ? now_ms() : 0;
     if (!check(ctx, hipMemcpyAsync(&c, ctr, sizeof(c), hipMemcpyDeviceToHost, st), "counters") ||
         !check(ctx, hipStreamSynchronize(st), "sync"))
         return BLZ_EDEVICE;
-    std::vector<ResultItem> res(c.result_count);
-    std::vector<Hsp> hs(c.hsp_used <= ctx->hsp_cap ? c.hsp_used : 0);
-    if ((c.result_count &&
-         !check(ctx, hipMemcpyAsync(res.data(), ctx->results, sizeof(ResultItem) * c.result_count,
-                                    hipMemcpyDeviceToHost, st), "results")) ||
-        (!hs.empty() &&
-         !check(ctx, hipMemcpyAsync(hs.data(), ctx->hsps, sizeof(Hsp) * hs.size(), hipMemcpyDeviceToHost, st),
-                "hsps")) ||
+    const double t1 = log_time() ? now_ms() : 0;
+    const size_t n_res = c.result_count, n_hs = c.hsp_used <= ctx->hsp_cap ? c.hsp_used : 0;
+    const size_t res_bytes = (sizeof(ResultItem) * n_res + 255) & ~size_t(255);
+    const size_t need = res_bytes + sizeof(Hsp) * n_hs;
+    if (need > ctx->pin_cap) {  // grow the pinned staging (rarely: it keeps its size)
+        if (ctx->pin) (void)hipHostFree(ctx->pin);
+        ctx->pin = nullptr;
+        ctx->pin_cap = 0;
+        const size_t cap = std::max<size_t>(need + need / 2, 1 << 20);
+        if (!check(ctx, hipHostMalloc(&ctx->pin, cap, hipHostMallocDefault), "pinned staging")) return BLZ_ENOMEM;
+        ctx->pin_cap = cap;
+    }
+    ResultItem *res_pin = static_cast<ResultItem *>(ctx->pin);
+    Hsp *hs_pin = reinterpret_cast<Hsp *>(static_cast<uint8_t *>(ctx->pin) + res_bytes);
+    if ((n_res && !check(ctx, hipMemcpyAsync(res_pin, ctx->results, sizeof(ResultItem) * n_res, hipMemcpyDeviceToHost,
+                                             st), "results")) ||
+        (n_hs && !check(ctx, hipMemcpyAsync(hs_pin, ctx->hsps, sizeof(Hsp) * n_hs, hipMemcpyDeviceToHost, st), "hsps")) ||
         !check(ctx, hipStreamSynchronize(st), "sync"))
         return BLZ_EDEVICE;
+    std::vector<ResultItem> res(res_pin, res_pin + n_res);
+    std::vector<Hsp> hs(hs_pin, hs_pin + n_hs);
+    const double t2 = log_time() ? now_ms() : 0;
     float ms_seed = 0, ms_gapped = 0;
     (void)hipEventElapsedTime(&ms_seed, ctx->ev[0], ctx->ev[1]);
     (void)hipEventElapsedTime(&ms_gapped, ctx->ev[1], ctx->ev[2]);
 
-    // per query (in the caller's order), subjects ascending
-    std::sort(res.begin(), res.end(), [](const ResultItem &a, const ResultItem &b) {
-        return a.qi != b.qi ? a.qi < b.qi : a.sid < b.sid;
-    });
+    // per query (in the caller's order), subjects ascending: sort (query, subject) keys with the
+    // item index, then gather
+    {
+        std::vector<std::pair<uint64_t, uint32_t>> key(res.size());
+        for (size_t i = 0; i < res.size(); i++)
+            key[i] = {((uint64_t)(uint32_t)res[i].qi << 40) | (uint64_t)res[i].sid, (uint32_t)i};
+        std::sort(key.begin(), key.end());
+        std::vector<ResultItem> sorted(res.size());
+        for (size_t i = 0; i < key.size(); i++) sorted[i] = res[key[i].second];
+        res.swap(sorted);
+    }
     ctx->last_first.assign((size_t)nq + 1, 0);
     ctx->last_nh.assign((size_t)nq, 0);
     int64_t total = 0;
@@ -1174,6 +1226,9 @@ blz_status collect(blz_ctx *ctx, int nq, const blz_db *db, blz_hsp_sink *sinks, 
         ctx->err = "subjects exceeded device caps: " + std::to_string(c.overflow_count);
         rc = BLZ_EOVERFLOW;
     }
+    if (log_time())
+        std::fprintf(stderr, "[blz] wait for device %.3f ms, lists back %.3f ms, host lists %.3f ms\n", t1 - t0,
+                     t2 - t1, now_ms() - t2);
     return rc;
 }
 
@@ -1185,7 +1240,7 @@ blz_status blz_prelim_run(blz_ctx *ctx, const blz_query *q, const blz_db *db, bl
                           blz_run_stats *stats) {
     if (!ctx || !q || !db || q->ctx->device != ctx->device || db->ctx->device != ctx->device) return BLZ_EINVAL;
     (void)hipSetDevice(ctx->device);
-    return run_queries(ctx, &q, 1, db, sink, stats, false);
+    return run_queries(ctx, &q, 1, db, sink, stats);
 }
 
 blz_status blz_prelim_run_batch(blz_ctx *ctx, const blz_query *const *queries, int32_t nq, const blz_db *db,
@@ -1196,13 +1251,13 @@ blz_status blz_prelim_run_batch(blz_ctx *ctx, const blz_query *const *queries, i
     if ((ctx->tune.seed_variant ? ctx->tune.seed_variant : 2) == 1) {
         blz_status worst = BLZ_OK;
         for (int32_t i = 0; i < nq; i++) {
-            blz_status rc = run_queries(ctx, &queries[i], 1, db, sinks ? &sinks[i] : nullptr, stats, false);
+            blz_status rc = run_queries(ctx, &queries[i], 1, db, sinks ? &sinks[i] : nullptr, stats);
             if (rc == BLZ_EDEVICE || rc == BLZ_ENOMEM || rc == BLZ_EINVAL) return rc;
             if (rc != BLZ_OK && worst == BLZ_OK) worst = rc;
         }
         return worst;
     }
-    return run_queries(ctx, queries, nq, db, sinks, stats, true);
+    return run_queries(ctx, queries, nq, db, sinks, stats);
 }
 
 blz_status blz_result_sizes(const blz_ctx *ctx, int32_t qi, int64_t *n_subjects, int64_t *n_hsps) {

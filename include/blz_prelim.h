@@ -126,9 +126,9 @@ typedef struct {
     int32_t pack_len;          /* letters of a query pack (concatenation); 0: 1024 - 128 - window */
     int32_t bin_ring;          /* query-size bins of a batch: diagonal-ring lengths up to bin_ring,
                                   2x, 4x, beyond (one seed launch per bin and table placement); 0: 1152 */
-    int32_t graph;             /* batches: 1 launches on the stream every time; else the launch
-                                  sequence is captured in a hipGraph and replayed while the same
-                                  queries and database come again */
+    int32_t graph;             /* 1 launches on the stream every time; else a run's launch sequence
+                                  is captured in hipGraphs and replayed while the same queries and
+                                  database come again */
 } blz_tuning;
 
 blz_status blz_ctx_set_tuning(blz_ctx *ctx, const blz_tuning *t);
@@ -159,9 +159,9 @@ blz_status blz_prelim_run(blz_ctx *ctx, const blz_query *q, const blz_db *db,
 /* Prelim stage of nq queries over the database in one persistent work-stealing launch per
  * query-size bin and stage; short queries share scans (packs).  sinks[i] receives query i's
  * lists; sinks may be NULL, then the lists are fetched afterwards with blz_result_*.  stats
- * (optional) sums over the batch.  Unless tuning.graph is 1 the launch sequence is captured
- * into a hipGraph and replayed while the same queries and database come again; under replay
- * the stage times are not separated (ms_seed holds the whole device time). */
+ * (optional) sums over the batch.  Unless tuning.graph is 1 the launch sequence of a run (this
+ * one and blz_prelim_run) is captured into two hipGraphs, one per stage, and replayed while the
+ * same queries and database come again. */
 blz_status blz_prelim_run_batch(blz_ctx *ctx, const blz_query *const *queries, int32_t nq,
                                 const blz_db *db, blz_hsp_sink *sinks, blz_run_stats *stats);
 
