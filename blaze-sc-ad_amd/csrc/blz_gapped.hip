@@ -239,6 +239,40 @@ __device__ void insertion_sort(Hsp *h, int n) {
 
 enum ItemOutcome { kDone = 0, kRedo = 1, kOverflow = 2 };  // kRedo: band or HSP count past this pass's caps
 
+// Cross-lane steps within a segment of kSeg lanes: the whole wave (64), or one DPP row (16), so
+// that a wave runs four independent DPs side by side.  Values are per lane and uniform within a
+// segment.
+template <int kSeg>
+struct Seg;
+template <>
+struct Seg<64> {
+    static __device__ __forceinline__ int lane() { return lane_id(); }
+    static __device__ __forceinline__ int incl_max(int x) { return wave_inclusive_max(x); }
+    static __device__ __forceinline__ int shift_up1(int x, int in) { return wave_shift_up1(x, in); }
+    static __device__ __forceinline__ int last(int x) { return (int)last_lane((uint32_t)x); }
+    static __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+    static __device__ __forceinline__ int first(int x) { return (int)first_lane((uint32_t)x); }
+};
+template <>
+struct Seg<16> {
+    static __device__ __forceinline__ int lane() { return lane_id() & 15; }
+    static __device__ __forceinline__ int incl_max(int x) {  // row_shr 1/2/4/8 inside the row
+        constexpr int kMin = -2147483647 - 1;
+        x = max(x, __builtin_amdgcn_update_dpp(kMin, x, 0x111, 0xf, 0xf, false));
+        x = max(x, __builtin_amdgcn_update_dpp(kMin, x, 0x112, 0xf, 0xf, false));
+        x = max(x, __builtin_amdgcn_update_dpp(kMin, x, 0x114, 0xf, 0xf, false));
+        x = max(x, __builtin_amdgcn_update_dpp(kMin, x, 0x118, 0xf, 0xf, false));
+        return x;
+    }
+    static __device__ __forceinline__ int shift_up1(int x, int in) {  // row_shr:1, the row's lane 0 gets in
+        return __builtin_amdgcn_update_dpp(in, x, 0x111, 0xf, 0xf, false);
+    }
+    // DPP row_share:n (gfx90a and later): every lane of the row reads the row's lane n
+    static __device__ __forceinline__ int last(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x15F, 0xf, 0xf, false); }
+    static __device__ __forceinline__ uint64_t ballot(bool p) { return (__ballot(p) >> (lane_id() & 48)) & 0xFFFFull; }
+    static __device__ __forceinline__ int first(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x150, 0xf, 0xf, false); }
+};
+
 // Wave-parallel semi-gapped X-drop DP for one extension (all 64 lanes call it with the same
 // arguments).  Rows are swept in order; the cells of a row's band are spread over the lanes, 64
 // per step.  Within a row: D(b) = H_prev(b-1) + s(A_a, B_b) (none for the band's first cell),
@@ -248,10 +282,12 @@ enum ItemOutcome { kDone = 0, kRedo = 1, kOverflow = 2 };  // kRedo: band or HSP
 // bounds and the gap run appended past a surviving last cell follow the same rules as the
 // lane-serial loop; the row-vector form is pinned against NCBI's loop in tests/test_oracle.py.
 // Cells that fail the X-drop store minus infinity for both scores (see LdsRing16).
+template <int kSeg>
 __device__ int wave_semi_gapped(const int16_t (*mat)[28], const uint8_t *A, const uint8_t *B, int M, int N,
                                 int x_dropoff, int gap_open, int gap_extend, bool reverse, int *a_off, int *b_off,
                                 int2 *ring, int wmask, bool *ovf, unsigned long long *cells) {
-    const int lane = lane_id();
+    using S = Seg<kSeg>;
+    const int lane = S::lane();
     const int goe = gap_open + gap_extend;
     const int ge = gap_extend;
     *a_off = 0;
@@ -267,7 +303,7 @@ __device__ int wave_semi_gapped(const int16_t (*mat)[28], const uint8_t *A, cons
         *ovf = true;
         return 0;
     }
-    for (int b = lane; b < size; b += 64) {
+    for (int b = lane; b < size; b += kSeg) {
         const int h = b == 0 ? 0 : -goe - (b - 1) * ge;
         ring[b & wmask] = make_int2(h, h - goe);
     }
@@ -280,22 +316,22 @@ __device__ int wave_semi_gapped(const int16_t (*mat)[28], const uint8_t *A, cons
         int carry_h = best;        // running best over earlier steps
         int carry_hp = kMinIntDp;  // previous row's H left of this step (none for the first cell)
         int fa = -1, la = -1, rmax = kMinIntDp, rmax_b = -1;
-        for (int c0 = first; c0 < size; c0 += 64) {
+        for (int c0 = first; c0 < size; c0 += kSeg) {
             const int b = c0 + lane;
             const bool valid = b < size;
             const int2 prev = valid ? ring[b & wmask] : make_int2(kMinIntDp, kMinIntDp);
-            const int hp_left = wave_shift_up1(prev.x, carry_hp);
+            const int hp_left = S::shift_up1(prev.x, carry_hp);
             const int sb = (valid && b >= 1 && b <= N) ? mrow[reverse ? B[N - b] : B[b]] : 0;
             const int de = valid ? max(hp_left + sb, prev.y) : kMinIntDp;
             const int g = valid ? de + b * ge : kMinIntDp;
-            const int gin = wave_inclusive_max(g);
-            const int gex = max(wave_shift_up1(gin, kMinIntDp), carry_g);
+            const int gin = S::incl_max(g);
+            const int gex = max(S::shift_up1(gin, kMinIntDp), carry_g);
             const int f = gex - goe - (b - 1) * ge;
             const int h = valid ? max(de, f) : kMinIntDp;
-            const int hin = wave_inclusive_max(h);
-            const int hex = max(wave_shift_up1(hin, kMinIntDp), carry_h);
+            const int hin = S::incl_max(h);
+            const int hex = max(S::shift_up1(hin, kMinIntDp), carry_h);
             const bool alive = valid && hex - h <= X;
-            const uint64_t am = __ballot(alive);
+            const uint64_t am = S::ballot(alive);
             if (am) {
                 if (fa < 0) fa = c0 + (int)__builtin_ctzll(am);
                 la = c0 + 63 - (int)__builtin_clzll(am);
@@ -303,16 +339,16 @@ __device__ int wave_semi_gapped(const int16_t (*mat)[28], const uint8_t *A, cons
             // The row's best live score: a dead cell lies below the running best, so only a chunk
             // whose maximum tops the running best (carry_h) can raise it, and the first cell
             // reaching that maximum is live (everything before it in row order is lower).
-            const int cm = (int)last_lane((uint32_t)hin);
+            const int cm = S::last(hin);
             if (cm > carry_h) {
                 rmax = cm;
-                rmax_b = c0 + (int)__builtin_ctzll(__ballot(valid && h == cm));
+                rmax_b = c0 + (int)__builtin_ctzll(S::ballot(valid && h == cm));
             }
             if (valid)
                 ring[b & wmask] = alive ? make_int2(h, max(h - goe, prev.y - ge)) : make_int2(kMinIntDp, kMinIntDp);
-            carry_g = max(carry_g, (int)last_lane((uint32_t)gin));
+            carry_g = max(carry_g, S::last(gin));
             carry_h = max(carry_h, cm);
-            carry_hp = (int)last_lane((uint32_t)prev.x);
+            carry_hp = S::last(prev.x);
         }
         nc += (unsigned long long)(size - first);
         if (fa < 0) break;
@@ -333,7 +369,7 @@ __device__ int wave_semi_gapped(const int16_t (*mat)[28], const uint8_t *A, cons
                     *ovf = true;
                     return 0;
                 }
-                for (int i = lane; i < k; i += 64) {
+                for (int i = lane; i < k; i += kSeg) {
                     const int hv = gr - i * ge;
                     ring[(size + i) & wmask] = make_int2(hv, hv - goe);
                 }
@@ -360,10 +396,10 @@ __device__ int wave_semi_gapped(const int16_t (*mat)[28], const uint8_t *A, cons
 // every lane runs the same control flow, the DP is shared across the lanes and lane 0 does the
 // sorting and the writes).  Returns kRedo if the DP band or the HSP count outgrew this pass's
 // storage (nothing was written), kOverflow if an output pool is full.
-template <bool kWave, class R>
+template <bool kWave, class R, int kSeg = 64>
 __device__ int gapped_item(const GappedArgs &a, const int16_t (*mat)[28], uint32_t widx, R &ring, Hsp *saved,
                            unsigned long long &cells, unsigned long long &gext) {
-    const bool writer = !kWave || lane_id() == 0;
+    const bool writer = !kWave || Seg<kSeg>::lane() == 0;
     const WorkItem wi = a.work[widx];
     InitHit *ih = a.pool + wi.off;
     const int n = wi.n;
@@ -403,10 +439,10 @@ __device__ int gapped_item(const GappedArgs &a, const int16_t (*mat)[28], uint32
         const int s_gs = h.s_start + (q_gs - h.q_start);
         int la, lb, ra, rb, left, right;
         if constexpr (kWave) {
-            left = wave_semi_gapped(mat, Q, S, q_gs + 1, s_gs + 1, qd.x_gapped, qd.gap_open, qd.gap_extend, true, &la,
+            left = wave_semi_gapped<kSeg>(mat, Q, S, q_gs + 1, s_gs + 1, qd.x_gapped, qd.gap_open, qd.gap_extend, true, &la,
                                     &lb, ring.ring, ring.wmask, &ovf, &my_cells);
             if (ovf) break;
-            right = wave_semi_gapped(mat, Q + q_gs, S + s_gs, qlen - q_gs - 1, slen - s_gs - 1, qd.x_gapped,
+            right = wave_semi_gapped<kSeg>(mat, Q + q_gs, S + s_gs, qlen - q_gs - 1, slen - s_gs - 1, qd.x_gapped,
                                      qd.gap_open, qd.gap_extend, false, &ra, &rb, ring.ring, ring.wmask, &ovf,
                                      &my_cells);
         } else {
@@ -458,7 +494,7 @@ __device__ int gapped_item(const GappedArgs &a, const int16_t (*mat)[28], uint32
             a.results[ri] = ResultItem{wi.sid, (uint64_t)base, ns, wi.qi};
         }
     }
-    if (kWave) outcome = (int)first_lane((uint32_t)outcome);
+    if (kWave) outcome = Seg<kSeg>::first(outcome);
     return outcome;
 }
 
@@ -598,6 +634,63 @@ __global__ void __launch_bounds__(64) k_gapped_wave(GappedArgs a) {
         atomicAdd(&a.ctr->dp_cells, cells);
         atomicAdd(&a.ctr->gapped_ext, gext);
     }
+}
+
+// Gapped stage with four subjects per wave, one per 16-lane DPP row: the DP bands of most
+// subjects are a few dozen cells or fewer, so a whole wave per row leaves most lanes idle.
+// Same per-subject steps, caps and outcomes as k_gapped_wave.
+__global__ void __launch_bounds__(64) k_gapped_seg(GappedArgs a) {
+    __shared__ int16_t mat[28][28];
+    __shared__ int2 ring_mem[4][kSegRing];
+    for (int i = threadIdx.x; i < 28 * 28; i += blockDim.x) mat[i / 28][i % 28] = kBlosum62Dev[i / 28][i % 28];
+    __syncthreads();
+    const int lane = lane_id();
+    const int seg = lane >> 4;
+    uint32_t heavy = 0;
+    for (int b = 0; b < a.heavy_classes; b++) heavy += a.ctr->bucket_count[b];
+    const uint32_t n = *a.count < a.count_cap ? *a.count : a.count_cap;
+    if (heavy > n) heavy = n;
+    struct {
+        int2 *ring;
+        int wmask;
+    } ring{ring_mem[seg], kSegRing - 1};
+    Hsp *saved = a.hsp_scratch + ((uint64_t)blockIdx.x * 4 + seg) * a.saved_cap;
+    unsigned long long cells = 0, gext = 0;
+    for (;;) {  // each segment takes its own items; segments leave the loop independently
+        uint32_t i = 0;
+        if ((lane & 15) == 0) i = atomicAdd(a.next_heavy, 1u);
+        i = (uint32_t)__shfl((int)i, lane & 48, 64);
+        if (i >= heavy) break;
+        const uint32_t widx = a.order[i];
+        const int r = gapped_item<true, decltype(ring), 16>(a, mat, widx, ring, saved, cells, gext);
+        if ((lane & 15) == 0) {
+            if (r == kRedo) {
+                const uint32_t k = atomicAdd(&a.ctr->redo_count, 1u);
+                if (k < a.redo_cap) a.redo[k] = widx;
+                else record_overflow(a, widx);
+            } else if (r == kOverflow) {
+                record_overflow(a, widx);
+            }
+        }
+    }
+    if ((lane & 15) == 0) {
+        atomicAdd(&a.ctr->dp_cells, cells);
+        atomicAdd(&a.ctr->gapped_ext, gext);
+    }
+}
+
+int gapped_seg_blocks_per_cu() {
+    static const int n = [] {
+        int k = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, k_gapped_seg, 64, 0) != hipSuccess || k <= 0) k = 8;
+        return k;
+    }();
+    return n;
+}
+
+hipError_t launch_gapped_seg(const GappedArgs &a, int blocks, hipStream_t st) {
+    hipLaunchKernelGGL(k_gapped_seg, dim3(blocks), dim3(64), 0, st, a);
+    return hipGetLastError();
 }
 
 int gapped_wave_blocks_per_cu() {

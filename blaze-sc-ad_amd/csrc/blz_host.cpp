@@ -447,7 +447,8 @@ blz_status blz_ctx_set_tuning(blz_ctx *ctx, const blz_tuning *t) {
         t->seed_blocks_per_cu < 0 || t->gapped_blocks < 0 || t->gapped_threads < 0 || t->gapped_threads > 1024 ||
         (t->gapped_threads % 64) != 0 || t->seed_table < 0 || t->seed_table > 3 || t->gapped_wave_classes < 0 ||
         t->gapped_wave_classes > kBuckets + 1 || t->query_packing < 0 || t->query_packing > 2 || t->pack_len < 0 ||
-        t->pack_len > 16000 || t->bin_ring < 0 || t->graph < 0 || t->graph > 1)
+        t->pack_len > 16000 || t->bin_ring < 0 || t->graph < 0 || t->graph > 1 ||
+        (t->gapped_lanes != 0 && t->gapped_lanes != 16 && t->gapped_lanes != 64))
         return BLZ_EINVAL;
     if (t->query_packing != ctx->tune.query_packing || t->pack_len != ctx->tune.pack_len) {
         // cached packs were made under the old choice
@@ -970,8 +971,10 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
                          pl.sa.n_entries, pl.waves, pl.blocks, pl.sa.n_slices, seed2_lds_bytes(pl.sa, pl.table, pl.waves));
     }
     const size_t lds_lanes = (size_t)(ctx->tune.gapped_blocks ? ctx->tune.gapped_blocks : ctx->num_cus * 4) * 64;
-    const int heavy_blocks = ctx->num_cus * gapped_wave_blocks_per_cu();  // every resident wave
-    const size_t saved_records = std::max(lds_lanes * kSavedCap, (size_t)heavy_blocks * kSavedCapWave);
+    const bool seg16 = ctx->tune.gapped_lanes != 64;  // four subjects per wave
+    const int heavy_blocks = ctx->num_cus * (seg16 ? gapped_seg_blocks_per_cu() : gapped_wave_blocks_per_cu());
+    const size_t saved_records =
+        std::max(lds_lanes * kSavedCap, (size_t)heavy_blocks * (seg16 ? 4 : 1) * kSavedCapWave);
     const size_t wide_lanes = (size_t)kGappedBlocks * 64;
     const size_t per_q = std::max<size_t>(nsubj, 1);
     const size_t work_cap = std::min(per_q * (size_t)nq, kWorkCapMax);
@@ -1091,7 +1094,8 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
         return (nsubj == 0 || check(ctx, launch_order(ctx->work, ctr, ctx->order, (uint32_t)ctx->work_cap, 512, st),
                                     "k_order launch")) &&
                (nsubj == 0 || ga.heavy_classes == 0 ||
-                check(ctx, launch_gapped_wave(gh, heavy_blocks, st), "k_gapped_wave launch")) &&
+                check(ctx, seg16 ? launch_gapped_seg(gh, heavy_blocks, st) : launch_gapped_wave(gh, heavy_blocks, st),
+                      "k_gapped_wave launch")) &&
                (nsubj == 0 || ga.heavy_classes >= kBuckets ||
                 check(ctx, launch_gapped_lds(ga, lds_blocks, st), "k_gapped_lds launch")) &&
                (nsubj == 0 || check(ctx, launch_gapped_wide(gw, kGappedBlocks, 64, st), "k_gapped_wide launch"));

@@ -129,6 +129,8 @@ typedef struct {
     int32_t graph;             /* 1 launches on the stream every time; else a run's launch sequence
                                   is captured in hipGraphs and replayed while the same queries and
                                   database come again */
+    int32_t gapped_lanes;      /* lanes per subject in the wave-parallel gapped DP: 16 (four
+                                  subjects per wave) or 64; 0: 16 */
 } blz_tuning;
 
 blz_status blz_ctx_set_tuning(blz_ctx *ctx, const blz_tuning *t);

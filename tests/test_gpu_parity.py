@@ -13,13 +13,13 @@ pytestmark = pytest.mark.gpu
 NTHREADS = min(16, os.cpu_count() or 1)
 
 
-# seed-stage variants: 1 = global tables with in-place extension, 2 = LDS tables with batched
-# extension
+# seed kernels: 1 = global tables with in-place extension, 2 = the batched kernel; the first
+# also runs the gapped DP with whole waves per subject, the second with four subjects per wave
 @pytest.fixture(scope="module", params=[1, 2], ids=["seed_v1", "seed_v2"])
 def engine(request):
     print(f"\n[engine] seed variant {request.param}", flush=True)
     with blz.Engine(0) as eng:
-        eng.set_tuning(seed_variant=request.param)
+        eng.set_tuning(seed_variant=request.param, gapped_lanes=64 if request.param == 1 else 16)
         yield eng
 
 
