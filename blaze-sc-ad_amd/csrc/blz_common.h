@@ -63,7 +63,8 @@ struct DevCounters {
     unsigned long long pool_used, hsp_used;
     unsigned int work_count, result_count, overflow_count, next_subject, next_work, next_item;
     unsigned int bucket_count[kBuckets], bucket_cursor[kBuckets];
-    unsigned int redo_count, next_redo, next_heavy, pad2;
+    unsigned int redo_count, next_redo, next_heavy, retry_count;
+    unsigned int next_retry, pad3;
 };
 
 }  // namespace blz

@@ -58,6 +58,15 @@ __device__ __forceinline__ uint64_t first_lane64(uint64_t v) {
     return ((uint64_t)first_lane((uint32_t)(v >> 32)) << 32) | first_lane((uint32_t)v);
 }
 
+// Order of a subject's init hits before the gapped stage (NCBI BLAST_GetGappedScore's sort:
+// score descending, subject start, length descending, query start).
+__device__ __forceinline__ bool init_before(const InitHit &a, const InitHit &b) {
+    if (a.score != b.score) return a.score > b.score;
+    if (a.s_start != b.s_start) return a.s_start < b.s_start;
+    if (a.len != b.len) return a.len > b.len;
+    return a.q_start < b.q_start;
+}
+
 // ---------------------------------------------------------------------------------------
 // Ungapped X-drop walks (NCBI s_BlastAaExtendLeft/Right; the reference restates the stop
 // rules at gpuBLAZE.cu:680-883).  Lane-serial; the matrix row comes from LDS.

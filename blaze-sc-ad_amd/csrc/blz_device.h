@@ -67,6 +67,9 @@ struct SeedArgs {
     const int64_t *long_ids;   // long-subject launch: the subjects longer than kSeedNarrowMax
     const int64_t *slice;      // subject slices of about equal residues: [slice[j], slice[j+1])
     uint32_t n_slices;         // work items = packs x slices, pack-major, last slice first
+    // retry launch: the (query << kOvfShift | subject) pairs the main launches could not stage,
+    // rescanned with qd = the run's per-query table and stage_cap records per wave
+    const int64_t *retry;
 };
 
 struct GappedArgs {
@@ -98,6 +101,10 @@ struct GappedArgs {
 size_t seed_lds_bytes(uint32_t ring_words, int waves_per_block);
 size_t seed2_lds_bytes(const SeedArgs &a, int table, int waves);
 hipError_t launch_seed2(const SeedArgs &a, int table, int blocks, int waves, bool long_subjects, hipStream_t st);
+// Second chance for the subjects the seed launches recorded as past a cap: moves the overflow
+// list to `retry` (and clears it), then rescans those pairs with a larger stage.
+hipError_t launch_seed_retry(const SeedArgs &a, int64_t *retry, int blocks, int waves, hipStream_t st);
+size_t seed_retry_lds_bytes(const SeedArgs &a, int waves);
 // Resident workgroups per CU of that launch shape (registers and LDS), 0 if it cannot run.
 int seed2_blocks_per_cu(const SeedArgs &a, int table, int waves, bool long_subjects);
 hipError_t launch_seed(const SeedArgs &a, int blocks, int waves_per_block, hipStream_t st);

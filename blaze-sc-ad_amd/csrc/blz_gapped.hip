@@ -162,13 +162,6 @@ __device__ int semi_gapped(const int16_t (*mat)[28], const uint8_t *A, const uin
     return best_score;
 }
 
-__device__ __forceinline__ bool init_before(const InitHit &a, const InitHit &b) {
-    if (a.score != b.score) return a.score > b.score;
-    if (a.s_start != b.s_start) return a.s_start < b.s_start;
-    if (a.len != b.len) return a.len > b.len;
-    return a.q_start < b.q_start;
-}
-
 // Gapped start: middle of a short segment, else the centre of its best 11-letter window when
 // that window scores above 0, else the segment start (reference restatement at
 // gpuBLAZE.cu:1166-1205).

@@ -109,6 +109,12 @@ struct blz_ctx {
     size_t slices_cap = 0;
     uint32_t *long_ring = nullptr; // seed rings for subjects the LDS ring does not take
     size_t long_ring_cap = 0;
+    int64_t *retry_list = nullptr; // seed pairs past a cap, rescanned by the retry launch
+    size_t retry_list_cap = 0;
+    InitHit *retry_stage = nullptr;
+    size_t retry_stage_cap = 0;
+    uint32_t *retry_ring = nullptr;
+    size_t retry_ring_cap = 0;
     // packs of the last batch (keyed by query ids), reused while the same queries come again
     std::vector<uint64_t> pack_key;
     std::vector<Pack> packs;
@@ -172,6 +178,8 @@ constexpr int kSeedWavesMax = 4;
 constexpr int kGappedBlocks = 64;        // wide gapped pass (redo list): blocks x 64 lanes
 constexpr size_t kRedoCap = 1 << 20;
 constexpr size_t kWorkCapMax = 48u << 20; // work items (subjects with init hits) per run
+constexpr int kRetryCap = 1 << 16;        // retry launch: init hits per subject
+constexpr int kRetryBlocks = 16, kRetryWaves = 4;
 
 bool check(blz_ctx *ctx, hipError_t e, const char *what) {
     if (e == hipSuccess) return true;
@@ -422,7 +430,7 @@ void blz_ctx_destroy(blz_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    void *bufs[] = {ctx->long_ring, ctx->slices, ctx->ctr,   ctx->stage,   ctx->pool,    ctx->work, ctx->ovf,     ctx->dp,
+    void *bufs[] = {ctx->retry_list, ctx->retry_stage, ctx->retry_ring, ctx->long_ring, ctx->slices, ctx->ctr,   ctx->stage,   ctx->pool,    ctx->work, ctx->ovf,     ctx->dp,
                     ctx->saved, ctx->hsps,    ctx->results, ctx->qdev, ctx->item_ctr,
                     ctx->order, ctx->redo, ctx->saved_wide, ctx->pack_meta};
     ctx->drop_graph();
@@ -939,8 +947,20 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
                 plans.push_back(lp);
             }
         }
+        // every query is also a pack of one of its own (the retry launch scans it alone)
+        std::vector<size_t> own_off(nq);
+        for (int i = 0; i < nq; i++) {
+            own_off[i] = meta.size();
+            meta.push_back(0);
+            meta.push_back((uint32_t)qs[i].qlen);
+            meta.push_back((uint32_t)i);
+        }
         if (!ensure(ctx, ctx->pack_meta, ctx->pack_meta_cap, std::max<size_t>(meta.size(), 1), "pack table"))
             return BLZ_ENOMEM;
+        for (int i = 0; i < nq; i++) {
+            qs[i].member_start = ctx->pack_meta + own_off[i];
+            qs[i].member_idx = qs[i].member_start + 2;
+        }
         for (int k : order) {
             QueryDev d = pq[k];
             d.member_start = ctx->pack_meta + meta_off[k];
@@ -978,6 +998,8 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
     const size_t wide_lanes = (size_t)kGappedBlocks * 64;
     const size_t per_q = std::max<size_t>(nsubj, 1);
     const size_t work_cap = std::min(per_q * (size_t)nq, kWorkCapMax);
+    uint32_t max_ring_len = 8;  // the retry launch's rings: any query of the run
+    for (int i = 0; i < nq; i++) max_ring_len = std::max(max_ring_len, qs[i].ring_len);
     if (!ensure(ctx, ctx->stage, ctx->stage_records, max_blocks * kStageCap, "stage") ||
         !ensure(ctx, ctx->pool, ctx->pool_cap, std::max<size_t>(1 << 22, 2 * work_cap), "pool") ||
         !ensure(ctx, ctx->work, ctx->work_cap, work_cap, "work") ||
@@ -992,7 +1014,12 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
         !ensure(ctx, ctx->qdev, ctx->qdev_cap, qs.size(), "query table") ||
         !ensure(ctx, ctx->item_ctr, ctx->item_ctr_cap, plans.size() + 1, "item counters") ||
         !ensure(ctx, ctx->slices, ctx->slices_cap, std::max<size_t>(slices.size(), 1), "slices") ||
-        (long_ring && !ensure(ctx, ctx->long_ring, ctx->long_ring_cap, long_ring, "long-subject rings")))
+        (long_ring && !ensure(ctx, ctx->long_ring, ctx->long_ring_cap, long_ring, "long-subject rings")) ||
+        (variant != 1 && (!ensure(ctx, ctx->retry_list, ctx->retry_list_cap, 1 << 16, "retry list") ||
+                          !ensure(ctx, ctx->retry_stage, ctx->retry_stage_cap,
+                                  (size_t)kRetryBlocks * kRetryWaves * kRetryCap, "retry stage") ||
+                          !ensure(ctx, ctx->retry_ring, ctx->retry_ring_cap,
+                                  (size_t)kRetryBlocks * kRetryWaves * max_ring_len, "retry rings"))))
         return BLZ_ENOMEM;
 
     if (!check(ctx, hipMemcpyAsync(ctx->qdev, qs.data(), sizeof(QueryDev) * qs.size(), hipMemcpyHostToDevice, st),
@@ -1087,6 +1114,26 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
             const hipError_t e = variant == 1 ? launch_seed(pl.sa, pl.blocks, pl.waves, st)
                                               : launch_seed2(pl.sa, pl.table, pl.blocks, pl.waves, pl.long_subjects, st);
             if (!check(ctx, e, "k_seed launch")) return false;
+        }
+        if (variant != 1 && nsubj > 0) {  // second chance for the subjects past a cap
+            SeedArgs r{};
+            r.res = db->res;
+            r.offs = db->offs;
+            r.qd = ctx->qdev;  // the run's own queries
+            r.claim_mask = 127;
+            r.ring_len = max_ring_len;
+            r.stage = ctx->retry_stage;
+            r.stage_cap = kRetryCap;
+            r.pool = ctx->pool;
+            r.pool_cap = ctx->pool_cap;
+            r.work = ctx->work;
+            r.work_cap = (uint32_t)ctx->work_cap;
+            r.ovf = ctx->ovf;
+            r.ovf_cap = (uint32_t)ctx->ovf_cap;
+            r.ctr = ctr;
+            r.long_ring = ctx->retry_ring;
+            if (!check(ctx, launch_seed_retry(r, ctx->retry_list, kRetryBlocks, kRetryWaves, st), "retry launch"))
+                return false;
         }
         return true;
     };

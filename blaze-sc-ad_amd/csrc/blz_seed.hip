@@ -446,7 +446,15 @@ __device__ void emit_init_hits(const SeedArgs &a, const QueryDev &qd, WaveState<
         int best = 0;
         for (uint32_t k = lane; k < nih; k += 64) {
             const InitHit h = c.stage[k];
-            a.pool[base + k] = h;
+            uint32_t r = k;
+            if (nih > 32) {  // long lists go out in the gapped stage's order (its sort is then linear)
+                r = 0;
+                for (uint32_t j = 0; j < nih; j++) {
+                    const InitHit o = c.stage[j];
+                    r += (uint32_t)(init_before(o, h) || (j < k && !init_before(h, o)));
+                }
+            }
+            a.pool[base + r] = h;
             best = h.score > best ? h.score : best;
         }
         best = (int)last_lane((uint32_t)wave_inclusive_max(best));
@@ -702,6 +710,102 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
 size_t seed2_lds_bytes(const SeedArgs &a, int table, int waves) {
     return block_lds_bytes(table, (uint32_t)a.qlen, a.n_entries) +
            (size_t)waves * wave_lds_bytes(a.ring_len, a.claim_mask + 1);
+}
+
+// Second chance for subjects the seed launches recorded as past a cap (init hits beyond the
+// per-wave stage, pool or work list full).  One workgroup moves the overflow list to the retry
+// list and clears it (entries past its capacity stay counted as overflow).
+__global__ void __launch_bounds__(256) k_seed_retry_prep(DevCounters *ctr, const int64_t *ovf, uint32_t ovf_cap,
+                                                         int64_t *retry) {
+    const uint32_t total = ctr->overflow_count;
+    const uint32_t n = total < ovf_cap ? total : ovf_cap;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) retry[i] = ovf[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ctr->retry_count = n;
+        ctr->next_retry = 0;
+        ctr->overflow_count = total - n;
+    }
+}
+
+// Rescans each retried (query, subject) pair with the query's own tables in global memory, a
+// 32-bit diagonal ring in global memory and a stage of a.stage_cap records per wave.  The pair's
+// word hits and extensions were counted by the first pass and are not counted again.
+__global__ void __launch_bounds__(256) k_seed_retry(SeedArgs a) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int lane = lane_id();
+    const int wave = threadIdx.x >> 6;
+    const uint32_t C = a.claim_mask + 1;
+    int16_t(*mat)[28] = reinterpret_cast<int16_t(*)[28]>(smem);
+    uint8_t *wb = smem + al16(28 * 28 * 2) + (size_t)wave * wave_lds_bytes(8, C);
+    WaveState<uint32_t> w;
+    const uint64_t gwave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+    w.ring = a.long_ring + gwave * a.ring_len;
+    w.claim = reinterpret_cast<uint32_t *>(wb + 16);
+    w.excl = w.claim + C;
+    w.p0 = w.excl + 68;
+    w.cnt = w.p0 + 64;
+    w.hits = w.cnt + 4;
+    w.task = reinterpret_cast<Task *>(w.hits + kHitQueue);
+    w.defer = reinterpret_cast<uint32_t *>(w.task + kTaskCap);
+    w.subj = reinterpret_cast<uint8_t *>(w.defer + kDeferCap);
+    for (int i = threadIdx.x; i < 28 * 28; i += blockDim.x) mat[i / 28][i % 28] = kBlosum62Dev[i / 28][i % 28];
+    for (uint32_t i = lane; i < C; i += 64) w.claim[i] = 0xFFFFFFFFu;
+    if (lane < 4) w.cnt[lane] = 0;
+    __syncthreads();
+    Ctx c;
+    c.mat = mat;
+    c.stage = a.stage + gwave * (uint64_t)a.stage_cap;
+    c.stage_cap = a.stage_cap;
+    c.claim_mask = a.claim_mask;
+    unsigned long long n_hits = 0, n_ext = 0;
+    for (;;) {
+        uint32_t k = 0;
+        if (lane == 0) k = atomicAdd(&a.ctr->next_retry, 1u);
+        k = first_lane(k);
+        if (k >= a.ctr->retry_count) break;
+        const int64_t e = a.retry[k];
+        const int qi = (int)(e >> kOvfShift);
+        const int64_t sid = e & ((int64_t(1) << kOvfShift) - 1);
+        const QueryDev &qd = a.qd[qi];
+        c.Q = qd.query;
+        c.qlen = qd.qlen;
+        c.ring_len = qd.ring_len;
+        c.ring_base = qd.ring_base;
+        c.ring_magic = qd.ring_magic;
+        c.window = qd.window;
+        c.xdrop = qd.x_ungapped;
+        c.cutoff = qd.ungapped_cutoff;
+        Tables t{};
+        t.gprefix = qd.prefix;
+        t.ent = qd.entries;
+        const uint64_t o0 = a.offs[sid];
+        const int slen = (int)(a.offs[sid + 1] - o0);
+        for (uint32_t i = lane; i < qd.ring_len; i += 64) w.ring[i] = 0;
+        ring_sync<uint32_t>();
+        scan_subject<0>(c, w, t, a.res + o0, slen, n_hits, n_ext);
+        wave_sync();
+        const uint32_t nih = w.cnt[2];
+        wave_sync();
+        if (nih > 0) {
+            emit_init_hits(a, qd, w, c, sid, nih);
+            wave_sync();
+            if (lane == 0) w.cnt[2] = 0;
+            wave_sync();
+        }
+    }
+}
+
+size_t seed_retry_lds_bytes(const SeedArgs &a, int waves) {
+    return al16(28 * 28 * 2) + (size_t)waves * wave_lds_bytes(8, a.claim_mask + 1);
+}
+
+hipError_t launch_seed_retry(const SeedArgs &a, int64_t *retry, int blocks, int waves, hipStream_t st) {
+    hipLaunchKernelGGL(k_seed_retry_prep, dim3(1), dim3(256), 0, st, a.ctr, a.ovf, a.ovf_cap, retry);
+    SeedArgs r = a;
+    r.retry = retry;
+    hipLaunchKernelGGL(k_seed_retry, dim3(blocks), dim3(64 * waves), seed_retry_lds_bytes(a, waves), st, r);
+    return hipGetLastError();
 }
 
 namespace {

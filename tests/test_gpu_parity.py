@@ -127,6 +127,34 @@ def test_long_subject_rings(engine):
     compare(engine, q, np.concatenate(subs), offs)
 
 
+def test_init_hit_cap(engine):
+    """A subject with more init hits than the per-wave stage holds (6,704 > 4,096; low ungapped
+    cutoff) and more HSPs than the wave pass keeps (1,799).  The batched kernel rescans it in its
+    retry launch and the wide pass finishes it: lists identical to the oracle's.  The first seed
+    kernel has no retry: it reports the pair (blz_overflow_drain) and every other list is exact."""
+    q = blz.synth_query(77, 2000)
+    subs = [blz.synth_query(78, 40000), blz.synth_query(79, 300), blz.synth_query(80, 5000)]
+    offs = np.zeros(len(subs) + 1, dtype=np.uint64)
+    offs[1:] = np.cumsum([len(s) for s in subs])
+    res = np.concatenate(subs)
+    kw = dict(ungapped_cutoff=12, gapped_cutoff=22, x_gapped=12)
+    if engine.get_tuning()["seed_variant"] != 1:
+        compare(engine, q, res, offs, blz.default_params(**kw), oracle.params(**kw))
+        return
+    engine.upload_db(res, offs)
+    qh = engine.prepare(q, blz.default_params(**kw))
+    try:
+        got, _ = engine.run(qh, allow_overflow=True)
+        assert engine.overflow_pairs() == [(0, 0)]
+    finally:
+        engine.free_query(qh)
+    counts, hsps, _ = oracle_run(q, res, offs, oracle.params(**kw))
+    want = oracle.as_lists(counts, hsps)
+    have = got.as_dict()
+    assert 0 not in have and sorted(set(want) - {0}) == sorted(have)
+    assert all(np.array_equal(want[k], have[k]) for k in have)
+
+
 def test_empty_db(engine):
     q = blz.synth_query(1, 100)
     res = np.zeros(0, np.uint8)
