@@ -555,7 +555,10 @@ blz_status blz_query_prepare(blz_ctx *ctx, const uint8_t *query, int32_t qlen, c
         p = *params;
     else
         blz_default_params(&p);
-    if (p.window < 1 || p.x_ungapped < 1 || p.x_gapped < 0 || p.gap_open < 0 || p.gap_extend < 1) {
+    // walks stop at sequence ends and pack separators on letter 0's score of -32768, which
+    // must exceed any ungapped X-drop
+    if (p.window < 1 || p.x_ungapped < 1 || p.x_ungapped > 16384 || p.x_gapped < 0 || p.gap_open < 0 ||
+        p.gap_extend < 1) {
         ctx->err = "invalid parameters";
         return BLZ_EINVAL;
     }

@@ -69,6 +69,7 @@ constexpr int kDeferCap = 256;
 constexpr int kHitBuf = 128;                // most hits of one step that go through the queue
 constexpr int kHitQueue = 192;              // queued hits: fewer than 64 waiting + one step
 constexpr int kSubjBuf = 1024;              // subject letters staged in LDS per wave
+constexpr int kSeqPad = 16;                 // zero letters before and after a sequence staged in LDS
 static_assert(kSubjBuf <= kNarrowMax, "staged subjects use the LDS ring");
 
 struct Task {
@@ -93,11 +94,11 @@ __host__ __device__ constexpr uint32_t al16(uint32_t b) { return (b + 15u) & ~15
 
 __host__ __device__ inline uint32_t wave_lds_bytes(uint32_t M, uint32_t C) {
     return al16(2 * M + 4 * (C + 68 + 64 + 4 + kHitQueue) + sizeof(Task) * kTaskCap + sizeof(uint32_t) * kDeferCap +
-                kSubjBuf);
+                kSubjBuf + 2 * kSeqPad);
 }
 
 __host__ __device__ inline uint32_t block_lds_bytes(int table, uint32_t qlen, uint32_t n_entries) {
-    uint32_t b = 16 + al16(28 * 28 * 2) + al16(qlen + 16);
+    uint32_t b = 16 + al16(28 * 28 * 2) + kSeqPad + al16(qlen + kSeqPad);
     if (table >= 1) b += al16(n_entries * 2 + 4);
     if (table >= 2) b += al16((kWords + 1) * 2) + al16(kRelBlocks * 4);
     return b;
@@ -122,74 +123,91 @@ __device__ __forceinline__ uint32_t ring_slot(const Ctx &c, int q, int s) {
     return x - __umulhi(x, c.ring_magic) * c.ring_len;
 }
 
-// Ungapped walks over letters held in one address space (LDS or global), four letters per
-// step: the loads of a step are independent, so one memory latency covers four positions.
-// Same stop rules as extend_left / extend_right.
-template <class SP>
+// Ungapped walks (NCBI s_BlastAaExtendLeft / s_BlastAaExtendRight, the stop rules of
+// extend_left / extend_right), four letters per step with no branch inside a step: the loads of
+// a step are independent, and a lane that stops keeps its score, best and stop position while
+// the step's remaining letters are masked out.
+//
+// kPad: the query and the subject sit in LDS with at least 4 letters of value 0 before and after
+// them.  Letter 0 scores -32768 against every letter, so at a sequence end the walk stops by its
+// own X-drop test (right: score <= 0) with the maximum, length and stop position it had there,
+// exactly where the bounded loop ends; no bounds test per letter.  Otherwise a position past an
+// end reads an in-bounds letter and scores kEndScore instead.
+constexpr int kEndScore = -32768;
+
+template <bool kPad, class SP>
 __device__ __forceinline__ int walk_left(const Ctx &c, SP S, int q_off, int s_off, int dropoff, int *length) {
-    const int n = s_off < q_off ? s_off : q_off;
+    const int n = s_off < q_off ? s_off : q_off;  // last k inside both sequences
     int maxscore = 0, score = 0, best = -1;
-    for (int k0 = 0; k0 <= n; k0 += 4) {
+    bool live = true;
+    for (int k0 = 0;; k0 += 4) {
         int sc[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const int k = k0 + j <= n ? k0 + j : n;
-            sc[j] = c.mat[c.Q[q_off - k]][S[s_off - k]];
-        }
-        bool stop = false;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            if (!stop && k0 + j <= n) {
-                score += sc[j];
-                if (score > maxscore) {
-                    maxscore = score;
-                    best = k0 + j;
-                }
-                if (maxscore - score >= dropoff) stop = true;
+            const int k = k0 + j;
+            if constexpr (kPad) {
+                sc[j] = c.mat[c.Q[q_off - k]][S[s_off - k]];
+            } else {
+                const int kk = k <= n ? k : n;
+                const int v = c.mat[c.Q[q_off - kk]][S[s_off - kk]];
+                sc[j] = k <= n ? v : kEndScore;
             }
         }
-        if (stop) break;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int s1 = score + sc[j];
+            const bool up = live && s1 > maxscore;
+            best = up ? k0 + j : best;
+            maxscore = up ? s1 : maxscore;
+            live = live && maxscore - s1 < dropoff;
+            score = s1;
+        }
+        if (!live) break;
     }
     *length = best + 1;
     return maxscore;
 }
 
-template <class SP>
+template <bool kPad, class SP>
 __device__ __forceinline__ int walk_right(const Ctx &c, SP S, int slen, int q_off, int s_off, int dropoff,
                                           int maxscore, int *length, int *s_last_off) {
-    const int n = slen - s_off < c.qlen - q_off ? slen - s_off : c.qlen - q_off;
-    int score = maxscore, best = -1, i = n;
-    for (int k0 = 0; k0 < n; k0 += 4) {
+    const int n = slen - s_off < c.qlen - q_off ? slen - s_off : c.qlen - q_off;  // letters inside both
+    const int nl = n > 0 ? n - 1 : 0;
+    int score = maxscore, best = -1, stop_k = 0;
+    bool live = true;
+    for (int k0 = 0;; k0 += 4) {
         int sc[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const int k = k0 + j < n ? k0 + j : n - 1;
-            sc[j] = c.mat[c.Q[q_off + k]][S[s_off + k]];
-        }
-        bool stop = false;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            if (!stop && k0 + j < n) {
-                score += sc[j];
-                if (score > maxscore) {
-                    maxscore = score;
-                    best = k0 + j;
-                }
-                if (score <= 0 || maxscore - score >= dropoff) {
-                    stop = true;
-                    i = k0 + j;
-                }
+            const int k = k0 + j;
+            if constexpr (kPad) {
+                sc[j] = c.mat[c.Q[q_off + k]][S[s_off + k]];
+            } else {
+                const int kk = k < n ? k : nl;
+                const int v = c.mat[c.Q[q_off + kk]][S[s_off + kk]];
+                sc[j] = k < n ? v : kEndScore;
             }
         }
-        if (stop) break;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int s1 = score + sc[j];
+            const bool up = live && s1 > maxscore;
+            best = up ? k0 + j : best;
+            maxscore = up ? s1 : maxscore;
+            const bool stop = live && (s1 <= 0 || maxscore - s1 >= dropoff);
+            stop_k = stop ? k0 + j : stop_k;
+            live = live && !stop;
+            score = s1;
+        }
+        if (!live) break;
     }
     *length = best + 1;
-    *s_last_off = s_off + i;
+    *s_last_off = s_off + stop_k;
     return maxscore;
 }
 
 // NCBI s_BlastAaExtendTwoHit (reference restatement gpuBLAZE.cu:1116-1154).
-template <class SP>
+template <bool kPad, class SP>
 __device__ __forceinline__ int two_hit_walk(const Ctx &c, SP S, int slen, int s_left_off, int s_right_off,
                                             int q_right_off, int *hsp_q, int *hsp_s, int *hsp_len,
                                             int *right_extend, int *s_last_off) {
@@ -207,10 +225,10 @@ __device__ __forceinline__ int two_hit_walk(const Ctx &c, SP S, int slen, int s_
     right_d = 0;
     *right_extend = 0;
     *s_last_off = s_right_off;
-    left_score = walk_left(c, S, q_right_off - 1, s_right_off - 1, c.xdrop, &left_d);
+    left_score = walk_left<kPad>(c, S, q_right_off - 1, s_right_off - 1, c.xdrop, &left_d);
     if (left_d >= s_right_off - s_left_off) {
         *right_extend = 1;
-        right_score = walk_right(c, S, slen, q_right_off, s_right_off, c.xdrop, left_score, &right_d, s_last_off);
+        right_score = walk_right<kPad>(c, S, slen, q_right_off, s_right_off, c.xdrop, left_score, &right_d, s_last_off);
     }
     *hsp_q = q_right_off - left_d;
     *hsp_s = s_right_off - left_d;
@@ -243,34 +261,40 @@ __device__ __forceinline__ void apply_hit(const Ctx &c, WaveState<R> &w, int q, 
     w.ring[slot] = (R)(kPending | t);
 }
 
+// One claim pass: of the pending lanes whose diagonals hash to the same claim entry, the
+// earliest rank wins and applies its hit; returns true for the lanes that lost (their hit is
+// still to be applied).  The claim and the diagonal's state are read in the same pass: a
+// winner's state is current, since no lane writes a ring slot before the claims are settled.
+template <class R>
+__device__ __forceinline__ bool claim_pass(const Ctx &c, WaveState<R> &w, bool pend, int q, int s, uint32_t slot,
+                                           uint32_t rank) {
+    const uint32_t h = slot & c.claim_mask;
+    uint32_t st = 0;
+    if (pend) {
+        atomicMin(&w.claim[h], rank);
+        st = w.ring[slot];
+    }
+    ring_sync<R>();
+    const bool win = pend && w.claim[h] == rank;
+    ring_sync<R>();
+    if (win) {
+        w.claim[h] = 0xFFFFFFFFu;
+        apply_hit(c, w, q, s, slot, st);
+    }
+    ring_sync<R>();
+    return pend && !win;
+}
+
 // Applies up to one hit per lane; same-diagonal hits go earliest-rank first.
-// The claim and the diagonal's state are read in the same pass: a winner's state is current,
-// since no lane writes a ring slot before the claims are settled, and a loser reads again.
 template <class R>
 __device__ __forceinline__ void resolve_round(const Ctx &c, WaveState<R> &w, bool pend, int q, int s, uint32_t rank) {
     const uint32_t slot = ring_slot(c, q, s);
-    const uint32_t h = slot & c.claim_mask;
-    while (__ballot(pend)) {
-        uint32_t st = 0;
-        if (pend) {
-            atomicMin(&w.claim[h], rank);
-            st = w.ring[slot];
-        }
-        ring_sync<R>();
-        const bool win = pend && w.claim[h] == rank;
-        ring_sync<R>();
-        if (win) {
-            w.claim[h] = 0xFFFFFFFFu;
-            pend = false;
-            apply_hit(c, w, q, s, slot, st);
-        }
-        ring_sync<R>();
-    }
+    while (__ballot(pend)) pend = claim_pass(c, w, pend, q, s, slot, rank);
 }
 
 // Extends every queued pair (one lane each), writes the outcome to its diagonal, then replays
 // the hits queued behind pending diagonals; repeats until nothing is pending.
-template <class R, class SP>
+template <bool kPad, class R, class SP>
 __device__ void flush(const Ctx &c, WaveState<R> &w, SP S, int slen, unsigned long long &n_ext) {
     constexpr uint32_t kFlag = RingFmt<R>::kFlag;
     const int lane = lane_id();
@@ -287,7 +311,7 @@ __device__ void flush(const Ctx &c, WaveState<R> &w, SP S, int slen, unsigned lo
                 const uint32_t slot = tk.q_slot >> 16;
                 const int ts = (int)(tk.s_pack & 0xFFFFu);
                 int hq, hs, hl, rext, slast;
-                const int score = two_hit_walk(c, S, slen, (int)(tk.s_pack >> 16), ts, q, &hq, &hs, &hl, &rext, &slast);
+                const int score = two_hit_walk<kPad>(c, S, slen, (int)(tk.s_pack >> 16), ts, q, &hq, &hs, &hl, &rext, &slast);
                 n_ext++;
                 if (score >= c.cutoff) {
                     const uint32_t k = atomicAdd(&w.cnt[2], 1u);
@@ -340,21 +364,28 @@ __device__ __forceinline__ void word_range(const Tables &t, uint32_t wd, uint32_
 }
 
 // Seed scan of one subject whose letters are read through S (LDS buffer or global memory).
-template <int kTable, class R, class SP>
+template <int kTable, bool kPad, class R, class SP>
 __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP S, int slen, unsigned long long &n_hits,
                              unsigned long long &n_ext) {
     const int lane = lane_id();
     // Hits wait in a queue (w.hits[head, tail), entries q << 16 | subject offset, subject order)
     // and go through the state machine 64 at a time, so a round is full even when a step
-    // yields a few more than 64 hits.
+    // yields a few more than 64 hits.  A round makes one claim pass; the hits that lost their
+    // claim go back to the front of the queue, in order, and lead the next round.  Per diagonal
+    // the order stays the subject order: a lost hit is behind its diagonal's winner and ahead
+    // of every later hit.
     uint32_t head = 0, tail = 0;
     auto round = [&]() {
-        if (w.cnt[0] > (uint32_t)(kTaskCap - 64) || w.cnt[1] > 64u) flush(c, w, S, slen, n_ext);
+        if (w.cnt[0] > (uint32_t)(kTaskCap - 64) || w.cnt[1] > 64u) flush<kPad>(c, w, S, slen, n_ext);
         const uint32_t n = tail - head < 64u ? tail - head : 64u;
         const bool pend = (uint32_t)lane < n;
         const uint32_t e = pend ? w.hits[head + lane] : 0u;
-        resolve_round(c, w, pend, (int)(e >> 16), (int)(e & 0xFFFFu), (uint32_t)lane);
-        head += n;
+        const int q = (int)(e >> 16), s = (int)(e & 0xFFFFu);
+        const bool lost = claim_pass(c, w, pend, q, s, ring_slot(c, q, s), (uint32_t)lane);
+        const uint64_t lm = __ballot(lost);
+        head += n - (uint32_t)__popcll(lm);
+        if (lost) w.hits[head + __builtin_amdgcn_mbcnt_hi((uint32_t)(lm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)lm, 0u))] = e;
+        wave_sync();
     };
     for (int c0 = 0; c0 + kWord <= slen; c0 += 64) {
         const int s = c0 + lane;
@@ -394,7 +425,7 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
             w.p0[lane] = pa;
             wave_sync();
             for (uint32_t r = 0; r < H; r += 64) {
-                if (w.cnt[0] > (uint32_t)(kTaskCap - 64) || w.cnt[1] > 64u) flush(c, w, S, slen, n_ext);
+                if (w.cnt[0] > (uint32_t)(kTaskCap - 64) || w.cnt[1] > 64u) flush<kPad>(c, w, S, slen, n_ext);
                 const uint32_t g = r + lane;
                 const bool pend = g < H;
                 int src = 0, qo = 0;
@@ -415,7 +446,7 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
         wave_sync();
     }
     while (tail > head) round();
-    flush(c, w, S, slen, n_ext);
+    flush<kPad>(c, w, S, slen, n_ext);
 }
 
 // The subject's init hits (staged by this wave) go to the pool, one work item per query of
@@ -547,8 +578,8 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
     off += 16;
     int16_t(*mat)[28] = reinterpret_cast<int16_t(*)[28]>(smem + off);
     off += al16(28 * 28 * 2);
-    uint8_t *Qs = smem + off;
-    off += al16((uint32_t)a.qlen + 16);
+    uint8_t *Qs = smem + off + kSeqPad;  // kSeqPad zero letters on both sides
+    off += kSeqPad + al16((uint32_t)a.qlen + kSeqPad);
     uint16_t *ent_l = nullptr;
     uint16_t *rel = nullptr;
     uint32_t *bb = nullptr;
@@ -572,9 +603,10 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
     w.hits = w.cnt + 4;
     w.task = reinterpret_cast<Task *>(w.hits + kHitQueue);
     w.defer = reinterpret_cast<uint32_t *>(w.task + kTaskCap);
-    w.subj = reinterpret_cast<uint8_t *>(w.defer + kDeferCap);
+    w.subj = reinterpret_cast<uint8_t *>(w.defer + kDeferCap) + kSeqPad;  // kSeqPad zero letters on both sides
 
     for (int i = threadIdx.x; i < 28 * 28; i += blockDim.x) mat[i / 28][i % 28] = kBlosum62Dev[i / 28][i % 28];
+    if (lane < kSeqPad) w.subj[lane - kSeqPad] = 0;
     for (uint32_t i = lane; i < C; i += 64) w.claim[i] = 0xFFFFFFFFu;
     if (lane < 4) w.cnt[lane] = 0;
 
@@ -616,7 +648,8 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
         const int64_t sub1 = a.slice[j + 1];
         const QueryDev &qd = a.qd[qi];
         if (qi != cur_q) {  // stage this query
-            for (int i = threadIdx.x; i < qd.qlen; i += blockDim.x) Qs[i] = qd.query[i];
+            for (int i = (int)threadIdx.x - kSeqPad; i < qd.qlen + kSeqPad; i += blockDim.x)
+                Qs[i] = i >= 0 && i < qd.qlen ? qd.query[i] : 0;
             if (kTable >= 1) {
                 const uint32_t *src = reinterpret_cast<const uint32_t *>(qd.entries);
                 uint32_t *dst = reinterpret_cast<uint32_t *>(ent_l);
@@ -655,7 +688,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
             if constexpr (kLong) {
                 for (uint32_t i = lane; i < Mq; i += 64) wl.ring[i] = 0;
                 ring_sync<uint32_t>();
-                scan_subject<kTable>(c, wl, t, Sg, slen, n_hits, n_ext);
+                scan_subject<kTable, false>(c, wl, t, Sg, slen, n_hits, n_ext);
             } else {
                 if (slen > kNarrowMax) continue;  // the long-subject launch takes it
                 uint4 *r4 = reinterpret_cast<uint4 *>(w.ring);
@@ -675,11 +708,12 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
                             if (p >= 0 && p < slen) w.subj[p] = (uint8_t)(xw[k >> 2] >> (8 * (k & 3)));
                         }
                     }
+                    if (lane < kSeqPad) w.subj[slen + lane] = 0;
                     wave_sync();
-                    scan_subject<kTable>(c, w, t, (const uint8_t *)w.subj, slen, n_hits, n_ext);
+                    scan_subject<kTable, true>(c, w, t, (const uint8_t *)w.subj, slen, n_hits, n_ext);
                 } else {
                     wave_sync();
-                    scan_subject<kTable>(c, w, t, Sg, slen, n_hits, n_ext);
+                    scan_subject<kTable, false>(c, w, t, Sg, slen, n_hits, n_ext);
                 }
             }
 
@@ -783,7 +817,7 @@ __global__ void __launch_bounds__(256) k_seed_retry(SeedArgs a) {
         const int slen = (int)(a.offs[sid + 1] - o0);
         for (uint32_t i = lane; i < qd.ring_len; i += 64) w.ring[i] = 0;
         ring_sync<uint32_t>();
-        scan_subject<0>(c, w, t, a.res + o0, slen, n_hits, n_ext);
+        scan_subject<0, false>(c, w, t, a.res + o0, slen, n_hits, n_ext);
         wave_sync();
         const uint32_t nih = w.cnt[2];
         wave_sync();
