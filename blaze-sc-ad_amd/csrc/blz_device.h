@@ -13,6 +13,10 @@ struct QueryDev {
     const uint16_t *entries;   // query offsets per word, ascending
     const uint16_t *rel16;     // compact prefix: offset of word w inside its 64-word block
     const uint32_t *blockbase; // compact prefix: first entry of each 64-word block
+    // word records, one 16-byte load per word: x = entry count (255: at least 255, or first
+    // entry past 2^24; read the prefix) | first entry << 8; y, z, w = the first kWordRecInline
+    // entries (u16 pairs, low half first)
+    const uint4 *rec;
     uint32_t n_entries;
     int32_t qlen;
     uint32_t ring_mask, claim_mask;
@@ -30,6 +34,7 @@ struct QueryDev {
     int32_t n_members;
 };
 constexpr int kMaxMembers = 32;
+constexpr int kWordRecInline = 6;
 
 // Longest subject the seed kernel's LDS diagonal ring (16-bit entries) takes; longer subjects
 // use SeedArgs::long_ring.

@@ -41,10 +41,11 @@ struct DevTables {
     uint64_t n_entries = 0;
     uint16_t *d_rel16 = nullptr;
     uint32_t *d_blockbase = nullptr;
+    uint4 *d_rec = nullptr;  // word records (kWordRecInline entries inline)
     bool compact_ok = false;
     hipStream_t st = nullptr;  // stream-ordered allocations: freed without a device-wide sync
     void release() {
-        void *bufs[] = {d_query, d_prefix, d_entries, d_rel16, d_blockbase};
+        void *bufs[] = {d_query, d_prefix, d_entries, d_rel16, d_blockbase, d_rec};
         for (void *b : bufs)
             if (b) (void)(st ? hipFreeAsync(b, st) : hipFree(b));
         *this = DevTables{};
@@ -292,6 +293,18 @@ blz_status build_tables(blz_ctx *ctx, const uint8_t *letters, int len, const std
         if (r > 0xFFFFu) compact_ok = false;
         rel16[u] = (uint16_t)r;
     }
+    // word records: count (saturating at 255: then read the prefix) | first entry << 8 in x,
+    // the first kWordRecInline entries in y, z, w
+    std::vector<uint4> rec(kWords);
+    const bool rec_start_fits = prefix[kWords] < (1u << 24);
+    for (int u = 0; u < kWords; u++) {
+        const uint32_t a = prefix[u], n = prefix[u + 1] - a;
+        uint16_t e[kWordRecInline] = {};
+        for (uint32_t k = 0; k < n && k < (uint32_t)kWordRecInline; k++) e[k] = entries[a + k];
+        const uint32_t cb = n < 255 && rec_start_fits ? n : 255u;
+        rec[u] = uint4{cb | (rec_start_fits ? a << 8 : 0u), e[0] | (uint32_t)e[1] << 16, e[2] | (uint32_t)e[3] << 16,
+                       e[4] | (uint32_t)e[5] << 16};
+    }
     t.len = len;
     t.n_entries = prefix[kWords];
     t.compact_ok = compact_ok;
@@ -305,7 +318,8 @@ blz_status build_tables(blz_ctx *ctx, const uint8_t *letters, int len, const std
         !check(ctx, alloc((void **)&t.d_prefix, sizeof(uint32_t) * (kWords + 1)), "prefix") ||
         !check(ctx, alloc((void **)&t.d_entries, sizeof(uint16_t) * entries.size()), "entries") ||
         !check(ctx, alloc((void **)&t.d_rel16, sizeof(uint16_t) * (kWords + 1)), "rel16") ||
-        !check(ctx, alloc((void **)&t.d_blockbase, sizeof(uint32_t) * kRelBlocks), "blockbase")) {
+        !check(ctx, alloc((void **)&t.d_blockbase, sizeof(uint32_t) * kRelBlocks), "blockbase") ||
+        !check(ctx, alloc((void **)&t.d_rec, sizeof(uint4) * kWords), "word records")) {
         t.release();
         return BLZ_ENOMEM;
     }
@@ -320,6 +334,8 @@ blz_status build_tables(blz_ctx *ctx, const uint8_t *letters, int len, const std
                                    st), "rel16 copy") ||
         !check(ctx, hipMemcpyAsync(t.d_blockbase, blockbase.data(), sizeof(uint32_t) * kRelBlocks,
                                    hipMemcpyHostToDevice, st), "blockbase copy") ||
+        !check(ctx, hipMemcpyAsync(t.d_rec, rec.data(), sizeof(uint4) * kWords, hipMemcpyHostToDevice, st),
+               "word records copy") ||
         !check(ctx, hipStreamSynchronize(st), "table upload")) {
         t.release();
         return BLZ_EDEVICE;
@@ -609,6 +625,7 @@ QueryDev query_dev(const blz_query *q) {
     d.query = q->t.d_query;
     d.prefix = q->t.d_prefix;
     d.entries = q->t.d_entries;
+    d.rec = q->t.d_rec;
     d.rel16 = q->t.d_rel16;
     d.blockbase = q->t.d_blockbase;
     d.n_entries = (uint32_t)q->t.n_entries;
@@ -895,6 +912,7 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
             d.query = t.d_query;
             d.prefix = t.d_prefix;
             d.entries = t.d_entries;
+            d.rec = t.d_rec;
             d.rel16 = t.d_rel16;
             d.blockbase = t.d_blockbase;
             d.n_entries = (uint32_t)t.n_entries;

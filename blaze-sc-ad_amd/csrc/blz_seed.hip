@@ -71,6 +71,7 @@ constexpr int kHitQueue = 192;              // queued hits: fewer than 64 waitin
 constexpr int kSubjBuf = 1024;              // subject letters staged in LDS per wave
 constexpr int kSeqPad = 16;                 // zero letters before and after a sequence staged in LDS
 static_assert(kSubjBuf <= kNarrowMax, "staged subjects use the LDS ring");
+static_assert(kWordRecInline == 6, "scan_subject writes six inline entries");
 
 struct Task {
     uint32_t q_slot;  // query offset | diagonal slot << 16
@@ -343,13 +344,30 @@ __device__ void flush(const Ctx &c, WaveState<R> &w, SP S, int slen, unsigned lo
     }
 }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Global-memory view of a table pointer: loads become global loads (no flat address-space test,
+// and LDS waits do not wait on them).
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gmem(const T *p) {
+    return (const __attribute__((address_space(1))) T *)p;
+}
+
 // Lookup tables of the current query, wherever kTable puts them.
 struct Tables {
     const uint16_t *rel;  // LDS (kTable 2)
     const uint32_t *bb;   // LDS (kTable 2)
     const uint32_t *gprefix;
     const uint16_t *ent;  // LDS (kTable >= 1) or global
+    const uint4 *rec;     // word records in global memory (kTable 0)
 };
+
+// Lookup entry i: a global load for kTable 0, an LDS load otherwise.
+template <int kTable>
+__device__ __forceinline__ uint32_t entry(const Tables &t, uint32_t i) {
+    if constexpr (kTable == 0) return gmem(t.ent)[i];
+    return t.ent[i];
+}
 
 // Lookup entries [pa, pb) of word wd.
 template <int kTable>
@@ -389,9 +407,24 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
     };
     for (int c0 = 0; c0 + kWord <= slen; c0 += 64) {
         const int s = c0 + lane;
-        uint32_t pa = 0, pb = 0;
-        if (s + kWord <= slen) word_range<kTable>(t, S[s] * 784u + S[s + 1] * 28u + S[s + 2], pa, pb);
-        const uint32_t nh = pb - pa;
+        uint32_t pa = 0, nh = 0;
+        u32x4 r = {0u, 0u, 0u, 0u};  // kTable 0: the word's record
+        if (s + kWord <= slen) {
+            const uint32_t wd = S[s] * 784u + S[s + 1] * 28u + S[s + 2];
+            if constexpr (kTable == 0) {
+                r = gmem(reinterpret_cast<const u32x4 *>(t.rec))[wd];
+                nh = r.x & 0xFFu;
+                pa = r.x >> 8;
+                if (nh == 0xFFu) {  // 255 entries or more, or a first entry past 2^24
+                    pa = gmem(t.gprefix)[wd];
+                    nh = gmem(t.gprefix)[wd + 1] - pa;
+                }
+            } else {
+                uint32_t pb;
+                word_range<kTable>(t, wd, pa, pb);
+                nh = pb - pa;
+            }
+        }
         const uint32_t inc = wave_inclusive_sum(nh);
         const uint32_t H = last_lane(inc);
         const uint32_t ex = inc - nh;
@@ -405,12 +438,24 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
                 head = 0;
                 tail = rest;
             }
-            // append: four independent lookup loads per pass (indices clamped to the word's
-            // last entry)
-            for (uint32_t k = 0; k < nh; k += 4) {
+            // append: the record's inline entries, then four independent lookup loads per pass
+            // (indices clamped to the word's last entry)
+            uint32_t k0 = 0;
+            if constexpr (kTable == 0) {
+                const uint32_t b = tail + ex;
+                const uint32_t su = (uint32_t)s;
+                if (nh > 0) w.hits[b + 0] = (r.y << 16) | su;
+                if (nh > 1) w.hits[b + 1] = (r.y & 0xFFFF0000u) | su;
+                if (nh > 2) w.hits[b + 2] = (r.z << 16) | su;
+                if (nh > 3) w.hits[b + 3] = (r.z & 0xFFFF0000u) | su;
+                if (nh > 4) w.hits[b + 4] = (r.w << 16) | su;
+                if (nh > 5) w.hits[b + 5] = (r.w & 0xFFFF0000u) | su;
+                k0 = kWordRecInline;
+            }
+            for (uint32_t k = k0; k < nh; k += 4) {
                 uint32_t e[4];
 #pragma unroll
-                for (int j = 0; j < 4; j++) e[j] = t.ent[pa + min(k + j, nh - 1)];
+                for (int j = 0; j < 4; j++) e[j] = entry<kTable>(t, pa + min(k + j, nh - 1));
 #pragma unroll
                 for (int j = 0; j < 4; j++)
                     if (k + j < nh) w.hits[tail + ex + k + j] = (e[j] << 16) | (uint32_t)s;
@@ -437,7 +482,7 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
                         else hi = mid - 1;
                     }
                     src = lo;
-                    qo = t.ent[w.p0[src] + (g - w.excl[src])];
+                    qo = entry<kTable>(t, w.p0[src] + (g - w.excl[src]));
                 }
                 resolve_round(c, w, pend, qo, c0 + src, (uint32_t)lane);
             }
@@ -674,6 +719,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
         t.bb = bb;
         t.gprefix = qd.prefix;
         t.ent = kTable >= 1 ? ent_l : qd.entries;
+        t.rec = qd.rec;
 
         for (;;) {
             int64_t k = 0;
@@ -813,6 +859,7 @@ __global__ void __launch_bounds__(256) k_seed_retry(SeedArgs a) {
         Tables t{};
         t.gprefix = qd.prefix;
         t.ent = qd.entries;
+        t.rec = qd.rec;
         const uint64_t o0 = a.offs[sid];
         const int slen = (int)(a.offs[sid + 1] - o0);
         for (uint32_t i = lane; i < qd.ring_len; i += 64) w.ring[i] = 0;
