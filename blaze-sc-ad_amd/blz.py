@@ -56,7 +56,8 @@ class Tuning(C.Structure):
     _fields_ = [("seed_variant", C.c_int32), ("seed_waves", C.c_int32), ("seed_blocks_per_cu", C.c_int32),
                 ("gapped_blocks", C.c_int32), ("gapped_threads", C.c_int32), ("seed_table", C.c_int32),
                 ("gapped_wave_classes", C.c_int32), ("query_packing", C.c_int32), ("pack_len", C.c_int32),
-                ("bin_ring", C.c_int32), ("graph", C.c_int32), ("gapped_lanes", C.c_int32)]
+                ("bin_ring", C.c_int32), ("graph", C.c_int32), ("gapped_lanes", C.c_int32),
+                ("subject_groups", C.c_int32)]
 
 
 class ServerConfig(C.Structure):

@@ -32,6 +32,9 @@ struct QueryDev {
     const uint32_t *member_start;
     const uint32_t *member_idx;
     int32_t n_members;
+    // subject groups (seed stage): virtual distance between the subjects of a group, 0 when a
+    // wave scans one subject at a time
+    int32_t group_gap;
 };
 constexpr int kMaxMembers = 32;
 constexpr int kWordRecInline = 6;

@@ -78,6 +78,20 @@ struct Task {
     uint32_t s_pack;  // subject offset of the second hit | end of the first hit's word << 16
 };
 
+// Short subjects scanned together by one wave (a group).  Their letters sit in the wave's LDS
+// buffer at 64-aligned starts with at least 4 zero letters between them, so a step of 64 words
+// never spans two subjects and a walk stops at a subject's end.  The two-hit state sees virtual
+// positions: subject i's letters shifted by i * gap, gap >= window, so no pair, flag or diagonal
+// state carries from one subject to the next, and one diagonal ring, task queue and flush serve
+// the whole group (fuller extension batches, one ring clear per group).
+constexpr int kGroupMax = 8;
+struct Group {
+    int64_t sid[kGroupMax];
+    uint32_t start[kGroupMax];   // first letter in the buffer
+    uint32_t len[kGroupMax];
+    uint32_t vstart[kGroupMax];  // start + i * gap
+};
+
 template <class R>
 struct WaveState {
     R *ring;
@@ -89,13 +103,14 @@ struct WaveState {
     Task *task;
     uint32_t *defer;  // query offset | subject offset << 16
     uint8_t *subj;
+    Group *grp;
 };
 
 __host__ __device__ constexpr uint32_t al16(uint32_t b) { return (b + 15u) & ~15u; }
 
 __host__ __device__ inline uint32_t wave_lds_bytes(uint32_t M, uint32_t C) {
     return al16(2 * M + 4 * (C + 68 + 64 + 4 + kHitQueue) + sizeof(Task) * kTaskCap + sizeof(uint32_t) * kDeferCap +
-                kSubjBuf + 2 * kSeqPad);
+                kSubjBuf + 2 * kSeqPad + sizeof(Group));
 }
 
 __host__ __device__ inline uint32_t block_lds_bytes(int table, uint32_t qlen, uint32_t n_entries) {
@@ -114,6 +129,7 @@ struct Ctx {
     int window;
     int xdrop;
     int cutoff;
+    int gap;  // subject groups: virtual distance between subjects (0: no groups)
     InitHit *stage;
     int stage_cap;
 };
@@ -295,8 +311,11 @@ __device__ __forceinline__ void resolve_round(const Ctx &c, WaveState<R> &w, boo
 
 // Extends every queued pair (one lane each), writes the outcome to its diagonal, then replays
 // the hits queued behind pending diagonals; repeats until nothing is pending.
-template <bool kPad, class R, class SP>
-__device__ void flush(const Ctx &c, WaveState<R> &w, SP S, int slen, unsigned long long &n_ext) {
+// kGrouped: the tasks belong to the subjects of w.grp (ng of them); a task's subject follows from
+// its virtual position, its letters from the subject's place in the buffer.  Staged init hits
+// carry their subject's index in bits 24.. of len and subject-relative offsets.
+template <bool kPad, bool kGrouped, class R, class SP>
+__device__ void flush(const Ctx &c, WaveState<R> &w, SP S, int slen, int ng, uint32_t &n_ext) {
     constexpr uint32_t kFlag = RingFmt<R>::kFlag;
     const int lane = lane_id();
     for (;;) {
@@ -311,14 +330,21 @@ __device__ void flush(const Ctx &c, WaveState<R> &w, SP S, int slen, unsigned lo
                 const int q = (int)(tk.q_slot & 0xFFFFu);
                 const uint32_t slot = tk.q_slot >> 16;
                 const int ts = (int)(tk.s_pack & 0xFFFFu);
+                int sub = 0, shift = 0, rel = 0;  // subject in the group, virtual shift, first letter
+                if constexpr (kGrouped) {
+                    for (int j = 1; j < ng; j++) sub += ts >= (int)w.grp->vstart[j];
+                    shift = sub * c.gap;
+                    rel = (int)w.grp->start[sub];
+                }
                 int hq, hs, hl, rext, slast;
-                const int score = two_hit_walk<kPad>(c, S, slen, (int)(tk.s_pack >> 16), ts, q, &hq, &hs, &hl, &rext, &slast);
+                const int score = two_hit_walk<kPad>(c, S, slen, (int)(tk.s_pack >> 16) - shift, ts - shift, q, &hq, &hs,
+                                                     &hl, &rext, &slast);
                 n_ext++;
                 if (score >= c.cutoff) {
                     const uint32_t k = atomicAdd(&w.cnt[2], 1u);
-                    if (k < (uint32_t)c.stage_cap) c.stage[k] = InitHit{hq, hs, hl, score};
+                    if (k < (uint32_t)c.stage_cap) c.stage[k] = InitHit{hq, hs - rel, hl | (sub << 24), score};
                 }
-                w.ring[slot] = (R)(rext ? (((uint32_t)slast - 2u + 1u) | kFlag) : ((uint32_t)ts + 1u));
+                w.ring[slot] = (R)(rext ? (((uint32_t)(slast + shift) - 2u + 1u) | kFlag) : ((uint32_t)ts + 1u));
             }
         }
         ring_sync<R>();
@@ -382,35 +408,105 @@ __device__ __forceinline__ void word_range(const Tables &t, uint32_t wd, uint32_
 }
 
 // Seed scan of one subject whose letters are read through S (LDS buffer or global memory).
-template <int kTable, bool kPad, class R, class SP>
-__device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP S, int slen, unsigned long long &n_hits,
-                             unsigned long long &n_ext) {
+// Seed scan of one subject (slen letters read through S), or with kGrouped of the ng subjects of
+// w.grp in the wave's LDS buffer S.
+//
+// Hits wait in a queue (w.hits[head, tail), entries q << 16 | position, subject order) and go
+// through the two-hit state 64 at a time, so a round is full even when a step yields a few more
+// than 64 hits.  A round makes one claim pass; the hits that lost their claim go back to the
+// front of the queue, in order, and lead the next round: per diagonal the order stays the
+// subject order, since a lost hit is behind its diagonal's winner and ahead of every later hit.
+// A step whose hits exceed kHitBuf (low-complexity letters) is expanded into the queue 64 hits
+// at a time, each hit finding its position by binary search.
+// The loop is one state machine with a single site for each of its phases (claim round with its
+// extension flush, dense expansion, word step), which keeps the inlined code and its registers
+// small.
+template <int kTable, bool kPad, bool kGrouped = false, class R, class SP>
+__device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP S, int slen, uint32_t &n_hits,
+                             uint32_t &n_ext, int ng = 1) {
     const int lane = lane_id();
-    // Hits wait in a queue (w.hits[head, tail), entries q << 16 | subject offset, subject order)
-    // and go through the state machine 64 at a time, so a round is full even when a step
-    // yields a few more than 64 hits.  A round makes one claim pass; the hits that lost their
-    // claim go back to the front of the queue, in order, and lead the next round.  Per diagonal
-    // the order stays the subject order: a lost hit is behind its diagonal's winner and ahead
-    // of every later hit.
     uint32_t head = 0, tail = 0;
-    auto round = [&]() {
-        if (w.cnt[0] > (uint32_t)(kTaskCap - 64) || w.cnt[1] > 64u) flush<kPad>(c, w, S, slen, n_ext);
-        const uint32_t n = tail - head < 64u ? tail - head : 64u;
-        const bool pend = (uint32_t)lane < n;
-        const uint32_t e = pend ? w.hits[head + lane] : 0u;
-        const int q = (int)(e >> 16), s = (int)(e & 0xFFFFu);
-        const bool lost = claim_pass(c, w, pend, q, s, ring_slot(c, q, s), (uint32_t)lane);
-        const uint64_t lm = __ballot(lost);
-        head += n - (uint32_t)__popcll(lm);
-        if (lost) w.hits[head + __builtin_amdgcn_mbcnt_hi((uint32_t)(lm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)lm, 0u))] = e;
-        wave_sync();
-    };
-    for (int c0 = 0; c0 + kWord <= slen; c0 += 64) {
-        const int s = c0 + lane;
+    int gi = 0;                           // subject of the group
+    int p_begin = 0, p_end = slen, voff = 0;  // its letters [p_begin, p_end), virtual shift
+    if constexpr (kGrouped) {
+        p_begin = (int)w.grp->start[0];
+        p_end = p_begin + (int)w.grp->len[0];
+    }
+    int c0 = p_begin;                     // next step
+    uint32_t dense_h = 0, dense_r = 0;    // dense step in progress: its hits, those expanded
+    int dense_c0 = 0, dense_voff = 0;
+    bool input = true;                    // steps (or dense hits) remain
+    for (;;) {
+        // a claim round over the queue's first 64 hits, after the extensions when the task or
+        // deferred lists fill
+        if (tail - head >= 64u || (!input && tail > head)) {
+            if (w.cnt[0] > (uint32_t)(kTaskCap - 64) || w.cnt[1] > 64u) flush<kPad, kGrouped>(c, w, S, slen, ng, n_ext);
+            const uint32_t n = tail - head < 64u ? tail - head : 64u;
+            const bool pend = (uint32_t)lane < n;
+            const uint32_t e = pend ? w.hits[head + lane] : 0u;
+            const int q = (int)(e >> 16), sp = (int)(e & 0xFFFFu);
+            const bool lost = claim_pass(c, w, pend, q, sp, ring_slot(c, q, sp), (uint32_t)lane);
+            const uint64_t lm = __ballot(lost);
+            head += n - (uint32_t)__popcll(lm);
+            if (lost)
+                w.hits[head + __builtin_amdgcn_mbcnt_hi((uint32_t)(lm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)lm, 0u))] = e;
+            wave_sync();
+            continue;
+        }
+        if (!input) break;
+        // fewer than 64 hits wait; room for `need` more: move the waiting hits to the front
+        auto make_room = [&](uint32_t need) {
+            if (tail + need > (uint32_t)kHitQueue) {
+                const uint32_t rest = tail - head;
+                const uint32_t v = (uint32_t)lane < rest ? w.hits[head + lane] : 0u;
+                wave_sync();
+                if ((uint32_t)lane < rest) w.hits[lane] = v;
+                wave_sync();
+                head = 0;
+                tail = rest;
+            }
+        };
+        // 64 more hits of a dense step
+        if (dense_r < dense_h) {
+            make_room(64u);
+            const uint32_t g = dense_r + lane;
+            if (g < dense_h) {
+                int lo = 0, hi = 63;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (w.excl[mid] <= g) lo = mid;
+                    else hi = mid - 1;
+                }
+                const uint32_t qo = entry<kTable>(t, w.p0[lo] + (g - w.excl[lo]));
+                w.hits[tail + lane] = (qo << 16) | (uint32_t)(dense_c0 + lo + dense_voff);
+            }
+            const uint32_t n = dense_h - dense_r < 64u ? dense_h - dense_r : 64u;
+            wave_sync();
+            tail += n;
+            dense_r += n;
+            continue;
+        }
+        // the next step of 64 words
+        if (c0 + kWord > p_end) {  // this subject is done: the next one of the group, or the end
+            if constexpr (kGrouped) {
+                if (gi + 1 < ng) {
+                    gi++;
+                    p_begin = (int)w.grp->start[gi];
+                    p_end = p_begin + (int)w.grp->len[gi];
+                    voff = gi * c.gap;
+                    c0 = p_begin;
+                    continue;
+                }
+            }
+            input = false;
+            continue;
+        }
+        const int p = c0 + lane;
+        const int s = p + voff;  // position seen by the two-hit state
         uint32_t pa = 0, nh = 0;
         u32x4 r = {0u, 0u, 0u, 0u};  // kTable 0: the word's record
-        if (s + kWord <= slen) {
-            const uint32_t wd = S[s] * 784u + S[s + 1] * 28u + S[s + 2];
+        if (p + kWord <= p_end) {
+            const uint32_t wd = S[p] * 784u + S[p + 1] * 28u + S[p + 2];
             if constexpr (kTable == 0) {
                 r = gmem(reinterpret_cast<const u32x4 *>(t.rec))[wd];
                 nh = r.x & 0xFFu;
@@ -430,14 +526,7 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
         const uint32_t ex = inc - nh;
         n_hits += nh;
         if (H <= (uint32_t)kHitBuf) {
-            if (tail + H > (uint32_t)kHitQueue) {  // move the (< 64) waiting hits to the front
-                const uint32_t rest = tail - head;
-                const uint32_t v = (uint32_t)lane < rest ? w.hits[head + lane] : 0u;
-                wave_sync();
-                if ((uint32_t)lane < rest) w.hits[lane] = v;
-                head = 0;
-                tail = rest;
-            }
+            make_room(H);
             // append: the record's inline entries, then four independent lookup loads per pass
             // (indices clamped to the word's last entry)
             uint32_t k0 = 0;
@@ -462,36 +551,19 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
             }
             wave_sync();
             tail += H;
-            while (tail - head >= 64u) round();
         } else {
-            while (tail > head) round();  // earlier hits first
-            // dense step (low-complexity letters): find each hit's position by binary search
+            // dense step: its hits enter the queue 64 at a time (above)
             w.excl[lane] = ex;
             w.p0[lane] = pa;
             wave_sync();
-            for (uint32_t r = 0; r < H; r += 64) {
-                if (w.cnt[0] > (uint32_t)(kTaskCap - 64) || w.cnt[1] > 64u) flush<kPad>(c, w, S, slen, n_ext);
-                const uint32_t g = r + lane;
-                const bool pend = g < H;
-                int src = 0, qo = 0;
-                if (pend) {
-                    int lo = 0, hi = 63;
-                    while (lo < hi) {
-                        const int mid = (lo + hi + 1) >> 1;
-                        if (w.excl[mid] <= g) lo = mid;
-                        else hi = mid - 1;
-                    }
-                    src = lo;
-                    qo = entry<kTable>(t, w.p0[src] + (g - w.excl[src]));
-                }
-                resolve_round(c, w, pend, qo, c0 + src, (uint32_t)lane);
-            }
-            head = tail = 0;
+            dense_h = H;
+            dense_r = 0;
+            dense_c0 = c0;
+            dense_voff = voff;
         }
-        wave_sync();
+        c0 += 64;
     }
-    while (tail > head) round();
-    flush<kPad>(c, w, S, slen, n_ext);
+    flush<kPad, kGrouped>(c, w, S, slen, ng, n_ext);
 }
 
 // The subject's init hits (staged by this wave) go to the pool, one work item per query of
@@ -601,6 +673,74 @@ __device__ void emit_init_hits(const SeedArgs &a, const QueryDev &qd, WaveState<
     }
 }
 
+// A group's init hits (staged by this wave, tagged with their subject) go to the pool, one work
+// item per subject that has any.  Single-query items only.
+template <class R>
+__device__ void emit_group(const SeedArgs &a, const QueryDev &qd, WaveState<R> &w, const Ctx &c, int ng, uint32_t nih) {
+    const int lane = lane_id();
+    const int qi = (int)qd.member_idx[0];
+    auto overflow_all = [&]() {
+        if (lane < ng) {
+            const uint32_t k = atomicAdd(&a.ctr->overflow_count, 1u);
+            if (k < a.ovf_cap) a.ovf[k] = ((int64_t)qi << kOvfShift) | w.grp->sid[lane];
+        }
+    };
+    if (nih > (uint32_t)a.stage_cap) {
+        overflow_all();
+        return;
+    }
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(&a.ctr->pool_used, (unsigned long long)nih);
+    base = first_lane64(base);
+    if (base + nih > a.pool_cap) {
+        overflow_all();
+        return;
+    }
+    uint32_t *cnt = w.p0;                 // [kGroupMax] counts, then cursors
+    uint32_t *bestm = w.excl;             // [kGroupMax] best score per subject
+    uint32_t *offm = w.excl + kGroupMax;  // [kGroupMax] first slot per subject
+    if (lane < ng) {
+        cnt[lane] = 0;
+        bestm[lane] = 0;
+    }
+    wave_sync();
+    for (uint32_t k = lane; k < nih; k += 64) {
+        const InitHit h = c.stage[k];
+        const int g = h.len >> 24;
+        atomicAdd(&cnt[g], 1u);
+        atomicMax(&bestm[g], (uint32_t)h.score);
+    }
+    wave_sync();
+    const uint32_t mine = lane < ng ? cnt[lane] : 0u;
+    const uint32_t off = wave_inclusive_sum(mine) - mine;
+    wave_sync();
+    if (lane < ng) {
+        offm[lane] = off;
+        cnt[lane] = 0;
+    }
+    wave_sync();
+    for (uint32_t k = lane; k < nih; k += 64) {
+        InitHit h = c.stage[k];
+        const int g = h.len >> 24;
+        h.len &= 0xFFFFFF;
+        a.pool[base + offm[g] + atomicAdd(&cnt[g], 1u)] = h;
+    }
+    wave_sync();
+    if (lane < ng && mine > 0) {
+        const int64_t sid = w.grp->sid[lane];
+        const uint32_t wi = atomicAdd(&a.ctr->work_count, 1u);
+        if (wi < a.work_cap) {
+            const int bucket = cost_bucket((int)bestm[lane]);
+            a.work[wi] = WorkItem{sid, (uint64_t)(base + off), (int32_t)mine, qi, bucket, 0};
+            atomicAdd(&a.ctr->bucket_count[bucket], 1u);
+            atomicAdd(&a.ctr->init_hits, (unsigned long long)mine);
+        } else {
+            const uint32_t k = atomicAdd(&a.ctr->overflow_count, 1u);
+            if (k < a.ovf_cap) a.ovf[k] = ((int64_t)qi << kOvfShift) | sid;
+        }
+    }
+}
+
 }  // namespace
 
 // kLong: the launch over the subjects longer than kNarrowMax (a.long_ids), with diagonal rings
@@ -649,6 +789,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
     w.task = reinterpret_cast<Task *>(w.hits + kHitQueue);
     w.defer = reinterpret_cast<uint32_t *>(w.task + kTaskCap);
     w.subj = reinterpret_cast<uint8_t *>(w.defer + kDeferCap) + kSeqPad;  // kSeqPad zero letters on both sides
+    w.grp = reinterpret_cast<Group *>(w.subj + kSubjBuf + kSeqPad);
 
     for (int i = threadIdx.x; i < 28 * 28; i += blockDim.x) mat[i / 28][i % 28] = kBlosum62Dev[i / 28][i % 28];
     if (lane < kSeqPad) w.subj[lane - kSeqPad] = 0;
@@ -666,12 +807,13 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
     wl.task = w.task;
     wl.defer = w.defer;
     wl.subj = w.subj;
+    wl.grp = nullptr;
     Ctx c;
     c.mat = mat;
     c.Q = Qs;
     c.stage = a.stage + gwave * (uint64_t)a.stage_cap;
     c.stage_cap = a.stage_cap;
-    unsigned long long n_hits = 0, n_ext = 0;
+    uint32_t n_hits = 0, n_ext = 0;  // this lane's word hits and extensions
     int cur_q = -1;
     const uint32_t n_items = (uint32_t)a.nq * a.n_slices;
     Tables t{};
@@ -683,7 +825,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
             blk[1] = 0;
         }
         __syncthreads();
-        const uint32_t item = (uint32_t)blk[0];
+        const uint32_t item = first_lane((uint32_t)blk[0]);  // uniform: the item's values live in scalars
         if (item >= n_items) break;
         const int qi = (int)(item / a.n_slices);
         // a length-sorted database ends with its longest subjects: take those first, so the
@@ -715,12 +857,39 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
         c.window = qd.window;
         c.xdrop = qd.x_ungapped;
         c.cutoff = qd.ungapped_cutoff;
+        c.gap = kLong ? 0 : qd.group_gap;
         t.rel = rel;
         t.bb = bb;
-        t.gprefix = qd.prefix;
-        t.ent = kTable >= 1 ? ent_l : qd.entries;
-        t.rec = qd.rec;
+        // table pointers in scalar registers: uniform, and not worth a vector register each
+        t.gprefix = reinterpret_cast<const uint32_t *>(first_lane64(reinterpret_cast<uint64_t>(qd.prefix)));
+        t.ent = kTable >= 1 ? ent_l
+                            : reinterpret_cast<const uint16_t *>(first_lane64(reinterpret_cast<uint64_t>(qd.entries)));
+        t.rec = reinterpret_cast<const uint4 *>(first_lane64(reinterpret_cast<uint64_t>(qd.rec)));
 
+        // subjects of up to kSubjBuf letters join the wave's group while they fit its buffer; a
+        // full group is scanned as one
+        int ng = 0;
+        uint32_t used = 0;
+        auto run_group = [&]() {
+            uint4 *r4 = reinterpret_cast<uint4 *>(w.ring);
+            for (uint32_t i = lane; i < c.ring_len / 8; i += 64) r4[i] = uint4{0, 0, 0, 0};
+            wave_sync();
+            scan_subject<kTable, true, true>(c, w, t, (const uint8_t *)w.subj, 0, n_hits, n_ext, ng);
+            wave_sync();
+            const uint32_t nih = w.cnt[2];
+            wave_sync();
+            if (nih > 0) {
+                if (ng == 1)
+                    emit_init_hits(a, qd, w, c, w.grp->sid[0], nih);
+                else
+                    emit_group(a, qd, w, c, ng, nih);
+                wave_sync();
+                if (lane == 0) w.cnt[2] = 0;
+                wave_sync();
+            }
+            ng = 0;
+            used = 0;
+        };
         for (;;) {
             int64_t k = 0;
             if (lane == 0) k = sub0 + (int64_t)atomicAdd(reinterpret_cast<uint32_t *>(&blk[1]), 1u);
@@ -731,6 +900,39 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
             const int slen = (int)(a.offs[sid + 1] - o0);
             const uint8_t *Sg = a.res + o0;
             const uint32_t Mq = c.ring_len;
+            if (!kLong && slen <= kSubjBuf) {  // staged in LDS, alone when groups are off (gap 0)
+                if (ng == (c.gap > 0 ? kGroupMax : 1) || used + (uint32_t)slen > (uint32_t)kSubjBuf) run_group();
+                // letters at [used, used + slen) by 16-byte loads, then zeros up to the next
+                // 64-aligned start (at least 4) or the buffer's end pad
+                const uint64_t a0 = o0 & ~uint64_t(15);
+                const int lead = (int)(o0 - a0);
+                const int nvec = (lead + slen + 15) >> 4;
+                const uint4 *src = reinterpret_cast<const uint4 *>(a.res + a0);
+                uint8_t *dst = w.subj + used;
+                for (int v = lane; v < nvec; v += 64) {
+                    const uint4 x = src[v];
+                    const uint32_t xw[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                    for (int kk = 0; kk < 16; kk++) {
+                        const int p = v * 16 + kk - lead;
+                        if (p >= 0 && p < slen) dst[p] = (uint8_t)(xw[kk >> 2] >> (8 * (kk & 3)));
+                    }
+                }
+                const uint32_t next = (used + (uint32_t)slen + 4u + 63u) & ~63u;
+                const uint32_t zend = next < (uint32_t)(kSubjBuf + kSeqPad) ? next : (uint32_t)(kSubjBuf + kSeqPad);
+                for (uint32_t i = used + (uint32_t)slen + lane; i < zend; i += 64) w.subj[i] = 0;
+                if (lane == 0) {
+                    w.grp->sid[ng] = sid;
+                    w.grp->start[ng] = used;
+                    w.grp->len[ng] = (uint32_t)slen;
+                    w.grp->vstart[ng] = used + (uint32_t)(ng * c.gap);
+                }
+                wave_sync();
+                ng++;
+                used = next;
+                continue;
+            }
+            if (ng > 0) run_group();  // the single-subject paths below reuse the buffer
             if constexpr (kLong) {
                 for (uint32_t i = lane; i < Mq; i += 64) wl.ring[i] = 0;
                 ring_sync<uint32_t>();
@@ -739,28 +941,8 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
                 if (slen > kNarrowMax) continue;  // the long-subject launch takes it
                 uint4 *r4 = reinterpret_cast<uint4 *>(w.ring);
                 for (uint32_t i = lane; i < Mq / 8; i += 64) r4[i] = uint4{0, 0, 0, 0};
-                if (slen <= kSubjBuf) {
-                    // stage the subject: 16-byte loads from a 16-byte aligned start, then realign
-                    const uint64_t a0 = o0 & ~uint64_t(15);
-                    const int lead = (int)(o0 - a0);
-                    const int nvec = (lead + slen + 15) >> 4;
-                    const uint4 *src = reinterpret_cast<const uint4 *>(a.res + a0);
-                    for (int v = lane; v < nvec; v += 64) {
-                        const uint4 x = src[v];
-                        const uint32_t xw[4] = {x.x, x.y, x.z, x.w};
-#pragma unroll
-                        for (int k = 0; k < 16; k++) {
-                            const int p = v * 16 + k - lead;
-                            if (p >= 0 && p < slen) w.subj[p] = (uint8_t)(xw[k >> 2] >> (8 * (k & 3)));
-                        }
-                    }
-                    if (lane < kSeqPad) w.subj[slen + lane] = 0;
-                    wave_sync();
-                    scan_subject<kTable, true>(c, w, t, (const uint8_t *)w.subj, slen, n_hits, n_ext);
-                } else {
-                    wave_sync();
-                    scan_subject<kTable, false>(c, w, t, Sg, slen, n_hits, n_ext);
-                }
+                wave_sync();
+                scan_subject<kTable, false>(c, w, t, Sg, slen, n_hits, n_ext);  // read from global memory
             }
 
             // hand the subject's init hits to the gapped stage
@@ -774,6 +956,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
                 wave_sync();
             }
         }
+        if (ng > 0) run_group();
     }
     unsigned long long ext_tot = n_ext, hit_tot = n_hits;
 #pragma unroll
@@ -829,6 +1012,7 @@ __global__ void __launch_bounds__(256) k_seed_retry(SeedArgs a) {
     w.task = reinterpret_cast<Task *>(w.hits + kHitQueue);
     w.defer = reinterpret_cast<uint32_t *>(w.task + kTaskCap);
     w.subj = reinterpret_cast<uint8_t *>(w.defer + kDeferCap);
+    w.grp = nullptr;
     for (int i = threadIdx.x; i < 28 * 28; i += blockDim.x) mat[i / 28][i % 28] = kBlosum62Dev[i / 28][i % 28];
     for (uint32_t i = lane; i < C; i += 64) w.claim[i] = 0xFFFFFFFFu;
     if (lane < 4) w.cnt[lane] = 0;
@@ -838,7 +1022,8 @@ __global__ void __launch_bounds__(256) k_seed_retry(SeedArgs a) {
     c.stage = a.stage + gwave * (uint64_t)a.stage_cap;
     c.stage_cap = a.stage_cap;
     c.claim_mask = a.claim_mask;
-    unsigned long long n_hits = 0, n_ext = 0;
+    c.gap = 0;
+    uint32_t n_hits = 0, n_ext = 0;  // not counted again
     for (;;) {
         uint32_t k = 0;
         if (lane == 0) k = atomicAdd(&a.ctr->next_retry, 1u);

@@ -131,6 +131,8 @@ typedef struct {
                                   database come again */
     int32_t gapped_lanes;      /* lanes per subject in the wave-parallel gapped DP: 16 (four
                                   subjects per wave) or 64; 0: 16 */
+    int32_t subject_groups;    /* 1: a wave scans one subject at a time; else short subjects of a
+                                  single-query scan share a wave's scan (up to 8, 1,024 letters) */
 } blz_tuning;
 
 blz_status blz_ctx_set_tuning(blz_ctx *ctx, const blz_tuning *t);
