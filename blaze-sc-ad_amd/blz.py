@@ -25,7 +25,7 @@ HSP_DTYPE = np.dtype(
 )
 
 STATUS = {0: "BLZ_OK", 1: "BLZ_EINVAL", 2: "BLZ_ENOMEM", 3: "BLZ_EDEVICE", 4: "BLZ_ECAPACITY",
-          5: "BLZ_EOVERFLOW"}
+          5: "BLZ_EOVERFLOW", 6: "BLZ_ETIMEOUT"}
 
 
 class BlzError(RuntimeError):
@@ -271,7 +271,10 @@ class Engine:
     def set_tuning(self, **kw):
         t = Tuning()
         self.lib.blz_ctx_get_tuning(self.ctx, C.byref(t))
+        names = {f for f, _ in Tuning._fields_}
         for k, v in kw.items():
+            if k not in names:
+                raise TypeError(f"unknown tuning field {k!r}")
             setattr(t, k, v)
         st = self.lib.blz_ctx_set_tuning(self.ctx, C.byref(t))
         if st:

@@ -26,6 +26,28 @@ def test_exports_match_header():
         assert hasattr(lib, name), name
 
 
+def header_struct_fields(name):
+    """Field names of a typedef struct in the header, in declaration order."""
+    text = (ROOT / "include" / "blz_prelim.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    body = re.search(r"typedef struct\s*\{([^}]*)\}\s*" + name + r"\s*;", text).group(1)
+    names = []
+    for decl in body.split(";"):
+        if decl.strip():  # "type a, b" declares a and b
+            names += [part.split()[-1].lstrip("*") for part in decl.split(",")]
+    return names
+
+
+@pytest.mark.parametrize("cname,pytype", [("blz_params", "Params"), ("blz_tuning", "Tuning"),
+                                          ("blz_run_stats", "RunStats"), ("blz_hsp_sink", "Sink"),
+                                          ("blz_server_config", "ServerConfig"),
+                                          ("blz_request_info", "RequestInfo"),
+                                          ("blz_server_stats", "ServerStats")])
+def test_binding_structs_match_header(cname, pytype):
+    """The ctypes mirrors declare the header's fields, in the header's order."""
+    assert header_struct_fields(cname) == [f for f, _ in getattr(blz, pytype)._fields_]
+
+
 def test_no_device_is_an_error_not_a_fallback():
     import torch
 
