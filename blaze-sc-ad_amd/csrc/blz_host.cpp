@@ -1239,24 +1239,48 @@ blz_status collect(blz_ctx *ctx, int nq, const blz_db *db, blz_hsp_sink *sinks, 
         (n_hs && !check(ctx, hipMemcpyAsync(hs_pin, ctx->hsps, sizeof(Hsp) * n_hs, hipMemcpyDeviceToHost, st), "hsps")) ||
         !check(ctx, hipStreamSynchronize(st), "sync"))
         return BLZ_EDEVICE;
-    std::vector<ResultItem> res(res_pin, res_pin + n_res);
-    std::vector<Hsp> hs(hs_pin, hs_pin + n_hs);
     const double t2 = log_time() ? now_ms() : 0;
     float ms_seed = 0, ms_gapped = 0;
     (void)hipEventElapsedTime(&ms_seed, ctx->ev[0], ctx->ev[1]);
     (void)hipEventElapsedTime(&ms_gapped, ctx->ev[1], ctx->ev[2]);
 
-    // per query (in the caller's order), subjects ascending: sort (query, subject) keys with the
-    // item index, then gather
+    // per query (in the caller's order), subjects ascending: the (query, subject) keys
+    // query * nsubj + subject are distinct, so an LSD radix sort of key << 24 | item index gives
+    // the order (a comparison sort when that does not fit 64 bits); then gather
+    std::vector<ResultItem> res(n_res);
     {
-        std::vector<std::pair<uint64_t, uint32_t>> key(res.size());
-        for (size_t i = 0; i < res.size(); i++)
-            key[i] = {((uint64_t)(uint32_t)res[i].qi << 40) | (uint64_t)res[i].sid, (uint32_t)i};
-        std::sort(key.begin(), key.end());
-        std::vector<ResultItem> sorted(res.size());
-        for (size_t i = 0; i < key.size(); i++) sorted[i] = res[key[i].second];
-        res.swap(sorted);
+        std::vector<uint64_t> key(n_res), tmp(n_res);
+        uint64_t kmax = 0;
+        for (size_t i = 0; i < n_res; i++) {
+            key[i] = (uint64_t)(uint32_t)res_pin[i].qi * (uint64_t)db->nsubj + (uint64_t)res_pin[i].sid;
+            kmax = std::max(kmax, key[i]);
+        }
+        const int kbits = kmax ? 64 - __builtin_clzll(kmax) : 1;
+        if (n_res < (1u << 24) && kbits + 24 <= 64) {
+            for (size_t i = 0; i < n_res; i++) key[i] = (key[i] << 24) | i;
+            constexpr int kDigit = 11;
+            std::vector<uint32_t> cnt(1u << kDigit);
+            for (int sh = 24; sh < 24 + kbits; sh += kDigit) {
+                std::fill(cnt.begin(), cnt.end(), 0u);
+                for (size_t i = 0; i < n_res; i++) cnt[(key[i] >> sh) & ((1u << kDigit) - 1)]++;
+                uint32_t acc = 0;
+                for (auto &c0 : cnt) {
+                    const uint32_t v = c0;
+                    c0 = acc;
+                    acc += v;
+                }
+                for (size_t i = 0; i < n_res; i++) tmp[cnt[(key[i] >> sh) & ((1u << kDigit) - 1)]++] = key[i];
+                key.swap(tmp);
+            }
+            for (size_t i = 0; i < n_res; i++) res[i] = res_pin[key[i] & 0xFFFFFFu];
+        } else {
+            std::vector<std::pair<uint64_t, uint32_t>> kv(n_res);
+            for (size_t i = 0; i < n_res; i++) kv[i] = {key[i], (uint32_t)i};
+            std::sort(kv.begin(), kv.end());
+            for (size_t i = 0; i < n_res; i++) res[i] = res_pin[kv[i].second];
+        }
     }
+    std::vector<Hsp> hs(hs_pin, hs_pin + n_hs);
     ctx->last_first.assign((size_t)nq + 1, 0);
     ctx->last_nh.assign((size_t)nq, 0);
     int64_t total = 0;
