@@ -163,7 +163,6 @@ struct blz_query {
     int32_t qlen = 0;
     blz_params params{};
     std::vector<uint8_t> letters;
-    bool has_stop = false;      // contains letter 0 (the separator of packs and subject groups)
     DevTables t;
 };
 
@@ -592,7 +591,6 @@ blz_status blz_query_prepare(blz_ctx *ctx, const uint8_t *query, int32_t qlen, c
     qq->qlen = qlen;
     qq->params = p;
     qq->letters.assign(query, query + qlen);
-    qq->has_stop = std::find(query, query + qlen, (uint8_t)0) != query + qlen;
     const blz_status st = build_tables(ctx, query, qlen, {{0, qlen}}, p.word_threshold, qq->t);
     if (st != BLZ_OK) {
         delete qq;
@@ -929,12 +927,10 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
             }
             d.claim_mask = std::min<uint32_t>(ring, 128) - 1;
             d.n_members = (int32_t)pk.members.size();
-            // short subjects share a scan when their letter-0 separators cannot hit the query
-            // and the virtual positions stay within the 14-bit ring entries
-            d.group_gap = d.n_members == 1 && ctx->tune.subject_groups != 1 && !q0->has_stop &&
-                                  q0->params.window <= 1024
-                              ? q0->params.window
-                              : 0;
+            // short subjects share a scan while the virtual positions stay within the 14-bit
+            // ring entries (no word spanning two subjects is looked up, so the letter-0
+            // separators never hit)
+            d.group_gap = ctx->tune.subject_groups != 1 && q0->params.window <= 1024 ? q0->params.window : 0;
             meta_off.push_back(meta.size());
             for (uint32_t v : pk.starts) meta.push_back(v);
             for (int m : pk.members) meta.push_back((uint32_t)m);

@@ -674,71 +674,87 @@ __device__ void emit_init_hits(const SeedArgs &a, const QueryDev &qd, WaveState<
 }
 
 // A group's init hits (staged by this wave, tagged with their subject) go to the pool, one work
-// item per subject that has any.  Single-query items only.
+// item per (subject, query of the item's pack) that has any.  Query offsets become offsets in
+// that query.  Bins are subject-major; their counts, best scores and first slots sit in the
+// wave's task, deferred and hit-queue lists, all empty once the group is scanned, so a group
+// holds at most kHitQueue / n_members subjects.
 template <class R>
 __device__ void emit_group(const SeedArgs &a, const QueryDev &qd, WaveState<R> &w, const Ctx &c, int ng, uint32_t nih) {
     const int lane = lane_id();
-    const int qi = (int)qd.member_idx[0];
-    auto overflow_all = [&]() {
-        if (lane < ng) {
-            const uint32_t k = atomicAdd(&a.ctr->overflow_count, 1u);
-            if (k < a.ovf_cap) a.ovf[k] = ((int64_t)qi << kOvfShift) | w.grp->sid[lane];
-        }
+    const int nm = qd.n_members;
+    const int nb = ng * nm;
+    auto record = [&](int bin) {
+        const uint32_t k = atomicAdd(&a.ctr->overflow_count, 1u);
+        if (k < a.ovf_cap) a.ovf[k] = ((int64_t)qd.member_idx[bin % nm] << kOvfShift) | w.grp->sid[bin / nm];
     };
     if (nih > (uint32_t)a.stage_cap) {
-        overflow_all();
+        for (int bin = lane; bin < nb; bin += 64) record(bin);
         return;
     }
     unsigned long long base = 0;
     if (lane == 0) base = atomicAdd(&a.ctr->pool_used, (unsigned long long)nih);
     base = first_lane64(base);
     if (base + nih > a.pool_cap) {
-        overflow_all();
+        for (int bin = lane; bin < nb; bin += 64) record(bin);
         return;
     }
-    uint32_t *cnt = w.p0;                 // [kGroupMax] counts, then cursors
-    uint32_t *bestm = w.excl;             // [kGroupMax] best score per subject
-    uint32_t *offm = w.excl + kGroupMax;  // [kGroupMax] first slot per subject
-    if (lane < ng) {
-        cnt[lane] = 0;
-        bestm[lane] = 0;
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(w.task);  // [nb] counts, then cursors
+    uint32_t *bestm = w.defer;                             // [nb] best score
+    uint32_t *offm = w.hits;                               // [nb] first slot
+    for (int bin = lane; bin < nb; bin += 64) {
+        cnt[bin] = 0;
+        bestm[bin] = 0;
     }
     wave_sync();
+    auto bin_of = [&](const InitHit &h, int &m) {
+        m = 0;
+        while (m + 1 < nm && (int)qd.member_start[m + 1] <= h.q_start) m++;
+        return (h.len >> 24) * nm + m;
+    };
     for (uint32_t k = lane; k < nih; k += 64) {
         const InitHit h = c.stage[k];
-        const int g = h.len >> 24;
-        atomicAdd(&cnt[g], 1u);
-        atomicMax(&bestm[g], (uint32_t)h.score);
+        int m;
+        const int bin = bin_of(h, m);
+        atomicAdd(&cnt[bin], 1u);
+        atomicMax(&bestm[bin], (uint32_t)h.score);
     }
     wave_sync();
-    const uint32_t mine = lane < ng ? cnt[lane] : 0u;
-    const uint32_t off = wave_inclusive_sum(mine) - mine;
-    wave_sync();
-    if (lane < ng) {
-        offm[lane] = off;
-        cnt[lane] = 0;
+    uint32_t run = 0;  // exclusive scan of the counts, 64 bins per pass
+    for (int b0 = 0; b0 < nb; b0 += 64) {
+        const uint32_t mine = b0 + lane < nb ? cnt[b0 + lane] : 0u;
+        const uint32_t inc = wave_inclusive_sum(mine);
+        if (b0 + lane < nb) offm[b0 + lane] = run + inc - mine;
+        run += last_lane(inc);
     }
+    wave_sync();
+    for (int bin = lane; bin < nb; bin += 64) cnt[bin] = 0;
     wave_sync();
     for (uint32_t k = lane; k < nih; k += 64) {
         InitHit h = c.stage[k];
-        const int g = h.len >> 24;
+        int m;
+        const int bin = bin_of(h, m);
         h.len &= 0xFFFFFF;
-        a.pool[base + offm[g] + atomicAdd(&cnt[g], 1u)] = h;
+        h.q_start -= (int)qd.member_start[m];
+        a.pool[base + offm[bin] + atomicAdd(&cnt[bin], 1u)] = h;
     }
     wave_sync();
-    if (lane < ng && mine > 0) {
-        const int64_t sid = w.grp->sid[lane];
+    for (int bin = lane; bin < nb; bin += 64) {
+        const uint32_t n = cnt[bin];
+        if (n == 0) continue;
+        const int qi = (int)qd.member_idx[bin % nm];
+        const int64_t sid = w.grp->sid[bin / nm];
         const uint32_t wi = atomicAdd(&a.ctr->work_count, 1u);
         if (wi < a.work_cap) {
-            const int bucket = cost_bucket((int)bestm[lane]);
-            a.work[wi] = WorkItem{sid, (uint64_t)(base + off), (int32_t)mine, qi, bucket, 0};
+            const int bucket = cost_bucket((int)bestm[bin]);
+            a.work[wi] = WorkItem{sid, (uint64_t)(base + offm[bin]), (int32_t)n, qi, bucket, 0};
             atomicAdd(&a.ctr->bucket_count[bucket], 1u);
-            atomicAdd(&a.ctr->init_hits, (unsigned long long)mine);
+            atomicAdd(&a.ctr->init_hits, (unsigned long long)n);
         } else {
             const uint32_t k = atomicAdd(&a.ctr->overflow_count, 1u);
             if (k < a.ovf_cap) a.ovf[k] = ((int64_t)qi << kOvfShift) | sid;
         }
     }
+    wave_sync();
 }
 
 }  // namespace
@@ -870,6 +886,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
         // full group is scanned as one
         int ng = 0;
         uint32_t used = 0;
+        const int group_cap = c.gap > 0 ? min(kGroupMax, kHitQueue / qd.n_members) : 1;
         auto run_group = [&]() {
             uint4 *r4 = reinterpret_cast<uint4 *>(w.ring);
             for (uint32_t i = lane; i < c.ring_len / 8; i += 64) r4[i] = uint4{0, 0, 0, 0};
@@ -879,8 +896,8 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
             const uint32_t nih = w.cnt[2];
             wave_sync();
             if (nih > 0) {
-                if (ng == 1)
-                    emit_init_hits(a, qd, w, c, w.grp->sid[0], nih);
+                if (ng == 1 && qd.n_members == 1)
+                    emit_init_hits(a, qd, w, c, w.grp->sid[0], nih);  // long lists leave sorted
                 else
                     emit_group(a, qd, w, c, ng, nih);
                 wave_sync();
@@ -901,7 +918,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
             const uint8_t *Sg = a.res + o0;
             const uint32_t Mq = c.ring_len;
             if (!kLong && slen <= kSubjBuf) {  // staged in LDS, alone when groups are off (gap 0)
-                if (ng == (c.gap > 0 ? kGroupMax : 1) || used + (uint32_t)slen > (uint32_t)kSubjBuf) run_group();
+                if (ng == group_cap || used + (uint32_t)slen > (uint32_t)kSubjBuf) run_group();
                 // letters at [used, used + slen) by 16-byte loads, then zeros up to the next
                 // 64-aligned start (at least 4) or the buffer's end pad
                 const uint64_t a0 = o0 & ~uint64_t(15);
