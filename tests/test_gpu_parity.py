@@ -95,6 +95,49 @@ def test_edge_subjects(engine):
     compare(engine, q, res, offs)
 
 
+def _mutated(rng, src, ident):
+    """Copy of src with a fraction 1 - ident of its letters replaced by random standard letters."""
+    out = src.copy()
+    std = np.array([1, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 22], np.uint8)
+    m = rng.random(len(out)) > ident
+    out[m] = std[rng.integers(0, len(std), size=int(m.sum()))]
+    return out
+
+
+def test_subject_group_boundaries(engine):
+    """Subjects around the sizes where a wave's group of subjects closes: eight-subject groups of
+    tiny subjects, 1,020-1,025 letters (alone, or too long for the LDS buffer), pairs whose
+    64-aligned starts just fit or just miss 1,024 letters, with homologous stretches at subject
+    starts and ends so HSPs and extensions run up to the zero separators."""
+    rng = np.random.default_rng(77)
+    q = blz.synth_query(31, 600)
+    lens = [3, 5, 9, 17, 30, 31, 40, 2, 4, 64, 60, 61, 1020, 1021, 1024, 1025, 500, 520, 448, 508,
+            459, 505, 100, 900, 200, 200, 200, 200, 200, 12, 700, 300]
+    subs = []
+    for k, L in enumerate(lens):
+        s = blz.synth_query(900 + k, L).copy()
+        if L >= 60:  # homologous stretch of the query at the start and at the end
+            n = min(L // 2, 150)
+            s[:n] = _mutated(rng, q[100:100 + n], 0.7)
+            s[L - n:] = _mutated(rng, q[300:300 + n], 0.8)
+        subs.append(s)
+    offs = np.zeros(len(subs) + 1, dtype=np.uint64)
+    offs[1:] = np.cumsum([len(s) for s in subs])
+    stats, nsub = compare(engine, q, np.concatenate(subs), offs)
+    assert nsub >= 10
+
+
+def test_wide_window_and_ungapped_xdrop_cap(engine):
+    """A two-hit window above 1,024 (subject groups off) is honoured; an ungapped X-drop above
+    16,384 (walks stop on letter 0's -32,768) is refused."""
+    q, res, offs, _ = make(23, 300, 800, homolog_frac=0.1, ident_min=0.4, ident_max=0.6)
+    kw = dict(window=1500)
+    compare(engine, q, res, offs, blz.default_params(**kw), oracle.params(**kw))
+    with pytest.raises(blz.BlzError) as e:
+        engine.prepare(q, blz.default_params(x_ungapped=16385))
+    assert e.value.status == 1  # BLZ_EINVAL
+
+
 def test_low_complexity(engine):
     """Poly-residue and short-period repeats: many hits per word and per diagonal."""
     q = np.concatenate([np.full(120, 1, np.uint8), np.tile(np.array([11, 4, 9], np.uint8), 60),
