@@ -127,7 +127,7 @@ hipError_t launch_gapped_lds(const GappedArgs &a, int blocks, hipStream_t st);
 hipError_t launch_gapped_wide(const GappedArgs &a, int blocks, int threads, hipStream_t st);
 constexpr int kLdsRing = 128;   // LDS DP ring entries per lane (lane-serial pass)
 constexpr int kWaveRing = 512;  // LDS DP ring entries per wave (wave pass); wider bands are redone
-constexpr int kSegRing = 256;   // LDS DP ring entries per 16-lane segment (segment pass)
+constexpr int kSegRing = 128;   // LDS DP ring entries per 16-lane segment (segment pass); measured bands stay below 112
 // gapped stage with one wave per subject of the heavy cost classes
 hipError_t launch_gapped_wave(const GappedArgs &a, int blocks, hipStream_t st);
 // Resident k_gapped_wave workgroups (one wave each) per CU.

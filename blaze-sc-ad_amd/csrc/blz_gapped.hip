@@ -385,6 +385,8 @@ __device__ int wave_semi_gapped(const int16_t (*mat)[28], const uint8_t *A, cons
     return best;
 }
 
+__device__ int finish_hsps(const GappedArgs &a, const WorkItem &wi, Hsp *saved, int ns);
+
 // Whole gapped stage of one subject, by one lane (kWave false) or by a whole wave (kWave true:
 // every lane runs the same control flow, the DP is shared across the lanes and lane 0 does the
 // sorting and the writes).  Returns kRedo if the DP band or the HSP count outgrew this pass's
@@ -460,9 +462,17 @@ __device__ int gapped_item(const GappedArgs &a, const int16_t (*mat)[28], uint32
         cells += my_cells;
         gext += my_ext;
     }
+    int outcome = writer ? finish_hsps(a, wi, saved, ns) : kDone;
+    if (kWave) outcome = Seg<kSeg>::first(outcome);
+    return outcome;
+}
+
+// The subject's saved HSPs: common-endpoint purge (NCBI blast_engine.c:792), score sort
+// (blast_engine.c:801), then the list and its result item out.  One lane.
+__device__ int finish_hsps(const GappedArgs &a, const WorkItem &wi, Hsp *saved, int ns) {
     if (ns == 0) return kDone;
     int outcome = kDone;
-    if (writer) {
+    {
         insertion_sort<cmp_start>(saved, ns);
         int m = 0;
         for (int i = 0; i < ns; i++) {
@@ -487,7 +497,6 @@ __device__ int gapped_item(const GappedArgs &a, const int16_t (*mat)[28], uint32
             a.results[ri] = ResultItem{wi.sid, (uint64_t)base, ns, wi.qi};
         }
     }
-    if (kWave) outcome = Seg<kSeg>::first(outcome);
     return outcome;
 }
 
@@ -629,44 +638,278 @@ __global__ void __launch_bounds__(64) k_gapped_wave(GappedArgs a) {
     }
 }
 
-// Gapped stage with four subjects per wave, one per 16-lane DPP row: the DP bands of most
-// subjects are a few dozen cells or fewer, so a whole wave per row leaves most lanes idle.
-// Same per-subject steps, caps and outcomes as k_gapped_wave.
-__global__ void __launch_bounds__(64) k_gapped_seg(GappedArgs a) {
+// Gapped stage with four subjects per wave, one per 16-lane DPP row.  Each segment runs its own
+// sequence of subjects, init hits and DPs as a state machine; every pass of the wave's loop
+// advances each segment by one step of what it is doing (take a subject, pick the next init hit
+// and its gapped start, one DP row), so a segment whose DP ends starts the next at once instead
+// of waiting at a loop exit for the other segments' rows.  Same per-subject steps, arithmetic,
+// caps and outcomes as gapped_item / wave_semi_gapped.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_gapped_seg(GappedArgs a) {
+    using S = Seg<16>;
     __shared__ int16_t mat[28][28];
     __shared__ int2 ring_mem[4][kSegRing];
     for (int i = threadIdx.x; i < 28 * 28; i += blockDim.x) mat[i / 28][i % 28] = kBlosum62Dev[i / 28][i % 28];
     __syncthreads();
     const int lane = lane_id();
     const int seg = lane >> 4;
+    const int sl = S::lane();
+    const bool writer = sl == 0;
     uint32_t heavy = 0;
     for (int b = 0; b < a.heavy_classes; b++) heavy += a.ctr->bucket_count[b];
-    const uint32_t n = *a.count < a.count_cap ? *a.count : a.count_cap;
-    if (heavy > n) heavy = n;
-    struct {
-        int2 *ring;
-        int wmask;
-    } ring{ring_mem[seg], kSegRing - 1};
+    const uint32_t navail = *a.count < a.count_cap ? *a.count : a.count_cap;
+    if (heavy > navail) heavy = navail;
+    int2 *ring = ring_mem[seg];
+    constexpr int wmask = kSegRing - 1, W = kSegRing;
     Hsp *saved = a.hsp_scratch + ((uint64_t)blockIdx.x * 4 + seg) * a.saved_cap;
     unsigned long long cells = 0, gext = 0;
-    for (;;) {  // each segment takes its own items; segments leave the loop independently
-        uint32_t i = 0;
-        if ((lane & 15) == 0) i = atomicAdd(a.next_heavy, 1u);
-        i = (uint32_t)__shfl((int)i, lane & 48, 64);
-        if (i >= heavy) break;
-        const uint32_t widx = a.order[i];
-        const int r = gapped_item<true, decltype(ring), 16>(a, mat, widx, ring, saved, cells, gext);
-        if ((lane & 15) == 0) {
-            if (r == kRedo) {
-                const uint32_t k = atomicAdd(&a.ctr->redo_count, 1u);
-                if (k < a.redo_cap) a.redo[k] = widx;
-                else record_overflow(a, widx);
-            } else if (r == kOverflow) {
-                record_overflow(a, widx);
+
+    enum { kTake, kHit, kDp, kDone };
+    int phase = kTake;
+    // the segment's subject
+    uint32_t widx = 0;
+    WorkItem wi{};
+    const InitHit *ih = nullptr;
+    const uint8_t *Sj = nullptr, *Q = nullptr;
+    int slen = 0, qlen = 0, n = 0, k = 0, ns = 0;
+    bool redo = false;
+    int xg = 0, gopen = 0, gext1 = 1, gcut = 0;
+    unsigned long long item_cells = 0, item_ext = 0;
+    int q_gs = 0, s_gs = 0, left = 0, la_l = 0, lb_l = 0;
+    // its current DP (wave_semi_gapped's state)
+    int dir = 0, M = 0, N = 0, X = 0, goe = 0, ge = 1;
+    const uint8_t *A = nullptr, *B = nullptr;
+    bool rev = false;
+    enum { kRun, kEmpty, kOvf };
+    int dps = kRun;  // the DP's status: running, empty (no rows or columns), band past the ring
+    int arow = 1, first = 0, size = 0, best = 0, ba = 0, bb = 0;
+    unsigned long long nc = 0;
+
+    // sets up the DP of direction d for init hit k (rows walk the query, the band the subject)
+    auto dp_begin = [&](int d) {
+        dir = d;
+        if (d == 0) {  // left: reversed prefixes, the gapped start included
+            A = Q; B = Sj; M = q_gs + 1; N = s_gs + 1; rev = true;
+        } else {       // right: the suffixes after the gapped start
+            A = Q + q_gs; B = Sj + s_gs; M = qlen - q_gs - 1; N = slen - s_gs - 1; rev = false;
+        }
+        goe = gopen + gext1;
+        ge = gext1;
+        X = xg < goe ? goe : xg;
+        best = 0; ba = 0; bb = 0; nc = 0; first = 0; arow = 1;
+        dps = N <= 0 || M <= 0 ? kEmpty : kRun;
+        if (dps == kEmpty) return;
+        const int k0 = (X - goe) / ge + 1;  // row 0: a run of gaps while it stays within the X-drop
+        size = (N < k0 ? N : k0) + 1;
+        if (size > W) {
+            dps = kOvf;
+            return;
+        }
+        for (int b = sl; b < size; b += 16) {
+            const int h = b == 0 ? 0 : -goe - (b - 1) * ge;
+            ring[b & wmask] = make_int2(h, h - goe);
+        }
+    };
+
+    for (;;) {
+        if (phase == kTake) {  // the next subject of the cost order
+            uint32_t i = 0;
+            if (writer) i = atomicAdd(a.next_heavy, 1u);
+            i = (uint32_t)__shfl((int)i, lane & 48, 64);
+            if (i >= heavy) {
+                phase = kDone;
+            } else {
+                widx = a.order[i];
+                wi = a.work[widx];
+                InitHit *ihw = a.pool + wi.off;
+                n = wi.n;
+                if (writer) {
+                    for (int x = 1; x < n; x++) {  // score order of the init hits (sorted in place once)
+                        InitHit h = ihw[x];
+                        int j = x - 1;
+                        while (j >= 0 && init_before(h, ihw[j])) {
+                            ihw[j + 1] = ihw[j];
+                            j--;
+                        }
+                        ihw[j + 1] = h;
+                    }
+                }
+                ih = ihw;
+                const uint64_t o0 = a.offs[wi.sid];
+                slen = (int)(a.offs[wi.sid + 1] - o0);
+                Sj = a.res + o0;
+                const QueryDev &qd = a.qd[wi.qi];
+                Q = qd.query;
+                qlen = qd.qlen;
+                xg = qd.x_gapped;
+                gopen = qd.gap_open;
+                gext1 = qd.gap_extend;
+                gcut = qd.gapped_cutoff;
+                k = 0;
+                ns = 0;
+                redo = false;
+                item_cells = 0;
+                item_ext = 0;
+                phase = kHit;
             }
         }
+        wave_sync();  // the sorted init hits before anyone reads them
+        if (phase == kHit) {  // the next init hit outside the HSPs saved so far, or the end of the subject
+            bool found = false;
+            while (k < n) {
+                const InitHit h = ih[k];
+                const int q0 = h.q_start, q1 = h.q_start + h.len - 1;
+                const int s0 = h.s_start, s1 = h.s_start + h.len - 1;
+                bool skip = false;
+                for (int t = 0; t < ns && !skip; t++) {
+                    const Hsp &u = saved[t];
+                    skip = h.score <= u.score && u.q_off <= q0 && q0 <= u.q_end && u.s_off <= s0 && s0 <= u.s_end &&
+                           u.q_off <= q1 && q1 <= u.q_end && u.s_off <= s1 && s1 <= u.s_end;
+                }
+                if (!skip) {
+                    q_gs = gapped_start(mat, Q, Sj, h.q_start, h.s_start, h.len);
+                    s_gs = h.s_start + (q_gs - h.q_start);
+                    found = true;
+                    break;
+                }
+                k++;
+            }
+            if (found) {
+                dp_begin(0);
+                phase = kDp;
+            } else {  // all init hits done: the subject's list out
+                if (writer) {
+                    int r = kRedo;
+                    if (!redo) {
+                        cells += item_cells;
+                        gext += item_ext;
+                        r = finish_hsps(a, wi, saved, ns);
+                    }
+                    if (r == kRedo) {
+                        const uint32_t kk = atomicAdd(&a.ctr->redo_count, 1u);
+                        if (kk < a.redo_cap) a.redo[kk] = widx;
+                        else record_overflow(a, widx);
+                    } else if (r == kOverflow) {
+                        record_overflow(a, widx);
+                    }
+                }
+                phase = kTake;
+            }
+        } else if (phase == kDp) {  // one row of the DP
+            bool done = dps != kRun || arow > M;
+            if (!done) {
+                const int16_t *mrow = mat[rev ? A[M - arow] : A[arow]];
+                int carry_g = kMinIntDp;   // max of max(D,E)(b') + b' * extend over earlier chunks
+                int carry_h = best;        // running best over earlier chunks
+                int carry_hp = kMinIntDp;  // previous row's H left of this chunk
+                int fa = -1, la = -1, rmax = kMinIntDp, rmax_b = -1;
+                for (int c0 = first; c0 < size; c0 += 16) {
+                    const int b = c0 + sl;
+                    const bool valid = b < size;
+                    const int2 prev = valid ? ring[b & wmask] : make_int2(kMinIntDp, kMinIntDp);
+                    const int hp_left = S::shift_up1(prev.x, carry_hp);
+                    const int sb = (valid && b >= 1 && b <= N) ? mrow[rev ? B[N - b] : B[b]] : 0;
+                    const int de = valid ? max(hp_left + sb, prev.y) : kMinIntDp;
+                    const int g = valid ? de + b * ge : kMinIntDp;
+                    const int gin = S::incl_max(g);
+                    const int gex = max(S::shift_up1(gin, kMinIntDp), carry_g);
+                    const int f = gex - goe - (b - 1) * ge;
+                    const int h = valid ? max(de, f) : kMinIntDp;
+                    const int hin = S::incl_max(h);
+                    const int hex = max(S::shift_up1(hin, kMinIntDp), carry_h);
+                    const bool alive = valid && hex - h <= X;
+                    const uint64_t am = S::ballot(alive);
+                    if (am) {
+                        if (fa < 0) fa = c0 + (int)__builtin_ctzll(am);
+                        la = c0 + 63 - (int)__builtin_clzll(am);
+                    }
+                    const int cm = S::last(hin);
+                    if (cm > carry_h) {
+                        rmax = cm;
+                        rmax_b = c0 + (int)__builtin_ctzll(S::ballot(valid && h == cm));
+                    }
+                    if (valid)
+                        ring[b & wmask] = alive ? make_int2(h, max(h - goe, prev.y - ge)) : make_int2(kMinIntDp, kMinIntDp);
+                    carry_g = max(carry_g, S::last(gin));
+                    carry_h = max(carry_h, cm);
+                    carry_hp = S::last(prev.x);
+                }
+                nc += (unsigned long long)(size - first);
+                if (fa < 0) {
+                    done = true;
+                } else {
+                    if (rmax > best) {
+                        best = rmax;
+                        ba = arow;
+                        bb = rmax_b;
+                    }
+                    first = fa;
+                    if (la < size - 1) {
+                        size = la + 1;
+                    } else {
+                        const int gr = carry_g - goe - (size - 1) * ge;  // row gap one past the last cell
+                        if (gr >= best - X) {
+                            int kk = (gr - (best - X)) / ge + 1;
+                            if (kk > N + 1 - size) kk = N + 1 - size;
+                            if (size + kk - first > W) {
+                                dps = kOvf;
+                            } else {
+                                for (int i = sl; i < kk; i += 16) {
+                                    const int hv = gr - i * ge;
+                                    ring[(size + i) & wmask] = make_int2(hv, hv - goe);
+                                }
+                                size += kk;
+                            }
+                        }
+                    }
+                    if (dps == kRun && size <= N) {
+                        if (size + 1 - first > W) {
+                            dps = kOvf;
+                        } else {
+                            if (sl == 0) ring[size & wmask] = make_int2(kMinIntDp, kMinIntDp);
+                            size++;
+                        }
+                    }
+                    arow++;
+                    done = dps != kRun || arow > M;
+                }
+            }
+            if (done) {
+                const bool empty = dps == kEmpty;
+                const int score = empty ? 0 : best;
+                const int oa = empty ? 0 : ba, ob = empty ? 0 : bb;
+                if (dps == kOvf) {  // band past the ring: the subject goes to the wide pass
+                    redo = true;
+                    k = n;
+                    phase = kHit;
+                } else if (dir == 0) {
+                    if (!empty) item_cells += nc;
+                    left = score;
+                    la_l = oa;
+                    lb_l = ob;
+                    dp_begin(1);
+                } else {
+                    if (!empty) item_cells += nc;
+                    item_ext++;
+                    const int total = left + score;
+                    if (total >= gcut) {
+                        if (ns >= a.saved_cap) {  // more HSPs than this pass can hold
+                            redo = true;
+                            k = n;
+                        } else {
+                            if (writer)
+                                saved[ns] = Hsp{total, q_gs + 1 - la_l, q_gs + 1 + oa, s_gs + 1 - lb_l, s_gs + 1 + ob, q_gs, s_gs};
+                            ns++;
+                        }
+                    }
+                    if (k < n) k++;
+                    phase = kHit;
+                }
+            }
+        }
+        wave_sync();
+        if (__ballot(phase != kDone) == 0) break;
     }
-    if ((lane & 15) == 0) {
+    if (writer) {
         atomicAdd(&a.ctr->dp_cells, cells);
         atomicAdd(&a.ctr->gapped_ext, gext);
     }
