@@ -356,7 +356,7 @@ __device__ int wave_semi_gapped(const int16_t (*mat)[28], const uint8_t *A, cons
         } else {
             const int gr = carry_g - goe - (size - 1) * ge;  // row gap one past the last cell
             if (gr >= best - X) {
-                int k = (gr - (best - X)) / ge + 1;
+                int k = (ge == 1 ? gr - (best - X) : (gr - (best - X)) / ge) + 1;
                 if (k > N + 1 - size) k = N + 1 - size;
                 if (size + k - first > W) {
                     *ovf = true;
@@ -848,7 +848,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
                     } else {
                         const int gr = carry_g - goe - (size - 1) * ge;  // row gap one past the last cell
                         if (gr >= best - X) {
-                            int kk = (gr - (best - X)) / ge + 1;
+                            // cells of the run (a division only for a gap extension other than 1)
+                            int kk = (ge == 1 ? gr - (best - X) : (gr - (best - X)) / ge) + 1;
                             if (kk > N + 1 - size) kk = N + 1 - size;
                             if (size + kk - first > W) {
                                 dps = kOvf;
