@@ -31,10 +31,13 @@ def time_batch(engine, qhs, tuning: dict, repeats: int = 1) -> float:
     return best
 
 
-def tune(engine, qhs, candidates: dict[str, list[int]] | None = None, repeats: int = 1,
-         log: Callable[[str], None] | None = None) -> tuple[dict, float]:
-    """Returns (best tuning, its device ms) and leaves the engine set to it.  Graph capture is
-    switched off while tuning (every candidate is a new launch sequence)."""
+def tune(engine, qhs, candidates: dict[str, list[int]] | None = None, repeats: int = 2,
+         margin: float = 0.05, log: Callable[[str], None] | None = None) -> tuple[dict, float]:
+    """Returns (best tuning, its device ms) and leaves the engine set to it.  Each candidate is
+    timed `repeats` times (best kept) and replaces the current choice only when it is faster by
+    more than `margin`: run-to-run noise on a sample batch is a few percent, and a choice that
+    wins on the sample by less than that has lost on the full batch.  Graph capture is switched
+    off while tuning (every candidate is a new launch sequence)."""
     candidates = candidates or CANDIDATES
     start = {**engine.get_tuning(), "graph": 1}
     best = dict(start)
@@ -49,7 +52,7 @@ def tune(engine, qhs, candidates: dict[str, list[int]] | None = None, repeats: i
             ms = time_batch(engine, qhs, trial, repeats)
             if log:
                 log(f"tune: {key}={v} {ms:.2f} ms")
-            if ms < best_ms * 0.98:  # keep a change only when it is clearly faster
+            if ms < best_ms * (1.0 - margin):  # keep a change only when it is clearly faster
                 best, best_ms = trial, ms
     best["graph"] = 0  # capture again once the choice is made
     engine.set_tuning(**best)
