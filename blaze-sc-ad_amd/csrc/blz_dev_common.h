@@ -21,16 +21,29 @@ __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 
 // Inclusive prefix sum over the 64 lanes with DPP: row_shr 1/2/4/8 inside each 16-lane row,
 // then row_bcast:15 (rows 1 and 3 add the last lane of rows 0 and 2) and row_bcast:31 (rows 2
-// and 3 add lane 31).  Lanes whose DPP source is outside the row read 0.
+// and 3 add lane 31).  Each step is one v_add_u32_dpp whose destination is its own second
+// operand: a lane whose DPP source lies outside the row, or whose row the row mask leaves out,
+// is not written and keeps its sum.  (The compiler does not fuse the builtin form into the add
+// here, which costs two extra moves per step.)  Each DPP read follows an s_nop: a VALU write of
+// the register needs two wait states before DPP reads it, a VALU write of EXEC five (the first
+// one, whose predecessor is not known here).  All 64 lanes must be active.
 __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
-    int x = (int)v;
-    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);
-    return (uint32_t)x;
+    uint32_t x = v;
+    asm volatile(
+        "s_nop 4\n\t"
+        "v_add_u32_dpp %0, %0, %0 row_shr:1 row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 1\n\t"
+        "v_add_u32_dpp %0, %0, %0 row_shr:2 row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 1\n\t"
+        "v_add_u32_dpp %0, %0, %0 row_shr:4 row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 1\n\t"
+        "v_add_u32_dpp %0, %0, %0 row_shr:8 row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 1\n\t"
+        "v_add_u32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+        "s_nop 1\n\t"
+        "v_add_u32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf"
+        : "+v"(x));
+    return x;
 }
 
 // Inclusive prefix max over the 64 lanes with DPP (same lane pattern as wave_inclusive_sum);
