@@ -17,7 +17,7 @@ CANDIDATES: dict[str, list[int]] = {
     "seed_waves": [0, 6, 8, 12, 16],
     "bin_ring": [0, 800, 1600, 2400],
     "pack_len": [0, 600, 1400],
-    "gapped_wave_classes": [0, 3],
+    "gapped_wave_classes": [0, 8],
 }
 
 

@@ -35,10 +35,15 @@ struct WorkItem {
     int32_t pad;
 };
 
-constexpr int kBuckets = 4;
-// cost class of a subject from its best ungapped score
-__host__ __device__ inline int cost_bucket(int best_ungapped) {
-    return best_ungapped >= 150 ? 0 : best_ungapped >= 80 ? 1 : best_ungapped >= 55 ? 2 : 3;
+constexpr int kBuckets = 8;
+// Expected gapped-stage cost class of a subject, 0 = heaviest, from its init hit count and best
+// ungapped score.  Each init hit outside the HSPs found before it is a DP of its own, and a
+// chance hit's DP runs about as long as a homolog's (X-drop 38 in a random walk), so the count
+// comes first: 5 or more, 4, 3, 2, then single hits by score (150, 80, 55).
+__host__ __device__ inline int cost_bucket(int best_ungapped, int n_init) {
+    if (n_init >= 5) return 0;
+    if (n_init >= 2) return 5 - n_init;
+    return best_ungapped >= 150 ? 4 : best_ungapped >= 80 ? 5 : best_ungapped >= 55 ? 6 : 7;
 }
 
 // HSP record, same layout as blz_hsp in blz_prelim.h.

@@ -165,7 +165,8 @@ __device__ int semi_gapped(const int16_t (*mat)[28], const uint8_t *A, const uin
 // Gapped start: middle of a short segment, else the centre of its best 11-letter window when
 // that window scores above 0, else the segment start (reference restatement at
 // gpuBLAZE.cu:1166-1205).
-__device__ __forceinline__ int gapped_start(const int16_t (*mat)[28], const uint8_t *q, const uint8_t *s,
+template <int kStride>
+__device__ __forceinline__ int gapped_start(const int16_t (*mat)[kStride], const uint8_t *q, const uint8_t *s,
                                             int q_start, int s_start, int len) {
     if (len <= kHspMaxWindow) return q_start + len / 2;
     int score = 0;
@@ -264,6 +265,7 @@ struct Seg<16> {
     static __device__ __forceinline__ int last(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x15F, 0xf, 0xf, false); }
     static __device__ __forceinline__ uint64_t ballot(bool p) { return (__ballot(p) >> (lane_id() & 48)) & 0xFFFFull; }
     static __device__ __forceinline__ int first(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x150, 0xf, 0xf, false); }
+    static __device__ __forceinline__ int reduce_max(int x) { return last(incl_max(x)); }
 };
 
 // Wave-parallel semi-gapped X-drop DP for one extension (all 64 lanes call it with the same
@@ -646,9 +648,14 @@ __global__ void __launch_bounds__(64) k_gapped_wave(GappedArgs a) {
 // caps and outcomes as gapped_item / wave_semi_gapped.
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_gapped_seg(GappedArgs a) {
     using S = Seg<16>;
-    __shared__ int16_t mat[28][28];
+    constexpr int kLetRing = 256;   // letters of a DP's columns per segment (the band and 64 ahead)
+    constexpr int kZeroLetter = 28;  // scores 0 against every letter: the gapped start's own column
+    static_assert(kSegRing + 64 <= kLetRing, "a band and the 64 letters filled past it fit the letter ring");
+    __shared__ int16_t mat[28][32];
     __shared__ int2 ring_mem[4][kSegRing];
-    for (int i = threadIdx.x; i < 28 * 28; i += blockDim.x) mat[i / 28][i % 28] = kBlosum62Dev[i / 28][i % 28];
+    __shared__ uint8_t lett_mem[4][kLetRing];
+    for (int i = threadIdx.x; i < 28 * 32; i += blockDim.x) mat[i / 32][i % 32] = i % 32 < 28 ? kBlosum62Dev[i / 32][i % 32] : 0;
+    for (int i = threadIdx.x; i < 4 * kLetRing; i += blockDim.x) lett_mem[i / kLetRing][i % kLetRing] = kZeroLetter;
     __syncthreads();
     const int lane = lane_id();
     const int seg = lane >> 4;
@@ -659,6 +666,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
     const uint32_t navail = *a.count < a.count_cap ? *a.count : a.count_cap;
     if (heavy > navail) heavy = navail;
     int2 *ring = ring_mem[seg];
+    uint8_t *lett = lett_mem[seg];
     constexpr int wmask = kSegRing - 1, W = kSegRing;
     Hsp *saved = a.hsp_scratch + ((uint64_t)blockIdx.x * 4 + seg) * a.saved_cap;
     unsigned long long cells = 0, gext = 0;
@@ -682,7 +690,22 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
     enum { kRun, kEmpty, kOvf };
     int dps = kRun;  // the DP's status: running, empty (no rows or columns), band past the ring
     int arow = 1, first = 0, size = 0, best = 0, ba = 0, bb = 0;
+    int filled = 0;  // columns [0, filled) have their letters in the letter ring
     unsigned long long nc = 0;
+
+    // Letters of the DP's columns up to the band's end, 64 columns at a time; column 0 (the
+    // gapped start, no letter of B) takes the zero-score letter.  The band spans at most W = 128
+    // columns, so the 256 slots never hold two live columns.
+    auto fill_letters = [&]() {
+        while (filled < size) {
+            for (int j = sl; j < 64; j += 16) {
+                const int b = filled + j;
+                const int lt = b >= 1 && b <= N ? (int)(rev ? B[N - b] : B[b]) : kZeroLetter;
+                lett[b & (kLetRing - 1)] = (uint8_t)lt;
+            }
+            filled += 64;
+        }
+    };
 
     // sets up the DP of direction d for init hit k (rows walk the query, the band the subject)
     auto dp_begin = [&](int d) {
@@ -708,6 +731,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
             const int h = b == 0 ? 0 : -goe - (b - 1) * ge;
             ring[b & wmask] = make_int2(h, h - goe);
         }
+        filled = 0;
+        fill_letters();
     };
 
     for (;;) {
@@ -801,31 +826,33 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
                 int carry_g = kMinIntDp;   // max of max(D,E)(b') + b' * extend over earlier chunks
                 int carry_h = best;        // running best over earlier chunks
                 int carry_hp = kMinIntDp;  // previous row's H left of this chunk
-                int fa = -1, la = -1, rmax = kMinIntDp, rmax_b = -1;
+                int fa = 0x7fffffff, la = -1;  // this lane's first and last surviving column
+                int rmax = kMinIntDp, rmax_b = -1;
                 for (int c0 = first; c0 < size; c0 += 16) {
                     const int b = c0 + sl;
                     const bool valid = b < size;
-                    const int2 prev = valid ? ring[b & wmask] : make_int2(kMinIntDp, kMinIntDp);
+                    // lanes past the band read ring and letter slots they do not use
+                    const int2 prev = ring[b & wmask];
+                    const int sb = mrow[lett[b & (kLetRing - 1)]];
                     const int hp_left = S::shift_up1(prev.x, carry_hp);
-                    const int sb = (valid && b >= 1 && b <= N) ? mrow[rev ? B[N - b] : B[b]] : 0;
                     const int de = valid ? max(hp_left + sb, prev.y) : kMinIntDp;
                     const int g = valid ? de + b * ge : kMinIntDp;
+                    // The running best before a cell is a prefix max of max(D,E) alone: a row gap
+                    // F(b) is at most max(D,E)(b') - open for some earlier b', so it never sets a
+                    // new best.  The two prefix scans are then independent.
                     const int gin = S::incl_max(g);
+                    const int din = S::incl_max(de);
                     const int gex = max(S::shift_up1(gin, kMinIntDp), carry_g);
-                    const int f = gex - goe - (b - 1) * ge;
-                    const int h = valid ? max(de, f) : kMinIntDp;
-                    const int hin = S::incl_max(h);
-                    const int hex = max(S::shift_up1(hin, kMinIntDp), carry_h);
-                    const bool alive = valid && hex - h <= X;
-                    const uint64_t am = S::ballot(alive);
-                    if (am) {
-                        if (fa < 0) fa = c0 + (int)__builtin_ctzll(am);
-                        la = c0 + 63 - (int)__builtin_clzll(am);
-                    }
-                    const int cm = S::last(hin);
-                    if (cm > carry_h) {
+                    const int h = max(de, gex - goe - (b - 1) * ge);
+                    // (best before the cell) - H <= X, with the cell's own score counted: equal
+                    // when H is below that best, and 0 <= X otherwise
+                    const bool alive = valid && max(din, carry_h) - h <= X;
+                    fa = min(fa, alive ? b : 0x7fffffff);
+                    la = alive ? b : la;
+                    const int cm = S::last(din);
+                    if (cm > carry_h) {  // a new best in this chunk: its first cell (H equals D/E there)
                         rmax = cm;
-                        rmax_b = c0 + (int)__builtin_ctzll(S::ballot(valid && h == cm));
+                        rmax_b = c0 + (int)__builtin_ctzll(S::ballot(de == cm));
                     }
                     if (valid)
                         ring[b & wmask] = alive ? make_int2(h, max(h - goe, prev.y - ge)) : make_int2(kMinIntDp, kMinIntDp);
@@ -834,7 +861,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
                     carry_hp = S::last(prev.x);
                 }
                 nc += (unsigned long long)(size - first);
-                if (fa < 0) {
+                la = S::reduce_max(la);
+                fa = -S::reduce_max(-fa);
+                if (la < 0) {
                     done = true;
                 } else {
                     if (rmax > best) {
@@ -870,6 +899,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
                             size++;
                         }
                     }
+                    if (dps == kRun) fill_letters();
                     arow++;
                     done = dps != kRun || arow > M;
                 }

@@ -610,7 +610,7 @@ __device__ void emit_init_hits(const SeedArgs &a, const QueryDev &qd, WaveState<
             const int qi = (int)qd.member_idx[0];
             const uint32_t wi = atomicAdd(&a.ctr->work_count, 1u);
             if (wi < a.work_cap) {
-                const int bucket = cost_bucket(best);
+                const int bucket = cost_bucket(best, (int)nih);
                 a.work[wi] = WorkItem{sid, (uint64_t)base, (int32_t)nih, qi, bucket, 0};
                 atomicAdd(&a.ctr->bucket_count[bucket], 1u);
                 atomicAdd(&a.ctr->init_hits, (unsigned long long)nih);
@@ -662,7 +662,7 @@ __device__ void emit_init_hits(const SeedArgs &a, const QueryDev &qd, WaveState<
         const int qi = (int)qd.member_idx[lane];
         const uint32_t wi = atomicAdd(&a.ctr->work_count, 1u);
         if (wi < a.work_cap) {
-            const int bucket = cost_bucket((int)bestm[lane]);
+            const int bucket = cost_bucket((int)bestm[lane], (int)mine);
             a.work[wi] = WorkItem{sid, (uint64_t)(base + off), (int32_t)mine, qi, bucket, 0};
             atomicAdd(&a.ctr->bucket_count[bucket], 1u);
             atomicAdd(&a.ctr->init_hits, (unsigned long long)mine);
@@ -745,7 +745,7 @@ __device__ void emit_group(const SeedArgs &a, const QueryDev &qd, WaveState<R> &
         const int64_t sid = w.grp->sid[bin / nm];
         const uint32_t wi = atomicAdd(&a.ctr->work_count, 1u);
         if (wi < a.work_cap) {
-            const int bucket = cost_bucket((int)bestm[bin]);
+            const int bucket = cost_bucket((int)bestm[bin], (int)n);
             a.work[wi] = WorkItem{sid, (uint64_t)(base + offm[bin]), (int32_t)n, qi, bucket, 0};
             atomicAdd(&a.ctr->bucket_count[bucket], 1u);
             atomicAdd(&a.ctr->init_hits, (unsigned long long)n);

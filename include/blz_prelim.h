@@ -121,7 +121,7 @@ typedef struct {
     int32_t gapped_threads;    /* threads per workgroup of the wide gapped kernel */
     int32_t seed_table;        /* lookup tables: 1 global, 2 entries in LDS, 3 all in LDS (0 auto) */
     int32_t gapped_wave_classes;/* cost classes 0..k-2 run one wave per subject, the rest one lane;
-                                  k = 1..5 (0: all classes on waves) */
+                                  k = 1..9 (0: all classes on waves) */
     int32_t query_packing;     /* batches: 1 scans each query on its own, else short queries share scans */
     int32_t pack_len;          /* letters of a query pack (concatenation); 0: 1024 - 128 - window */
     int32_t bin_ring;          /* query-size bins of a batch: diagonal-ring lengths up to bin_ring,
