@@ -236,6 +236,19 @@ def test_params_variants(engine):
     compare(engine, q, res, offs, blz.default_params(**kw), oracle.params(**kw))
 
 
+@pytest.mark.parametrize("wave_classes", [1, 5])
+def test_gapped_routing(engine, wave_classes):
+    """Cost classes routed to the lane-per-subject DP (every class, or the single-hit classes
+    while subjects with several init hits stay on waves) give the oracle's lists too."""
+    saved = engine.get_tuning()
+    q, res, offs, _ = make(31, 500, 20000, homolog_frac=0.05, ident_min=0.35, ident_max=0.7)
+    engine.set_tuning(gapped_wave_classes=wave_classes)
+    try:
+        compare(engine, q, res, offs)
+    finally:
+        engine.set_tuning(**saved)
+
+
 def test_config1_full_size(engine):
     """Config 1 at full size: 500 aa query vs 2,000,000 subjects, 1% homologs at 35-70%.
     The oracle runs multi-threaded on the host."""
