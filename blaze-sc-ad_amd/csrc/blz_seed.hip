@@ -70,6 +70,8 @@ constexpr int kHitBuf = 128;                // most hits of one step that go thr
 constexpr int kHitQueue = 192;              // queued hits: fewer than 64 waiting + one step
 constexpr int kSubjBuf = 1024;              // subject letters staged in LDS per wave
 constexpr int kSeqPad = 16;                 // zero letters before and after a sequence staged in LDS
+constexpr int kMatStride = 32;              // LDS matrix row: 28 scores, 4 unused (64 bytes)
+constexpr uint32_t kMatBytes = 28 * kMatStride * 2;
 static_assert(kSubjBuf <= kNarrowMax, "staged subjects use the LDS ring");
 static_assert(kWordRecInline == 6, "scan_subject writes six inline entries");
 
@@ -114,15 +116,16 @@ __host__ __device__ inline uint32_t wave_lds_bytes(uint32_t M, uint32_t C) {
 }
 
 __host__ __device__ inline uint32_t block_lds_bytes(int table, uint32_t qlen, uint32_t n_entries) {
-    uint32_t b = 16 + al16(28 * 28 * 2) + kSeqPad + al16(qlen + kSeqPad);
+    uint32_t b = 16 + al16(kMatBytes) + kSeqPad + al16(qlen + kSeqPad) + al16(2 * (qlen + 2 * kSeqPad));
     if (table >= 1) b += al16(n_entries * 2 + 4);
     if (table >= 2) b += al16((kWords + 1) * 2) + al16(kRelBlocks * 4);
     return b;
 }
 
 struct Ctx {
-    const int16_t (*mat)[28];
+    const int16_t (*mat)[kMatStride];
     const uint8_t *Q;
+    const uint16_t *Qx;  // LDS: the query letters times a matrix row's bytes (kPad walks)
     int qlen;
     uint32_t ring_len, ring_base, ring_magic;
     uint32_t claim_mask;
@@ -152,6 +155,17 @@ __device__ __forceinline__ uint32_t ring_slot(const Ctx &c, int q, int s) {
 // end reads an in-bounds letter and scores kEndScore instead.
 constexpr int kEndScore = -32768;
 
+// Score of the query letter at qi against a subject letter.  With kPad both come from LDS in
+// byte-offset form: c.Qx holds each query letter times a matrix row's bytes and the wave's
+// subject buffer each subject letter doubled, so the lookup address is one add.  Otherwise sv is
+// the subject letter itself and the query letter comes from c.Q.
+template <bool kPad>
+__device__ __forceinline__ int pair_score(const Ctx &c, int qi, uint32_t sv) {
+    if constexpr (kPad)
+        return *reinterpret_cast<const int16_t *>(reinterpret_cast<const uint8_t *>(c.mat) + (uint32_t)c.Qx[qi] + sv);
+    return c.mat[c.Q[qi]][sv];
+}
+
 template <bool kPad, class SP>
 __device__ __forceinline__ int walk_left(const Ctx &c, SP S, int q_off, int s_off, int dropoff, int *length) {
     const int n = s_off < q_off ? s_off : q_off;  // last k inside both sequences
@@ -163,10 +177,10 @@ __device__ __forceinline__ int walk_left(const Ctx &c, SP S, int q_off, int s_of
         for (int j = 0; j < 4; j++) {
             const int k = k0 + j;
             if constexpr (kPad) {
-                sc[j] = c.mat[c.Q[q_off - k]][S[s_off - k]];
+                sc[j] = pair_score<true>(c, q_off - k, S[s_off - k]);
             } else {
                 const int kk = k <= n ? k : n;
-                const int v = c.mat[c.Q[q_off - kk]][S[s_off - kk]];
+                const int v = pair_score<false>(c, q_off - kk, S[s_off - kk]);
                 sc[j] = k <= n ? v : kEndScore;
             }
         }
@@ -198,10 +212,10 @@ __device__ __forceinline__ int walk_right(const Ctx &c, SP S, int slen, int q_of
         for (int j = 0; j < 4; j++) {
             const int k = k0 + j;
             if constexpr (kPad) {
-                sc[j] = c.mat[c.Q[q_off + k]][S[s_off + k]];
+                sc[j] = pair_score<true>(c, q_off + k, S[s_off + k]);
             } else {
                 const int kk = k < n ? k : nl;
-                const int v = c.mat[c.Q[q_off + kk]][S[s_off + kk]];
+                const int v = pair_score<false>(c, q_off + kk, S[s_off + kk]);
                 sc[j] = k < n ? v : kEndScore;
             }
         }
@@ -231,7 +245,7 @@ __device__ __forceinline__ int two_hit_walk(const Ctx &c, SP S, int slen, int s_
     int left_d = 0, right_d = 0, left_score = 0, right_score = 0, score = 0;
 #pragma unroll
     for (int i = 0; i < kWord; i++) {
-        score += c.mat[c.Q[q_right_off + i]][S[s_right_off + i]];
+        score += pair_score<kPad>(c, q_right_off + i, S[s_right_off + i]);
         if (score > left_score) {
             left_score = score;
             right_d = i + 1;
@@ -506,7 +520,8 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
         uint32_t pa = 0, nh = 0;
         u32x4 r = {0u, 0u, 0u, 0u};  // kTable 0: the word's record
         if (p + kWord <= p_end) {
-            const uint32_t wd = S[p] * 784u + S[p + 1] * 28u + S[p + 2];
+            // (letters in the LDS buffer are doubled)
+            const uint32_t wd = (S[p] * 784u + S[p + 1] * 28u + S[p + 2]) >> (kPad ? 1 : 0);
             if constexpr (kTable == 0) {
                 r = gmem(reinterpret_cast<const u32x4 *>(t.rec))[wd];
                 nh = r.x & 0xFFu;
@@ -777,10 +792,12 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
     uint32_t off = 0;
     int32_t *blk = reinterpret_cast<int32_t *>(smem);  // [0] item, [1] next subject of the item
     off += 16;
-    int16_t(*mat)[28] = reinterpret_cast<int16_t(*)[28]>(smem + off);
-    off += al16(28 * 28 * 2);
+    int16_t(*mat)[kMatStride] = reinterpret_cast<int16_t(*)[kMatStride]>(smem + off);
+    off += al16(kMatBytes);
     uint8_t *Qs = smem + off + kSeqPad;  // kSeqPad zero letters on both sides
     off += kSeqPad + al16((uint32_t)a.qlen + kSeqPad);
+    uint16_t *Qx = reinterpret_cast<uint16_t *>(smem + off) + kSeqPad;  // the same letters times 2 * kMatStride
+    off += al16(2 * ((uint32_t)a.qlen + 2 * kSeqPad));
     uint16_t *ent_l = nullptr;
     uint16_t *rel = nullptr;
     uint32_t *bb = nullptr;
@@ -807,7 +824,8 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
     w.subj = reinterpret_cast<uint8_t *>(w.defer + kDeferCap) + kSeqPad;  // kSeqPad zero letters on both sides
     w.grp = reinterpret_cast<Group *>(w.subj + kSubjBuf + kSeqPad);
 
-    for (int i = threadIdx.x; i < 28 * 28; i += blockDim.x) mat[i / 28][i % 28] = kBlosum62Dev[i / 28][i % 28];
+    for (int i = threadIdx.x; i < 28 * kMatStride; i += blockDim.x)
+        mat[i / kMatStride][i % kMatStride] = i % kMatStride < 28 ? kBlosum62Dev[i / kMatStride][i % kMatStride] : 0;
     if (lane < kSeqPad) w.subj[lane - kSeqPad] = 0;
     for (uint32_t i = lane; i < C; i += 64) w.claim[i] = 0xFFFFFFFFu;
     if (lane < 4) w.cnt[lane] = 0;
@@ -827,6 +845,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
     Ctx c;
     c.mat = mat;
     c.Q = Qs;
+    c.Qx = Qx;
     c.stage = a.stage + gwave * (uint64_t)a.stage_cap;
     c.stage_cap = a.stage_cap;
     uint32_t n_hits = 0, n_ext = 0;  // this lane's word hits and extensions
@@ -851,8 +870,11 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
         const int64_t sub1 = a.slice[j + 1];
         const QueryDev &qd = a.qd[qi];
         if (qi != cur_q) {  // stage this query
-            for (int i = (int)threadIdx.x - kSeqPad; i < qd.qlen + kSeqPad; i += blockDim.x)
-                Qs[i] = i >= 0 && i < qd.qlen ? qd.query[i] : 0;
+            for (int i = (int)threadIdx.x - kSeqPad; i < qd.qlen + kSeqPad; i += blockDim.x) {
+                const uint8_t l = i >= 0 && i < qd.qlen ? qd.query[i] : 0;
+                Qs[i] = l;
+                Qx[i] = (uint16_t)(l * (2 * kMatStride));
+            }
             if (kTable >= 1) {
                 const uint32_t *src = reinterpret_cast<const uint32_t *>(qd.entries);
                 uint32_t *dst = reinterpret_cast<uint32_t *>(ent_l);
@@ -932,7 +954,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
 #pragma unroll
                     for (int kk = 0; kk < 16; kk++) {
                         const int p = v * 16 + kk - lead;
-                        if (p >= 0 && p < slen) dst[p] = (uint8_t)(xw[kk >> 2] >> (8 * (kk & 3)));
+                        if (p >= 0 && p < slen) dst[p] = (uint8_t)((xw[kk >> 2] >> (8 * (kk & 3))) << 1);  // doubled
                     }
                 }
                 const uint32_t next = (used + (uint32_t)slen + 4u + 63u) & ~63u;
@@ -1016,8 +1038,8 @@ __global__ void __launch_bounds__(256) k_seed_retry(SeedArgs a) {
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;
     const uint32_t C = a.claim_mask + 1;
-    int16_t(*mat)[28] = reinterpret_cast<int16_t(*)[28]>(smem);
-    uint8_t *wb = smem + al16(28 * 28 * 2) + (size_t)wave * wave_lds_bytes(8, C);
+    int16_t(*mat)[kMatStride] = reinterpret_cast<int16_t(*)[kMatStride]>(smem);
+    uint8_t *wb = smem + al16(kMatBytes) + (size_t)wave * wave_lds_bytes(8, C);
     WaveState<uint32_t> w;
     const uint64_t gwave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wave;
     w.ring = a.long_ring + gwave * a.ring_len;
@@ -1030,12 +1052,14 @@ __global__ void __launch_bounds__(256) k_seed_retry(SeedArgs a) {
     w.defer = reinterpret_cast<uint32_t *>(w.task + kTaskCap);
     w.subj = reinterpret_cast<uint8_t *>(w.defer + kDeferCap);
     w.grp = nullptr;
-    for (int i = threadIdx.x; i < 28 * 28; i += blockDim.x) mat[i / 28][i % 28] = kBlosum62Dev[i / 28][i % 28];
+    for (int i = threadIdx.x; i < 28 * kMatStride; i += blockDim.x)
+        mat[i / kMatStride][i % kMatStride] = i % kMatStride < 28 ? kBlosum62Dev[i / kMatStride][i % kMatStride] : 0;
     for (uint32_t i = lane; i < C; i += 64) w.claim[i] = 0xFFFFFFFFu;
     if (lane < 4) w.cnt[lane] = 0;
     __syncthreads();
     Ctx c;
     c.mat = mat;
+    c.Qx = nullptr;  // subjects from global memory only: the walks read c.Q
     c.stage = a.stage + gwave * (uint64_t)a.stage_cap;
     c.stage_cap = a.stage_cap;
     c.claim_mask = a.claim_mask;
@@ -1080,7 +1104,7 @@ __global__ void __launch_bounds__(256) k_seed_retry(SeedArgs a) {
 }
 
 size_t seed_retry_lds_bytes(const SeedArgs &a, int waves) {
-    return al16(28 * 28 * 2) + (size_t)waves * wave_lds_bytes(8, a.claim_mask + 1);
+    return al16(kMatBytes) + (size_t)waves * wave_lds_bytes(8, a.claim_mask + 1);
 }
 
 hipError_t launch_seed_retry(const SeedArgs &a, int64_t *retry, int blocks, int waves, hipStream_t st) {
