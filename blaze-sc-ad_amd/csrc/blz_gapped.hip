@@ -510,9 +510,13 @@ __device__ __forceinline__ void record_overflow(const GappedArgs &a, uint32_t wi
 
 }  // namespace
 
-// Work list in cost order: histogram counts per class come from the seed stage.
+// Work list in cost order: histogram counts per class come from the seed stage.  A workgroup
+// orders a contiguous range of items: it counts them per class in LDS, reserves each class's
+// share with one global atomic, then scatters.  (With one global atomic per class and wave the
+// eight counters' contention cost 4.6 ms at 64 queries x 500,000 subjects.)
 __global__ void __launch_bounds__(256) k_order(const WorkItem *work, DevCounters *ctr, uint32_t *order,
                                                uint32_t work_cap) {
+    __shared__ uint32_t cnt[kBuckets], start[kBuckets];
     const uint32_t n = ctr->work_count < work_cap ? ctr->work_count : work_cap;
     uint32_t base[kBuckets];
     uint32_t acc = 0;
@@ -522,18 +526,39 @@ __global__ void __launch_bounds__(256) k_order(const WorkItem *work, DevCounters
     }
     const int lane = lane_id();
     const uint64_t below = (1ull << lane) - 1;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int b = work[i].bucket;
-        for (int k = 0; k < kBuckets; k++) {  // one counter update per class and wave
-            const uint64_t m = __ballot(b == k);
-            if (!m) continue;
-            const int leader = __ffsll((long long)m) - 1;
-            uint32_t at = 0;
-            if (lane == leader) at = atomicAdd(&ctr->bucket_cursor[k], (uint32_t)__popcll(m));
-            at = (uint32_t)__shfl((int)at, leader, 64);
-            if (b == k) order[base[k] + at + (uint32_t)__popcll(m & below)] = i;
+    const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
+    const uint32_t r0 = min(n, blockIdx.x * per), r1 = min(n, r0 + per);
+    if (threadIdx.x < kBuckets) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    // one LDS counter update per class present in a wave, the first waiting lane's class first;
+    // the second pass places each item at its class's start plus its rank
+    auto pass = [&](bool place) {
+        for (uint32_t i0 = r0; i0 < r1; i0 += blockDim.x) {
+            const uint32_t i = i0 + threadIdx.x;
+            const bool has = i < r1;
+            const int b = has ? work[i].bucket : -1;
+            uint64_t left = __ballot(has);
+            while (left) {
+                const int k = __shfl(b, __ffsll((long long)left) - 1, 64);
+                const uint64_t m = __ballot(b == k);
+                const int leader = __ffsll((long long)m) - 1;
+                uint32_t at = 0;
+                if (lane == leader) at = atomicAdd(&cnt[k], (uint32_t)__popcll(m));
+                at = (uint32_t)__shfl((int)at, leader, 64);
+                if (place && b == k) order[base[k] + start[k] + at + (uint32_t)__popcll(m & below)] = i;
+                left &= ~m;
+            }
         }
+    };
+    pass(false);
+    __syncthreads();
+    if (threadIdx.x < kBuckets) {
+        const uint32_t c = cnt[threadIdx.x];
+        start[threadIdx.x] = c ? atomicAdd(&ctr->bucket_cursor[threadIdx.x], c) : 0u;
+        cnt[threadIdx.x] = 0;
     }
+    __syncthreads();
+    pass(true);
 }
 
 __global__ void __launch_bounds__(64) k_gapped_lds(GappedArgs a) {
