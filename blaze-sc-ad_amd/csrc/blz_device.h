@@ -73,8 +73,8 @@ struct SeedArgs {
     uint32_t *item_ctr;        // this launch's work-item counter
     uint32_t *long_ring;       // long-subject launch: diagonal rings, ring_len words per wave
     const int64_t *long_ids;   // long-subject launch: the subjects longer than kSeedNarrowMax
-    const int64_t *slice;      // subject slices of about equal residues: [slice[j], slice[j+1])
-    uint32_t n_slices;         // work items = packs x slices, pack-major, last slice first
+    const int64_t *items;      // work items, three values each: pack, first subject, end subject
+    uint32_t n_items;
     // retry launch: the (query << kOvfShift | subject) pairs the main launches could not stage,
     // rescanned with qd = the run's per-query table and stage_cap records per wave
     const int64_t *retry;

@@ -708,28 +708,47 @@ struct SeedPlan {
     int waves = 0;
     int blocks = 0;
     int first_pack = 0;     // index of the launch's first pack in the query table
-    size_t slice_off = 0;   // its slice bounds in the run's slice table
+    size_t item_off = 0;    // its work items in the run's item table (three values each)
     bool long_subjects = false;  // the launch over the database's long subjects
 };
 
-// Cuts subjects [0, nsubj) into at most n contiguous slices of about equal residues (at least
-// one subject each); appends the bounds (count + 1 values) to out.
-void slice_db(const blz_db *db, int64_t n, std::vector<int64_t> &out) {
+// Work items of a seed launch over the packs of `qlens` (letters per pack): (pack, subject range),
+// pack-major, each pack's ranges from the database's end to its start (a length-sorted database
+// ends with its longest subjects, which go first).  Guided sizes: a range holds about 1/(2 blocks)
+// of the work not yet handed out (work = residues x (pack letters + 200)), and at least
+// R / (64 blocks) residues, so the launch starts on large items and drains on small ones (equal
+// slices of eight per workgroup left the last workgroups 1.2 ms behind the median at the scan
+// workload).  Appends three values per item to out.
+void seed_items(const blz_db *db, const std::vector<int> &qlens, int blocks, std::vector<int64_t> &out) {
     const int64_t ns = db->nsubj;
-    n = std::max<int64_t>(1, std::min<int64_t>(n, ns));
     const uint64_t R = db->nres;
-    out.push_back(0);
-    int64_t prev = 0;
-    for (int64_t k = 1; k < n; k++) {
-        const uint64_t target = (uint64_t)((unsigned __int128)R * (uint64_t)k / (uint64_t)n);
-        int64_t i = (int64_t)(std::lower_bound(db->h_offs.begin(), db->h_offs.begin() + ns, target) -
-                              db->h_offs.begin());
-        if (i > prev && i < ns) {
-            out.push_back(i);
-            prev = i;
+    if (ns == 0) return;
+    const double share = 1.0 / (2.0 * std::max(1, blocks));
+    const uint64_t min_res = std::max<uint64_t>(4096, R / (64 * (uint64_t)std::max(1, blocks)));
+    double later = 0;  // work of the packs after the current one
+    for (int l : qlens) later += (double)R * (l + 200);
+    for (size_t p = 0; p < qlens.size(); p++) {
+        const double w = qlens[p] + 200.0;
+        later -= (double)R * w;
+        int64_t hi = ns;
+        while (hi > 0) {
+            const uint64_t left = db->h_offs[hi];  // residues of this pack not yet handed out
+            const uint64_t size = std::max<uint64_t>(min_res, (uint64_t)(((double)left * w + later) * share / w));
+            const uint64_t target = left > size ? left - size : 0;
+            // first subject at or past target, searched backwards from hi (galloping: the range
+            // is near hi, and a plain binary search over the whole offset table costs 0.5 ms)
+            int64_t step = 1;
+            while (step < hi && db->h_offs[hi - step] >= target) step *= 2;
+            const int64_t from = step < hi ? hi - step : 0;
+            int64_t lo = (int64_t)(std::lower_bound(db->h_offs.begin() + from, db->h_offs.begin() + hi, target) -
+                                   db->h_offs.begin());
+            if (lo >= hi) lo = hi - 1;  // at least one subject
+            out.push_back((int64_t)p);
+            out.push_back(lo);
+            out.push_back(hi);
+            hi = lo;
         }
     }
-    out.push_back(ns);
 }
 
 bool plan_seed2(blz_ctx *ctx, const std::vector<QueryDev> &qs, const std::vector<int> &idx, int table,
@@ -994,27 +1013,33 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
             qs.push_back(d);
         }
     }
-    std::vector<int64_t> slices;
+    std::vector<int64_t> slices;  // the run's work items, three values each
     size_t long_ring = 0;
     for (auto &pl : plans) {
         max_waves = std::max(max_waves, pl.waves);
         max_blocks = std::max<size_t>(max_blocks, (size_t)pl.blocks * pl.waves);
         if (variant == 1) continue;
-        pl.slice_off = slices.size();
-        if (pl.long_subjects) {  // one subject per wave and item
+        pl.item_off = slices.size();
+        if (pl.long_subjects) {  // one subject per wave and item, the longest (last) first
             long_ring = std::max(long_ring, (size_t)pl.blocks * pl.waves * pl.sa.ring_len);
-            for (int64_t k = 0; k < db->n_long; k += pl.waves) slices.push_back(k);
-            slices.push_back(db->n_long);
-        } else {  // about eight items per workgroup
-            slice_db(db, ((int64_t)pl.blocks * 8 + pl.sa.nq - 1) / pl.sa.nq, slices);
+            for (int p = 0; p < pl.sa.nq; p++)
+                for (int64_t k = ((db->n_long - 1) / pl.waves) * pl.waves; k >= 0; k -= pl.waves) {
+                    slices.push_back(p);
+                    slices.push_back(k);
+                    slices.push_back(std::min<int64_t>(k + pl.waves, db->n_long));
+                }
+        } else {
+            std::vector<int> qlens;
+            for (int p = 0; p < pl.sa.nq; p++) qlens.push_back(qs[pl.first_pack + p].qlen);
+            seed_items(db, qlens, pl.blocks, slices);
         }
-        pl.sa.n_slices = (uint32_t)(slices.size() - pl.slice_off - 1);
+        pl.sa.n_items = (uint32_t)((slices.size() - pl.item_off) / 3);
         if (std::getenv("BLZ_LOG_PLAN"))  // launch shapes, for tuning by hand
             std::fprintf(stderr,
                          "[blz] seed launch: table %d%s, %d packs, qlen <= %d, ring %u, entries <= %u, %d waves x %d "
-                         "blocks, %u slices, lds %zu\n",
+                         "blocks, %u items, lds %zu\n",
                          pl.table, pl.long_subjects ? " (long subjects)" : "", pl.sa.nq, pl.sa.qlen, pl.sa.ring_len,
-                         pl.sa.n_entries, pl.waves, pl.blocks, pl.sa.n_slices, seed2_lds_bytes(pl.sa, pl.table, pl.waves));
+                         pl.sa.n_entries, pl.waves, pl.blocks, pl.sa.n_items, seed2_lds_bytes(pl.sa, pl.table, pl.waves));
     }
     const size_t lds_lanes = (size_t)(ctx->tune.gapped_blocks ? ctx->tune.gapped_blocks : ctx->num_cus * 4) * 64;
     const bool seg16 = ctx->tune.gapped_lanes != 64;  // four subjects per wave
@@ -1086,7 +1111,7 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
             sa.qd = ctx->qdev + pl.first_pack;
             sa.s_begin = 0;
             sa.s_end = db->nsubj;
-            sa.slice = ctx->slices + pl.slice_off;
+            sa.items = ctx->slices + pl.item_off;
             sa.long_ring = ctx->long_ring;
             sa.long_ids = db->long_ids;
             sa.item_ctr = ctx->item_ctr + pi;

@@ -850,7 +850,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
     c.stage_cap = a.stage_cap;
     uint32_t n_hits = 0, n_ext = 0;  // this lane's word hits and extensions
     int cur_q = -1;
-    const uint32_t n_items = (uint32_t)a.nq * a.n_slices;
+    const uint32_t n_items = a.n_items;
     Tables t{};
 
     for (;;) {
@@ -862,12 +862,9 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
         __syncthreads();
         const uint32_t item = first_lane((uint32_t)blk[0]);  // uniform: the item's values live in scalars
         if (item >= n_items) break;
-        const int qi = (int)(item / a.n_slices);
-        // a length-sorted database ends with its longest subjects: take those first, so the
-        // launch drains on short ones
-        const uint32_t j = a.n_slices - 1u - item % a.n_slices;
-        const int64_t sub0 = a.slice[j];
-        const int64_t sub1 = a.slice[j + 1];
+        const int qi = (int)a.items[3 * (uint64_t)item];
+        const int64_t sub0 = a.items[3 * (uint64_t)item + 1];
+        const int64_t sub1 = a.items[3 * (uint64_t)item + 2];
         const QueryDev &qd = a.qd[qi];
         if (qi != cur_q) {  // stage this query
             for (int i = (int)threadIdx.x - kSeqPad; i < qd.qlen + kSeqPad; i += blockDim.x) {
