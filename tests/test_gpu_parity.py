@@ -84,6 +84,24 @@ def test_query_lengths(engine, qlen):
     compare(engine, q, res, offs)
 
 
+@pytest.mark.parametrize("case", range(6))
+def test_random_workloads(engine, case):
+    """Workloads drawn at random: query length log-uniform 20-2,500 aa, 60,000 subjects of a
+    drawn length distribution (sorted or not), homolog fraction, identity range, indels and
+    repeat blocks; every list and counter matches the oracle."""
+    rng = np.random.default_rng(4100 + case)
+    qlen = int(np.exp(rng.uniform(np.log(20), np.log(2500))))
+    ident_min = float(rng.uniform(0.3, 0.7))
+    kw = dict(homolog_frac=float(rng.choice([0.0, 0.01, 0.05, 0.2])), ident_min=ident_min,
+              ident_max=min(0.95, ident_min + float(rng.uniform(0.0, 0.3))),
+              len_median=float(rng.choice([60.0, 300.0, 900.0])), len_sigma=float(rng.uniform(0.3, 1.0)),
+              indel_rate=float(rng.choice([0.0, 0.02])), repeat_frac=float(rng.choice([0.0, 0.05])),
+              sort_by_length=int(rng.integers(0, 2)))
+    print(f"[case] random {case}: qlen {qlen} {kw}", flush=True)
+    q, res, offs, _ = make(4200 + case, qlen, 60000, **kw)
+    compare(engine, q, res, offs)
+
+
 def test_edge_subjects(engine):
     """Subjects of length 1, 2, 3, the query itself, a reversed query and empty-range neighbours."""
     q = blz.synth_query(3, 200)
