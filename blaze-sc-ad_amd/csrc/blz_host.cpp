@@ -714,7 +714,7 @@ struct SeedPlan {
 
 // Work items of a seed launch over the packs of `qlens` (letters per pack): (pack, subject range),
 // pack-major, each pack's ranges from the database's end to its start (a length-sorted database
-// ends with its longest subjects, which go first).  Guided sizes: a range holds about 1/(2 blocks)
+// ends with its longest subjects, which go first).  Guided sizes: a range holds about 1/(1.5 blocks)
 // of the work not yet handed out (work = residues x (pack letters + 200)), and at least
 // R / (64 blocks) residues, so the launch starts on large items and drains on small ones (equal
 // slices of eight per workgroup left the last workgroups 1.2 ms behind the median at the scan
@@ -723,7 +723,7 @@ void seed_items(const blz_db *db, const std::vector<int> &qlens, int blocks, std
     const int64_t ns = db->nsubj;
     const uint64_t R = db->nres;
     if (ns == 0) return;
-    const double share = 1.0 / (2.0 * std::max(1, blocks));
+    const double share = 1.0 / (1.5 * std::max(1, blocks));
     const uint64_t min_res = std::max<uint64_t>(4096, R / (64 * (uint64_t)std::max(1, blocks)));
     double later = 0;  // work of the packs after the current one
     for (int l : qlens) later += (double)R * (l + 200);
