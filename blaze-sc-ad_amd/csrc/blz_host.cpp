@@ -106,8 +106,8 @@ struct blz_ctx {
     hipStream_t up_stream = nullptr;  // database uploads: side stream and pinned staging
     uint8_t *up_buf[2] = {nullptr, nullptr};
     hipEvent_t up_ev[2] = {nullptr, nullptr};
-    int64_t *slices = nullptr;     // subject slice bounds of the run's seed launches
-    size_t slices_cap = 0;
+    int64_t *items = nullptr;      // work items of the run's seed launches (pack, first, end subject)
+    size_t items_cap = 0;
     uint32_t *long_ring = nullptr; // seed rings for subjects the LDS ring does not take
     size_t long_ring_cap = 0;
     int64_t *retry_list = nullptr; // seed pairs past a cap, rescanned by the retry launch
@@ -446,7 +446,7 @@ void blz_ctx_destroy(blz_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    void *bufs[] = {ctx->retry_list, ctx->retry_stage, ctx->retry_ring, ctx->long_ring, ctx->slices, ctx->ctr,   ctx->stage,   ctx->pool,    ctx->work, ctx->ovf,     ctx->dp,
+    void *bufs[] = {ctx->retry_list, ctx->retry_stage, ctx->retry_ring, ctx->long_ring, ctx->items, ctx->ctr,   ctx->stage,   ctx->pool,    ctx->work, ctx->ovf,     ctx->dp,
                     ctx->saved, ctx->hsps,    ctx->results, ctx->qdev, ctx->item_ctr,
                     ctx->order, ctx->redo, ctx->saved_wide, ctx->pack_meta};
     ctx->drop_graph();
@@ -877,7 +877,7 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
     return collect(ctx, nq, db, sinks, stats, n_launches);
 }
 
-// Plans the run (packs, bins, launch shapes, slices), uploads the run's tables and enqueues the
+// Plans the run (packs, bins, launch shapes, work items), uploads the run's tables and enqueues the
 // launch sequence; with use_graph the sequence is captured into ctx->graph first.
 blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq, const blz_db *db, int variant,
                            bool use_graph, const std::vector<uint64_t> &key, int &n_launches) {
@@ -1013,27 +1013,27 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
             qs.push_back(d);
         }
     }
-    std::vector<int64_t> slices;  // the run's work items, three values each
+    std::vector<int64_t> items;  // the run's work items, three values each
     size_t long_ring = 0;
     for (auto &pl : plans) {
         max_waves = std::max(max_waves, pl.waves);
         max_blocks = std::max<size_t>(max_blocks, (size_t)pl.blocks * pl.waves);
         if (variant == 1) continue;
-        pl.item_off = slices.size();
+        pl.item_off = items.size();
         if (pl.long_subjects) {  // one subject per wave and item, the longest (last) first
             long_ring = std::max(long_ring, (size_t)pl.blocks * pl.waves * pl.sa.ring_len);
             for (int p = 0; p < pl.sa.nq; p++)
                 for (int64_t k = ((db->n_long - 1) / pl.waves) * pl.waves; k >= 0; k -= pl.waves) {
-                    slices.push_back(p);
-                    slices.push_back(k);
-                    slices.push_back(std::min<int64_t>(k + pl.waves, db->n_long));
+                    items.push_back(p);
+                    items.push_back(k);
+                    items.push_back(std::min<int64_t>(k + pl.waves, db->n_long));
                 }
         } else {
             std::vector<int> qlens;
             for (int p = 0; p < pl.sa.nq; p++) qlens.push_back(qs[pl.first_pack + p].qlen);
-            seed_items(db, qlens, pl.blocks, slices);
+            seed_items(db, qlens, pl.blocks, items);
         }
-        pl.sa.n_items = (uint32_t)((slices.size() - pl.item_off) / 3);
+        pl.sa.n_items = (uint32_t)((items.size() - pl.item_off) / 3);
         if (std::getenv("BLZ_LOG_PLAN"))  // launch shapes, for tuning by hand
             std::fprintf(stderr,
                          "[blz] seed launch: table %d%s, %d packs, qlen <= %d, ring %u, entries <= %u, %d waves x %d "
@@ -1064,7 +1064,7 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
         !ensure(ctx, ctx->results, ctx->results_cap, work_cap, "results") ||
         !ensure(ctx, ctx->qdev, ctx->qdev_cap, qs.size(), "query table") ||
         !ensure(ctx, ctx->item_ctr, ctx->item_ctr_cap, plans.size() + 1, "item counters") ||
-        !ensure(ctx, ctx->slices, ctx->slices_cap, std::max<size_t>(slices.size(), 1), "slices") ||
+        !ensure(ctx, ctx->items, ctx->items_cap, std::max<size_t>(items.size(), 1), "work items") ||
         (long_ring && !ensure(ctx, ctx->long_ring, ctx->long_ring_cap, long_ring, "long-subject rings")) ||
         (variant != 1 && (!ensure(ctx, ctx->retry_list, ctx->retry_list_cap, 1 << 16, "retry list") ||
                           !ensure(ctx, ctx->retry_stage, ctx->retry_stage_cap,
@@ -1077,8 +1077,8 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
                "query table") ||
         (!meta.empty() && !check(ctx, hipMemcpyAsync(ctx->pack_meta, meta.data(), sizeof(uint32_t) * meta.size(),
                                                       hipMemcpyHostToDevice, st), "pack table")) ||
-        (!slices.empty() && !check(ctx, hipMemcpyAsync(ctx->slices, slices.data(), sizeof(int64_t) * slices.size(),
-                                                        hipMemcpyHostToDevice, st), "slices")))
+        (!items.empty() && !check(ctx, hipMemcpyAsync(ctx->items, items.data(), sizeof(int64_t) * items.size(),
+                                                       hipMemcpyHostToDevice, st), "work items")))
         return BLZ_EDEVICE;
 
     for (size_t pi = 0; pi < plans.size() && nsubj > 0; pi++) {
@@ -1111,7 +1111,7 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
             sa.qd = ctx->qdev + pl.first_pack;
             sa.s_begin = 0;
             sa.s_end = db->nsubj;
-            sa.items = ctx->slices + pl.item_off;
+            sa.items = ctx->items + pl.item_off;
             sa.long_ring = ctx->long_ring;
             sa.long_ids = db->long_ids;
             sa.item_ctr = ctx->item_ctr + pi;

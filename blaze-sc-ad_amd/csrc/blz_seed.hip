@@ -1,7 +1,7 @@
 // blz_seed.hip -- seed stage: word lookup, two-hit diagonal state, batched ungapped X-drop.
 //
-// Work items are (query pack, slice of subjects), taken pack-major from a device counter by
-// persistent workgroups; slices hold about equal residues.  A workgroup stages its item's query in LDS: letters, BLOSUM62 and,
+// Work items are (query pack, range of subjects), taken pack-major from a device counter by
+// persistent workgroups; ranges shrink towards the end of the item list (guided sizes).  A workgroup stages its item's query in LDS: letters, BLOSUM62 and,
 // depending on the table placement kTable, the lookup entries (u16 query offsets per word) and
 // the compact lookup prefix (u32 base per 64-word block + u16 offset per word):
 //   kTable 2: prefix and entries in LDS, 1: entries in LDS, prefix in global memory,
