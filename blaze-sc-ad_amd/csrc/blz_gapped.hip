@@ -169,20 +169,40 @@ template <int kStride>
 __device__ __forceinline__ int gapped_start(const int16_t (*mat)[kStride], const uint8_t *q, const uint8_t *s,
                                             int q_start, int s_start, int len) {
     if (len <= kHspMaxWindow) return q_start + len / 2;
+    // both sequences are in global memory: global loads, not flat ones
+    const __attribute__((address_space(1))) uint8_t *qa = (const __attribute__((address_space(1))) uint8_t *)(q + q_start);
+    const __attribute__((address_space(1))) uint8_t *sa = (const __attribute__((address_space(1))) uint8_t *)(s + s_start);
     int score = 0;
-    for (int i = 0; i < kHspMaxWindow; i++) score += mat[q[q_start + i]][s[s_start + i]];
+    for (int i = 0; i < kHspMaxWindow; i++) score += mat[qa[i]][sa[i]];
     int max_score = score;
-    int max_offset = q_start + kHspMaxWindow - 1;
-    for (int i = q_start + kHspMaxWindow; i < q_start + len; i++) {
-        const int k = i - q_start;
-        score -= mat[q[i - kHspMaxWindow]][s[s_start + k - kHspMaxWindow]];
-        score += mat[q[i]][s[s_start + k]];
-        if (score > max_score) {
-            max_score = score;
-            max_offset = i;
+    int max_k = kHspMaxWindow - 1;  // end of the best window, relative to the segment start
+    int k = kHspMaxWindow;
+    for (; k + 4 <= len; k += 4) {  // four window moves per pass: their sixteen letters load together
+        uint32_t qo[4], so[4], qn[4], sn[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            qo[j] = qa[k + j - kHspMaxWindow];
+            so[j] = sa[k + j - kHspMaxWindow];
+            qn[j] = qa[k + j];
+            sn[j] = sa[k + j];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            score += mat[qn[j]][sn[j]] - mat[qo[j]][so[j]];
+            if (score > max_score) {
+                max_score = score;
+                max_k = k + j;
+            }
         }
     }
-    return max_score > 0 ? max_offset - kHspMaxWindow / 2 : q_start;
+    for (; k < len; k++) {
+        score += mat[qa[k]][sa[k]] - mat[qa[k - kHspMaxWindow]][sa[k - kHspMaxWindow]];
+        if (score > max_score) {
+            max_score = score;
+            max_k = k;
+        }
+    }
+    return max_score > 0 ? q_start + max_k - kHspMaxWindow / 2 : q_start;
 }
 
 // purge orders and the final order (NCBI s_QueryOffsetCompareHSPs, s_QueryEndCompareHSPs,
