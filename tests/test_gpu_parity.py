@@ -102,6 +102,21 @@ def test_random_workloads(engine, case):
     compare(engine, q, res, offs)
 
 
+@pytest.mark.parametrize("case", range(6))
+def test_random_params(engine, case):
+    """Search parameters drawn at random (word threshold, two-hit window, both X-drops, gap costs,
+    both cutoffs) over a homolog-rich workload; every list and counter matches the oracle."""
+    rng = np.random.default_rng(4300 + case)
+    kw = dict(word_threshold=int(rng.integers(10, 14)), window=int(rng.integers(8, 120)),
+              x_ungapped=int(rng.integers(6, 48)), x_gapped=int(rng.integers(10, 80)), gap_open=int(rng.integers(3, 16)),
+              gap_extend=int(rng.integers(1, 4)), ungapped_cutoff=int(rng.integers(18, 70)),
+              gapped_cutoff=int(rng.integers(25, 90)))
+    print(f"[case] random params {case}: {kw}", flush=True)
+    q, res, offs, _ = make(4400 + case, int(rng.integers(100, 1500)), 30000, homolog_frac=0.05,
+                           ident_min=0.3, ident_max=0.8, indel_rate=0.02)
+    compare(engine, q, res, offs, blz.default_params(**kw), oracle.params(**kw))
+
+
 def test_edge_subjects(engine):
     """Subjects of length 1, 2, 3, the query itself, a reversed query and empty-range neighbours."""
     q = blz.synth_query(3, 200)
