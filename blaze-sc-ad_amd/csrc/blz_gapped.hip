@@ -281,11 +281,14 @@ struct Seg<16> {
     static __device__ __forceinline__ int shift_up1(int x, int in) {  // row_shr:1, the row's lane 0 gets in
         return __builtin_amdgcn_update_dpp(in, x, 0x111, 0xf, 0xf, false);
     }
-    // DPP row_share:n (gfx90a and later): every lane of the row reads the row's lane n
+    // DPP row_share:n (gfx90a and later): every lane of the row reads the row's lane n.  last()
+    // folds into a following max (v_max_i32_dpp); bcast() writes every lane and needs no old
+    // value (update_dpp's would cost a move per use).
     static __device__ __forceinline__ int last(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x15F, 0xf, 0xf, false); }
+    static __device__ __forceinline__ int bcast(int x) { return __builtin_amdgcn_mov_dpp(x, 0x15F, 0xf, 0xf, false); }
     static __device__ __forceinline__ uint64_t ballot(bool p) { return (__ballot(p) >> (lane_id() & 48)) & 0xFFFFull; }
     static __device__ __forceinline__ int first(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x150, 0xf, 0xf, false); }
-    static __device__ __forceinline__ int reduce_max(int x) { return last(incl_max(x)); }
+    static __device__ __forceinline__ int reduce_max(int x) { return bcast(incl_max(x)); }
 };
 
 // Wave-parallel semi-gapped X-drop DP for one extension (all 64 lanes call it with the same
@@ -894,7 +897,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
                     const bool alive = valid && max(din, carry_h) - h <= X;
                     fa = min(fa, alive ? b : 0x7fffffff);
                     la = alive ? b : la;
-                    const int cm = S::last(din);
+                    const int cm = S::bcast(din);
                     if (cm > carry_h) {  // a new best in this chunk: its first cell (H equals D/E there)
                         rmax = cm;
                         rmax_b = c0 + (int)__builtin_ctzll(S::ballot(de == cm));
@@ -903,7 +906,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
                         ring[b & wmask] = alive ? make_int2(h, max(h - goe, prev.y - ge)) : make_int2(kMinIntDp, kMinIntDp);
                     carry_g = max(carry_g, S::last(gin));
                     carry_h = max(carry_h, cm);
-                    carry_hp = S::last(prev.x);
+                    carry_hp = S::bcast(prev.x);
                 }
                 nc += (unsigned long long)(size - first);
                 la = S::reduce_max(la);
