@@ -74,8 +74,10 @@ def cpu_baseline(q, res, offs, got: blz.Result, budget_s: float):
     exact = sorted(want) == sorted(have) and all(np.array_equal(want[k], have[k]) for k in want)
     return {
         "value": float(sub_offs[-1] / dt), "unit": "residues/s", "cores": threads, "kind": "port",
-        "sample": f"every {stride}th subject of the same DB: {len(idx)} subjects, {int(sub_offs[-1])} residues, "
-                  f"{dt:.1f} s, CPU restatement of the NCBI prelim stage (oracle/) on {threads} threads",
+        "sample": (f"every {stride}th subject of the same DB" if stride > 1 else "the whole DB")
+                  + f": {len(idx)} subjects, {int(sub_offs[-1])} residues, "
+                  f"{dt:.1f} s on {threads} threads ({dt * threads:.0f} thread-seconds), CPU restatement of the NCBI "
+                  "prelim stage (oracle/)",
     }, exact, len(idx)
 
 
