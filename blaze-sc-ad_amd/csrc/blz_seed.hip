@@ -543,17 +543,19 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
         if (H <= (uint32_t)kHitBuf) {
             make_room(H);
             // append: the record's inline entries, then four independent lookup loads per pass
-            // (indices clamped to the word's last entry)
+            // (indices clamped to the word's last entry).  A store past the lane's nh entries goes
+            // to the lane's own word of w.excl (unused outside dense steps), so the stores are
+            // unconditional and need no branches.
+            uint32_t *const sink = w.excl + lane;
+            uint32_t *const dst = w.hits + tail + ex;
             uint32_t k0 = 0;
             if constexpr (kTable == 0) {
-                const uint32_t b = tail + ex;
                 const uint32_t su = (uint32_t)s;
-                if (nh > 0) w.hits[b + 0] = (r.y << 16) | su;
-                if (nh > 1) w.hits[b + 1] = (r.y & 0xFFFF0000u) | su;
-                if (nh > 2) w.hits[b + 2] = (r.z << 16) | su;
-                if (nh > 3) w.hits[b + 3] = (r.z & 0xFFFF0000u) | su;
-                if (nh > 4) w.hits[b + 4] = (r.w << 16) | su;
-                if (nh > 5) w.hits[b + 5] = (r.w & 0xFFFF0000u) | su;
+                const uint32_t v[kWordRecInline] = {(r.y << 16) | su, (r.y & 0xFFFF0000u) | su,
+                                                    (r.z << 16) | su, (r.z & 0xFFFF0000u) | su,
+                                                    (r.w << 16) | su, (r.w & 0xFFFF0000u) | su};
+#pragma unroll
+                for (uint32_t j = 0; j < (uint32_t)kWordRecInline; j++) *(j < nh ? dst + j : sink) = v[j];
                 k0 = kWordRecInline;
             }
             for (uint32_t k = k0; k < nh; k += 4) {
@@ -561,8 +563,8 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
 #pragma unroll
                 for (int j = 0; j < 4; j++) e[j] = entry<kTable>(t, pa + min(k + j, nh - 1));
 #pragma unroll
-                for (int j = 0; j < 4; j++)
-                    if (k + j < nh) w.hits[tail + ex + k + j] = (e[j] << 16) | (uint32_t)s;
+                for (uint32_t j = 0; j < 4; j++)
+                    if (k + j < nh) dst[k + j] = (e[j] << 16) | (uint32_t)s;
             }
             wave_sync();
             tail += H;
