@@ -784,7 +784,7 @@ template <int kTable, bool kLong>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8))) k_seed2(SeedArgs a) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int lane = lane_id();
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: per-wave LDS bases live in SGPRs
     const int nwaves = blockDim.x >> 6;
     // LDS is sized for the largest query of the launch (a.qlen, a.ring_len, a.claim_mask,
     // a.n_entries); each item uses its own query's sizes
@@ -1035,7 +1035,7 @@ __global__ void __launch_bounds__(256) k_seed_retry_prep(DevCounters *ctr, const
 __global__ void __launch_bounds__(256) k_seed_retry(SeedArgs a) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int lane = lane_id();
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: per-wave LDS bases live in SGPRs
     const uint32_t C = a.claim_mask + 1;
     int16_t(*mat)[kMatStride] = reinterpret_cast<int16_t(*)[kMatStride]>(smem);
     uint8_t *wb = smem + al16(kMatBytes) + (size_t)wave * wave_lds_bytes(8, C);
