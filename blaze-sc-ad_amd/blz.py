@@ -46,7 +46,9 @@ class RunStats(C.Structure):
                 ("gapped_subjects", C.c_uint64), ("gapped_ext", C.c_uint64),
                 ("dp_cells", C.c_uint64), ("hsps", C.c_uint64), ("overflow_subjects", C.c_uint64),
                 ("ms_seed", C.c_double), ("ms_gapped", C.c_double), ("ms_total", C.c_double),
-                ("seed_launches", C.c_int32), ("gapped_launches", C.c_int32)]
+                ("seed_launches", C.c_int32), ("gapped_launches", C.c_int32),
+                ("overflow_left", C.c_uint64), ("reruns", C.c_int32), ("drain_rounds", C.c_int32),
+                ("ms_drain", C.c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -57,7 +59,10 @@ class Tuning(C.Structure):
                 ("gapped_blocks", C.c_int32), ("gapped_threads", C.c_int32), ("seed_table", C.c_int32),
                 ("gapped_wave_classes", C.c_int32), ("query_packing", C.c_int32), ("pack_len", C.c_int32),
                 ("bin_ring", C.c_int32), ("graph", C.c_int32), ("gapped_lanes", C.c_int32),
-                ("subject_groups", C.c_int32)]
+                ("subject_groups", C.c_int32), ("overflow_mode", C.c_int32), ("cap_stage", C.c_int32),
+                ("cap_retry", C.c_int32), ("cap_ovf", C.c_int32), ("cap_wide_ring", C.c_int32),
+                ("cap_wide_saved", C.c_int32), ("cap_work", C.c_int64), ("cap_pool", C.c_int64),
+                ("cap_hsps", C.c_int64)]
 
 
 class ServerConfig(C.Structure):
@@ -336,7 +341,7 @@ class Engine:
         return Result(oids[: ns.value], counts[: ns.value], hsps[: nh.value])
 
     def overflow_pairs(self) -> list[tuple[int, int]]:
-        """(query index, subject) pairs the last run could not finish within the device caps."""
+        """(query index, subject) pairs the last run handed back (BLZ_EOVERFLOW)."""
         n = C.c_int64()
         self.lib.blz_overflow_drain(self.ctx, None, 0, C.byref(n))
         buf = np.empty(max(1, n.value), dtype=np.int64)

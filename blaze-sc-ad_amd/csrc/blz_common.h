@@ -59,8 +59,12 @@ struct ResultItem {
     int32_t qi;
 };
 
-// Overflow record: query index in the high bits, subject id in the low 40 bits.
+// Overflow record: query index in bits 40..61, subject id in the low 40 bits; bit 62 marks a
+// pair the gapped stage gave up on (its init hits were already counted by the seed stage).
 constexpr int kOvfShift = 40;
+constexpr unsigned long long kOvfGapped = 1ull << 62;
+constexpr long long kOvfQiMask = (1ll << 22) - 1;
+constexpr long long kOvfSidMask = (1ll << kOvfShift) - 1;
 
 // Device counters of one run (atomically accumulated by the kernels).
 struct DevCounters {
@@ -69,7 +73,8 @@ struct DevCounters {
     unsigned int work_count, result_count, overflow_count, next_subject, next_work, next_item;
     unsigned int bucket_count[kBuckets], bucket_cursor[kBuckets];
     unsigned int redo_count, next_redo, next_heavy, retry_count;
-    unsigned int next_retry, pad3;
+    unsigned int next_retry;
+    unsigned int ovf_lost;  // overflow records that did not fit the overflow list
 };
 
 }  // namespace blz

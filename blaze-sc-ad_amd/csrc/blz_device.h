@@ -113,6 +113,9 @@ hipError_t launch_seed2(const SeedArgs &a, int table, int blocks, int waves, boo
 // list to `retry` (and clears it), then rescans those pairs with a larger stage.
 hipError_t launch_seed_retry(const SeedArgs &a, int64_t *retry, int blocks, int waves, hipStream_t st);
 size_t seed_retry_lds_bytes(const SeedArgs &a, int waves);
+// The retry scan alone over a.retry[0, ctr->retry_count) (the overflow queue's pass; the caller
+// sets retry_count and clears the rest of a.ctr).
+hipError_t launch_seed_pairs(const SeedArgs &a, int blocks, int waves, hipStream_t st);
 // Resident workgroups per CU of that launch shape (registers and LDS), 0 if it cannot run.
 int seed2_blocks_per_cu(const SeedArgs &a, int table, int waves, bool long_subjects);
 hipError_t launch_seed(const SeedArgs &a, int blocks, int waves_per_block, hipStream_t st);

@@ -528,7 +528,7 @@ __device__ int finish_hsps(const GappedArgs &a, const WorkItem &wi, Hsp *saved, 
 __device__ __forceinline__ void record_overflow(const GappedArgs &a, uint32_t widx) {
     const WorkItem wi = a.work[widx];
     const uint32_t k = atomicAdd(&a.ctr->overflow_count, 1u);
-    if (k < a.ovf_cap) a.ovf[k] = ((int64_t)wi.qi << kOvfShift) | wi.sid;
+    if (k < a.ovf_cap) a.ovf[k] = (int64_t)kOvfGapped | ((int64_t)wi.qi << kOvfShift) | wi.sid;
 }
 
 }  // namespace

@@ -116,6 +116,36 @@ struct blz_ctx {
     size_t retry_stage_cap = 0;
     uint32_t *retry_ring = nullptr;
     size_t retry_ring_cap = 0;
+    // caps the kernels of the run just enqueued were given
+    size_t run_work_cap = 0, run_pool_cap = 0, run_hsp_cap = 0, run_ovf_cap = 0;
+    uint32_t run_ring_max = 8;  // longest diagonal ring of the run's queries (retry scans)
+    // sizes later runs start from once a run filled a work buffer (then run again)
+    size_t grow_work = 0, grow_pool = 0, grow_hsps = 0, grow_ovf = 0;
+    // overflow queue: its own counters and buffers, sized to the pairs it runs
+    DevCounters *q_ctr = nullptr;
+    size_t q_ctr_cap = 0;
+    int64_t *q_list = nullptr;
+    size_t q_list_cap = 0;
+    InitHit *q_stage = nullptr;
+    size_t q_stage_cap = 0;
+    uint32_t *q_ring = nullptr;
+    size_t q_ring_cap = 0;
+    InitHit *q_pool = nullptr;
+    size_t q_pool_cap = 0;
+    WorkItem *q_work = nullptr;
+    size_t q_work_cap = 0;
+    uint32_t *q_order = nullptr;
+    size_t q_order_cap = 0;
+    int64_t *q_ovf = nullptr;
+    size_t q_ovf_cap = 0;
+    int2 *q_dp = nullptr;
+    size_t q_dp_cap = 0;
+    Hsp *q_saved = nullptr;
+    size_t q_saved_cap = 0;
+    Hsp *q_hsps = nullptr;
+    size_t q_hsps_cap = 0;
+    ResultItem *q_results = nullptr;
+    size_t q_results_cap = 0;
     // packs of the last batch (keyed by query ids), reused while the same queries come again
     std::vector<uint64_t> pack_key;
     std::vector<Pack> packs;
@@ -178,9 +208,15 @@ constexpr int kSavedCapWave = 1024;  // wave gapped pass: HSPs per subject befor
 constexpr int kSeedWavesMax = 4;
 constexpr int kGappedBlocks = 64;        // wide gapped pass (redo list): blocks x 64 lanes
 constexpr size_t kRedoCap = 1 << 20;
-constexpr size_t kWorkCapMax = 48u << 20; // work items (subjects with init hits) per run
+constexpr size_t kWorkCapStart = 48u << 20; // work items a run starts with (grown when a run fills them)
 constexpr int kRetryCap = 1 << 16;        // retry launch: init hits per subject
 constexpr int kRetryBlocks = 16, kRetryWaves = 4;
+constexpr size_t kOvfStart = 1 << 16;     // overflow-list entries a run starts with
+// overflow queue: init hits staged per pair in its first round (x8 each further round), and the
+// most device memory one round's per-pair staging and per-lane DP rings may take
+constexpr size_t kQueueStage0 = 1 << 20;
+constexpr size_t kQueueBytes = 2ull << 30;
+constexpr int kQueueRounds = 5;
 
 bool check(blz_ctx *ctx, hipError_t e, const char *what) {
     if (e == hipSuccess) return true;
@@ -188,13 +224,16 @@ bool check(blz_ctx *ctx, hipError_t e, const char *what) {
     return false;
 }
 
+// Grows a device buffer to `need` elements (contents are not kept).  A buffer that a captured
+// launch sequence may name bumps the context's generation, which retires the captured graphs;
+// the overflow queue's own buffers (never captured) pass bump = false.
 template <class T>
-bool ensure(blz_ctx *ctx, T *&ptr, size_t &cap, size_t need, const char *what) {
+bool ensure(blz_ctx *ctx, T *&ptr, size_t &cap, size_t need, const char *what, bool bump = true) {
     if (need <= cap && ptr) return true;
     if (ptr) (void)hipFree(ptr);
     ptr = nullptr;
     cap = 0;
-    ctx->gen++;
+    if (bump) ctx->gen++;
     size_t n = std::max<size_t>(need, 1);
     if (!check(ctx, hipMalloc((void **)&ptr, n * sizeof(T)), what)) return false;
     cap = n;
@@ -446,9 +485,13 @@ void blz_ctx_destroy(blz_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    void *bufs[] = {ctx->retry_list, ctx->retry_stage, ctx->retry_ring, ctx->long_ring, ctx->items, ctx->ctr,   ctx->stage,   ctx->pool,    ctx->work, ctx->ovf,     ctx->dp,
-                    ctx->saved, ctx->hsps,    ctx->results, ctx->qdev, ctx->item_ctr,
-                    ctx->order, ctx->redo, ctx->saved_wide, ctx->pack_meta};
+    void *bufs[] = {ctx->retry_list, ctx->retry_stage, ctx->retry_ring, ctx->long_ring, ctx->items,
+                    ctx->ctr,        ctx->stage,       ctx->pool,       ctx->work,      ctx->ovf,
+                    ctx->dp,         ctx->saved,       ctx->hsps,       ctx->results,   ctx->qdev,
+                    ctx->item_ctr,   ctx->order,       ctx->redo,       ctx->saved_wide, ctx->pack_meta,
+                    ctx->q_ctr,      ctx->q_list,      ctx->q_stage,    ctx->q_ring,    ctx->q_pool,
+                    ctx->q_work,     ctx->q_order,     ctx->q_ovf,      ctx->q_dp,      ctx->q_saved,
+                    ctx->q_hsps,     ctx->q_results};
     ctx->drop_graph();
     for (auto &pk : ctx->packs) pk.t.release();
     for (void *b : bufs)
@@ -473,7 +516,9 @@ blz_status blz_ctx_set_tuning(blz_ctx *ctx, const blz_tuning *t) {
         t->gapped_wave_classes > kBuckets + 1 || t->query_packing < 0 || t->query_packing > 2 || t->pack_len < 0 ||
         t->pack_len > 16000 || t->bin_ring < 0 || t->graph < 0 || t->graph > 1 ||
         (t->gapped_lanes != 0 && t->gapped_lanes != 16 && t->gapped_lanes != 64) || t->subject_groups < 0 ||
-        t->subject_groups > 1)
+        t->subject_groups > 1 || t->overflow_mode < 0 || t->overflow_mode > 1 || t->cap_stage < 0 ||
+        t->cap_retry < 0 || t->cap_ovf < 0 || t->cap_wide_ring < 0 || t->cap_wide_saved < 0 || t->cap_work < 0 ||
+        t->cap_work > 0xFFFFFFFFll || t->cap_pool < 0 || t->cap_hsps < 0)
         return BLZ_EINVAL;
     if (t->query_packing != ctx->tune.query_packing || t->pack_len != ctx->tune.pack_len) {
         // cached packs were made under the old choice
@@ -841,8 +886,11 @@ blz_status replay_graph(blz_ctx *ctx) {
                : BLZ_EDEVICE;
 }
 
-blz_status collect(blz_ctx *ctx, int nq, const blz_db *db, blz_hsp_sink *sinks, blz_run_stats *stats,
-                   int n_launches);
+int collect(blz_ctx *ctx, int nq, const blz_db *db, blz_hsp_sink *sinks, blz_run_stats *stats, int n_launches,
+            int reruns);
+constexpr int kMaxReruns = 3;
+// Internal outcome of collect: a work buffer of the run filled up; the caller runs it again.
+constexpr int kRunAgain = -1;
 
 // Prelim stage of queries[0..nq) over the database: seed launches (one per bin), the gapped
 // stage, results copied into the sinks.  The launch sequence is captured as two hipGraphs (one
@@ -861,20 +909,31 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
         for (int i = 0; i < nq; i++) key.push_back(queries[i]->uid);
         key.push_back(db->uid);
     }
-    int n_launches = 0;
-    if (use_graph && ctx->graph_exec[0] && key == ctx->graph_key && ctx->graph_gen == ctx->gen &&
-        std::memcmp(&ctx->graph_tune, &ctx->tune, sizeof(blz_tuning)) == 0) {
-        // the same queries again: the tables on the device are the ones the graphs were captured with
-        n_launches = ctx->graph_launches;
-        const blz_status rc = replay_graph(ctx);
-        if (rc != BLZ_OK) return rc;
-    } else {
-        const double t0 = log_time() ? now_ms() : 0;
-        const blz_status rc = plan_and_launch(ctx, queries, nq, db, variant, use_graph, key, n_launches);
-        if (rc != BLZ_OK) return rc;
-        if (log_time()) std::fprintf(stderr, "[blz] plan + enqueue %.3f ms\n", now_ms() - t0);
+    for (int reruns = 0;; reruns++) {
+        int n_launches = 0;
+        if (use_graph && ctx->graph_exec[0] && key == ctx->graph_key && ctx->graph_gen == ctx->gen &&
+            std::memcmp(&ctx->graph_tune, &ctx->tune, sizeof(blz_tuning)) == 0) {
+            // the same queries again: the tables on the device are the ones the graphs were captured with
+            n_launches = ctx->graph_launches;
+            const blz_status rc = replay_graph(ctx);
+            if (rc != BLZ_OK) return rc;
+        } else {
+            const double t0 = log_time() ? now_ms() : 0;
+            const blz_status rc = plan_and_launch(ctx, queries, nq, db, variant, use_graph, key, n_launches);
+            if (rc != BLZ_OK) return rc;
+            if (log_time()) std::fprintf(stderr, "[blz] plan + enqueue %.3f ms\n", now_ms() - t0);
+        }
+        const int rc = collect(ctx, nq, db, sinks, stats, n_launches, reruns);
+        if (rc != kRunAgain) return (blz_status)rc;
+        // a work buffer filled up: its size is raised to the run's count; the captured sequence
+        // holds the old caps
+        ctx->drop_graph();
+        ctx->gen++;
+        if (reruns + 1 >= kMaxReruns) {
+            ctx->err = "work buffers still full after growing them";
+            return BLZ_ENOMEM;
+        }
     }
-    return collect(ctx, nq, db, sinks, stats, n_launches);
 }
 
 // Plans the run (packs, bins, launch shapes, work items), uploads the run's tables and enqueues the
@@ -894,11 +953,31 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
     std::vector<size_t> meta_off;
     int max_waves = 1;
     size_t max_blocks = 1;
+    // Every query is also a pack of one of its own: the retry scan and the overflow queue scan
+    // (query, subject) pairs with the query's own tables and emit through its member table.
+    // Appended after the packs' entries; the table is uploaded with them.
+    auto own_packs = [&]() -> bool {
+        std::vector<size_t> own_off(nq);
+        for (int i = 0; i < nq; i++) {
+            own_off[i] = meta.size();
+            meta.push_back(0);
+            meta.push_back((uint32_t)qs[i].qlen);
+            meta.push_back((uint32_t)i);
+        }
+        if (!ensure(ctx, ctx->pack_meta, ctx->pack_meta_cap, std::max<size_t>(meta.size(), 1), "pack table"))
+            return false;
+        for (int i = 0; i < nq; i++) {
+            qs[i].member_start = ctx->pack_meta + own_off[i];
+            qs[i].member_idx = qs[i].member_start + 2;
+        }
+        return true;
+    };
     if (variant == 1) {
         if (nq != 1) {
             ctx->err = "seed variant 1 runs one query at a time";
             return BLZ_EINVAL;
         }
+        if (!own_packs()) return BLZ_ENOMEM;
         SeedPlan pl;
         const uint32_t ring = qs[0].ring_mask + 1;
         int waves = ctx->tune.seed_waves ? std::min(ctx->tune.seed_waves, kSeedWavesMax) : kSeedWavesMax;
@@ -992,20 +1071,7 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
                 plans.push_back(lp);
             }
         }
-        // every query is also a pack of one of its own (the retry launch scans it alone)
-        std::vector<size_t> own_off(nq);
-        for (int i = 0; i < nq; i++) {
-            own_off[i] = meta.size();
-            meta.push_back(0);
-            meta.push_back((uint32_t)qs[i].qlen);
-            meta.push_back((uint32_t)i);
-        }
-        if (!ensure(ctx, ctx->pack_meta, ctx->pack_meta_cap, std::max<size_t>(meta.size(), 1), "pack table"))
-            return BLZ_ENOMEM;
-        for (int i = 0; i < nq; i++) {
-            qs[i].member_start = ctx->pack_meta + own_off[i];
-            qs[i].member_idx = qs[i].member_start + 2;
-        }
+        if (!own_packs()) return BLZ_ENOMEM;
         for (int k : order) {
             QueryDev d = pq[k];
             d.member_start = ctx->pack_meta + meta_off[k];
@@ -1048,30 +1114,50 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
         std::max(lds_lanes * kSavedCap, (size_t)heavy_blocks * (seg16 ? 4 : 1) * kSavedCapWave);
     const size_t wide_lanes = (size_t)kGappedBlocks * 64;
     const size_t per_q = std::max<size_t>(nsubj, 1);
-    const size_t work_cap = std::min(per_q * (size_t)nq, kWorkCapMax);
+    // the run's caps: built-in sizes (or a tuned limit), raised to what an earlier run of this
+    // context needed when it filled a buffer
+    const blz_tuning &tn = ctx->tune;
+    auto cap_of = [](int64_t limit, size_t builtin, size_t grown) {
+        return std::max<size_t>(limit > 0 ? (size_t)limit : builtin, grown);
+    };
+    const size_t work_cap = cap_of(tn.cap_work, std::min(per_q * (size_t)nq, kWorkCapStart), ctx->grow_work);
+    const size_t pool_cap = cap_of(tn.cap_pool, std::max<size_t>(1 << 22, 2 * work_cap), ctx->grow_pool);
+    const size_t hsp_cap = cap_of(tn.cap_hsps, std::max<size_t>(1 << 22, per_q), ctx->grow_hsps);
+    const size_t ovf_cap = cap_of(tn.cap_ovf, kOvfStart, ctx->grow_ovf);
+    const int stage_cap = tn.cap_stage > 0 ? std::min(tn.cap_stage, kStageCap) : kStageCap;
+    const int retry_cap = tn.cap_retry > 0 ? std::min(tn.cap_retry, kRetryCap) : kRetryCap;
+    int wide_ring = kDpRing;
+    if (tn.cap_wide_ring > 0)
+        while (wide_ring > 16 && wide_ring / 2 >= tn.cap_wide_ring) wide_ring /= 2;
+    const int wide_saved = tn.cap_wide_saved > 0 ? std::min(tn.cap_wide_saved, kSavedCapWide) : kSavedCapWide;
     uint32_t max_ring_len = 8;  // the retry launch's rings: any query of the run
     for (int i = 0; i < nq; i++) max_ring_len = std::max(max_ring_len, qs[i].ring_len);
-    if (!ensure(ctx, ctx->stage, ctx->stage_records, max_blocks * kStageCap, "stage") ||
-        !ensure(ctx, ctx->pool, ctx->pool_cap, std::max<size_t>(1 << 22, 2 * work_cap), "pool") ||
+    if (!ensure(ctx, ctx->stage, ctx->stage_records, max_blocks * stage_cap, "stage") ||
+        !ensure(ctx, ctx->pool, ctx->pool_cap, pool_cap, "pool") ||
         !ensure(ctx, ctx->work, ctx->work_cap, work_cap, "work") ||
         !ensure(ctx, ctx->order, ctx->order_cap, work_cap, "order") ||
         !ensure(ctx, ctx->redo, ctx->redo_cap, kRedoCap, "redo") ||
-        !ensure(ctx, ctx->ovf, ctx->ovf_cap, 1 << 16, "ovf") ||
-        !ensure(ctx, ctx->dp, ctx->dp_entries, wide_lanes * kDpRing, "dp") ||
+        !ensure(ctx, ctx->ovf, ctx->ovf_cap, ovf_cap, "ovf") ||
+        !ensure(ctx, ctx->dp, ctx->dp_entries, wide_lanes * wide_ring, "dp") ||
         !ensure(ctx, ctx->saved, ctx->saved_records, saved_records, "saved") ||
-        !ensure(ctx, ctx->saved_wide, ctx->saved_wide_records, wide_lanes * kSavedCapWide, "saved wide") ||
-        !ensure(ctx, ctx->hsps, ctx->hsp_cap, std::max<size_t>(1 << 22, per_q), "hsps") ||
+        !ensure(ctx, ctx->saved_wide, ctx->saved_wide_records, wide_lanes * wide_saved, "saved wide") ||
+        !ensure(ctx, ctx->hsps, ctx->hsp_cap, hsp_cap, "hsps") ||
         !ensure(ctx, ctx->results, ctx->results_cap, work_cap, "results") ||
         !ensure(ctx, ctx->qdev, ctx->qdev_cap, qs.size(), "query table") ||
         !ensure(ctx, ctx->item_ctr, ctx->item_ctr_cap, plans.size() + 1, "item counters") ||
         !ensure(ctx, ctx->items, ctx->items_cap, std::max<size_t>(items.size(), 1), "work items") ||
         (long_ring && !ensure(ctx, ctx->long_ring, ctx->long_ring_cap, long_ring, "long-subject rings")) ||
-        (variant != 1 && (!ensure(ctx, ctx->retry_list, ctx->retry_list_cap, 1 << 16, "retry list") ||
+        (variant != 1 && (!ensure(ctx, ctx->retry_list, ctx->retry_list_cap, ovf_cap, "retry list") ||
                           !ensure(ctx, ctx->retry_stage, ctx->retry_stage_cap,
-                                  (size_t)kRetryBlocks * kRetryWaves * kRetryCap, "retry stage") ||
+                                  (size_t)kRetryBlocks * kRetryWaves * retry_cap, "retry stage") ||
                           !ensure(ctx, ctx->retry_ring, ctx->retry_ring_cap,
                                   (size_t)kRetryBlocks * kRetryWaves * max_ring_len, "retry rings"))))
         return BLZ_ENOMEM;
+    ctx->run_work_cap = work_cap;
+    ctx->run_pool_cap = pool_cap;
+    ctx->run_hsp_cap = hsp_cap;
+    ctx->run_ovf_cap = ovf_cap;
+    ctx->run_ring_max = max_ring_len;
 
     if (!check(ctx, hipMemcpyAsync(ctx->qdev, qs.data(), sizeof(QueryDev) * qs.size(), hipMemcpyHostToDevice, st),
                "query table") ||
@@ -1087,13 +1173,13 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
         sa.res = db->res;
         sa.offs = db->offs;
         sa.stage = ctx->stage;
-        sa.stage_cap = kStageCap;
+        sa.stage_cap = stage_cap;
         sa.pool = ctx->pool;
-        sa.pool_cap = ctx->pool_cap;
+        sa.pool_cap = pool_cap;
         sa.work = ctx->work;
-        sa.work_cap = (uint32_t)ctx->work_cap;
+        sa.work_cap = (uint32_t)work_cap;
         sa.ovf = ctx->ovf;
-        sa.ovf_cap = (uint32_t)ctx->ovf_cap;
+        sa.ovf_cap = (uint32_t)ovf_cap;
         sa.ctr = ctr;
         if (variant == 1) {
             const QueryDev &d = qs[0];
@@ -1125,21 +1211,21 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
     ga.work = ctx->work;
     ga.order = ctx->order;
     ga.count = &ctr->work_count;
-    ga.count_cap = (uint32_t)ctx->work_cap;
+    ga.count_cap = (uint32_t)work_cap;
     ga.next = &ctr->next_work;
     ga.next_heavy = &ctr->next_heavy;
     ga.heavy_classes = ctx->tune.gapped_wave_classes > 0 ? ctx->tune.gapped_wave_classes - 1 : kBuckets;
     ga.redo = ctx->redo;
     ga.redo_cap = (uint32_t)ctx->redo_cap;
     ga.dp_scratch = ctx->dp;
-    ga.dp_wmask = kDpRing - 1;
+    ga.dp_wmask = wide_ring - 1;
     ga.hsp_scratch = ctx->saved;
     ga.saved_cap = kSavedCap;
     ga.hsps = ctx->hsps;
-    ga.hsp_cap = ctx->hsp_cap;
+    ga.hsp_cap = hsp_cap;
     ga.results = ctx->results;
     ga.ovf = ctx->ovf;
-    ga.ovf_cap = (uint32_t)ctx->ovf_cap;
+    ga.ovf_cap = (uint32_t)ovf_cap;
     ga.ctr = ctr;
     GappedArgs gw = ga;  // wide pass over the redo list
     gw.order = ctx->redo;
@@ -1150,7 +1236,7 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
     gw.redo = nullptr;
     gw.redo_cap = 0;
     gw.hsp_scratch = ctx->saved_wide;
-    gw.saved_cap = kSavedCapWide;
+    gw.saved_cap = wide_saved;
     const int lds_blocks = ctx->tune.gapped_blocks ? ctx->tune.gapped_blocks : ctx->num_cus * 4;
     GappedArgs gh = ga;  // wave pass: per-block saved list of kSavedCapWave
     gh.saved_cap = kSavedCapWave;
@@ -1174,13 +1260,13 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
             r.claim_mask = 127;
             r.ring_len = max_ring_len;
             r.stage = ctx->retry_stage;
-            r.stage_cap = kRetryCap;
+            r.stage_cap = retry_cap;
             r.pool = ctx->pool;
-            r.pool_cap = ctx->pool_cap;
+            r.pool_cap = pool_cap;
             r.work = ctx->work;
-            r.work_cap = (uint32_t)ctx->work_cap;
+            r.work_cap = (uint32_t)work_cap;
             r.ovf = ctx->ovf;
-            r.ovf_cap = (uint32_t)ctx->ovf_cap;
+            r.ovf_cap = (uint32_t)ovf_cap;
             r.ctr = ctr;
             r.long_ring = ctx->retry_ring;
             if (!check(ctx, launch_seed_retry(r, ctx->retry_list, kRetryBlocks, kRetryWaves, st), "retry launch"))
@@ -1189,7 +1275,7 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
         return true;
     };
     auto enqueue_gapped = [&]() -> bool {
-        return (nsubj == 0 || check(ctx, launch_order(ctx->work, ctr, ctx->order, (uint32_t)ctx->work_cap, 512, st),
+        return (nsubj == 0 || check(ctx, launch_order(ctx->work, ctr, ctx->order, (uint32_t)work_cap, 512, st),
                                     "k_order launch")) &&
                (nsubj == 0 || ga.heavy_classes == 0 ||
                 check(ctx, seg16 ? launch_gapped_seg(gh, heavy_blocks, st) : launch_gapped_wave(gh, heavy_blocks, st),
@@ -1229,10 +1315,166 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
     return replay_graph(ctx);
 }
 
+// Result of the overflow queue: lists of the pairs it finished, its counters, what is left.
+struct QueueOut {
+    std::vector<ResultItem> res;  // HSP offsets into hs
+    std::vector<Hsp> hs;
+    unsigned long long init_hits = 0, gapped_ext = 0, dp_cells = 0, work_items = 0;
+    std::vector<int64_t> left;    // pairs it could not size buffers for
+    int rounds = 0;
+};
+
+// Overflow queue.  Pairs past a per-subject cap of the main passes (more init hits than the
+// retry scan stages, a DP band wider than the wide pass's ring, more saved HSPs than it holds)
+// are run again from their letters with buffers sized to them: the retry scan with a stage of
+// kQueueStage0 init hits per pair (x8 per further round for the pairs that still do not fit),
+// then the wide gapped pass with a DP ring as wide as the longest of their subjects (a band
+// never has more than subject length + 1 cells) and room for every init hit as a saved HSP.
+// BLAZE flags such subjects for re-evaluation on the CPU (README.md:36-37; flag 255 at
+// gpuBLAZE.cu:1088,1243-1247); here they stay on the device.  The run's query table (ctx->qdev)
+// is still in place, so a pair's query index selects its tables.
+blz_status run_overflow_queue(blz_ctx *ctx, const blz_db *db, std::vector<int64_t> pairs, QueueOut &out) {
+    hipStream_t st = ctx->stream;
+    size_t stage_cap = kQueueStage0;
+    size_t pool_cap = kQueueStage0;
+    if (!ensure(ctx, ctx->q_ctr, ctx->q_ctr_cap, 1, "queue counters", false)) return BLZ_ENOMEM;
+    for (int round = 0; round < kQueueRounds && !pairs.empty(); round++) {
+        out.rounds++;
+        const size_t n = pairs.size();
+        // one wave per pair at a time, as many waves as the staging budget allows
+        size_t waves = std::min<size_t>({n, (size_t)256, std::max<size_t>(1, kQueueBytes / (stage_cap * sizeof(InitHit)))});
+        const int wpb = (int)std::min<size_t>(waves, 4);
+        const int blocks = (int)((waves + wpb - 1) / wpb);
+        waves = (size_t)blocks * wpb;
+        if (!ensure(ctx, ctx->q_list, ctx->q_list_cap, n, "queue list", false) ||
+            !ensure(ctx, ctx->q_stage, ctx->q_stage_cap, waves * stage_cap, "queue stage", false) ||
+            !ensure(ctx, ctx->q_ring, ctx->q_ring_cap, waves * ctx->run_ring_max, "queue rings", false) ||
+            !ensure(ctx, ctx->q_pool, ctx->q_pool_cap, pool_cap, "queue pool", false) ||
+            !ensure(ctx, ctx->q_work, ctx->q_work_cap, n, "queue work", false) ||
+            !ensure(ctx, ctx->q_order, ctx->q_order_cap, n, "queue order", false) ||
+            !ensure(ctx, ctx->q_ovf, ctx->q_ovf_cap, n, "queue overflow", false) ||
+            !ensure(ctx, ctx->q_results, ctx->q_results_cap, n, "queue results", false))
+            return BLZ_ENOMEM;
+        DevCounters c0{};
+        c0.retry_count = (unsigned)n;
+        if (!check(ctx, hipMemcpyAsync(ctx->q_list, pairs.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, st),
+                   "queue list") ||
+            !check(ctx, hipMemcpyAsync(ctx->q_ctr, &c0, sizeof(c0), hipMemcpyHostToDevice, st), "queue counters"))
+            return BLZ_EDEVICE;
+        SeedArgs r{};
+        r.res = db->res;
+        r.offs = db->offs;
+        r.qd = ctx->qdev;
+        r.claim_mask = 127;
+        r.ring_len = ctx->run_ring_max;
+        r.stage = ctx->q_stage;
+        r.stage_cap = (int32_t)stage_cap;
+        r.pool = ctx->q_pool;
+        r.pool_cap = pool_cap;
+        r.work = ctx->q_work;
+        r.work_cap = (uint32_t)n;
+        r.ovf = ctx->q_ovf;
+        r.ovf_cap = (uint32_t)n;
+        r.ctr = ctx->q_ctr;
+        r.long_ring = ctx->q_ring;
+        r.retry = ctx->q_list;
+        DevCounters c{};
+        if (!check(ctx, launch_seed_pairs(r, blocks, wpb, st), "queue scan") ||
+            !check(ctx, launch_order(ctx->q_work, ctx->q_ctr, ctx->q_order, (uint32_t)n, 64, st), "queue order") ||
+            !check(ctx, hipMemcpyAsync(&c, ctx->q_ctr, sizeof(c), hipMemcpyDeviceToHost, st), "queue counters") ||
+            !check(ctx, hipStreamSynchronize(st), "queue sync"))
+            return BLZ_EDEVICE;
+        if (c.pool_used > pool_cap) {  // the round's init hits did not fit: the same round again
+            pool_cap = (size_t)c.pool_used + c.pool_used / 4;
+            round--;
+            if (out.rounds > 2 * kQueueRounds) break;
+            continue;
+        }
+        const size_t nw = c.work_count;
+        // sizes of the gapped pass: the longest subject's band, every init hit saved
+        std::vector<WorkItem> wk(nw);
+        if (nw && !check(ctx, hipMemcpy(wk.data(), ctx->q_work, sizeof(WorkItem) * nw, hipMemcpyDeviceToHost),
+                         "queue work"))
+            return BLZ_EDEVICE;
+        size_t max_n = 1, sum_n = 0;
+        uint64_t max_len = 1;
+        for (const WorkItem &w : wk) {
+            max_n = std::max<size_t>(max_n, (size_t)w.n);
+            sum_n += (size_t)w.n;
+            max_len = std::max<uint64_t>(max_len, db->h_offs[w.sid + 1] - db->h_offs[w.sid]);
+        }
+        size_t ring = 16;
+        while (ring < max_len + 2) ring <<= 1;
+        const size_t per_lane = ring * sizeof(int2) + max_n * sizeof(Hsp);
+        size_t lanes = std::min<size_t>({std::max<size_t>(nw, 1), (size_t)256, std::max<size_t>(1, kQueueBytes / per_lane)});
+        lanes = (lanes + 63) / 64 * 64;
+        if (!ensure(ctx, ctx->q_dp, ctx->q_dp_cap, lanes * ring, "queue DP rings", false) ||
+            !ensure(ctx, ctx->q_saved, ctx->q_saved_cap, lanes * max_n, "queue saved HSPs", false) ||
+            !ensure(ctx, ctx->q_hsps, ctx->q_hsps_cap, std::max<size_t>(sum_n, 1), "queue HSPs", false))
+            return BLZ_ENOMEM;
+        GappedArgs g{};
+        g.res = db->res;
+        g.offs = db->offs;
+        g.qd = ctx->qdev;
+        g.pool = ctx->q_pool;
+        g.work = ctx->q_work;
+        g.order = ctx->q_order;
+        g.count = &ctx->q_ctr->work_count;
+        g.count_cap = (uint32_t)n;
+        g.next = &ctx->q_ctr->next_redo;
+        g.next_heavy = &ctx->q_ctr->next_heavy;
+        g.heavy_classes = 0;
+        g.dp_scratch = ctx->q_dp;
+        g.dp_wmask = (int32_t)ring - 1;
+        g.hsp_scratch = ctx->q_saved;
+        g.saved_cap = (int32_t)max_n;
+        g.hsps = ctx->q_hsps;
+        g.hsp_cap = std::max<size_t>(sum_n, 1);
+        g.results = ctx->q_results;
+        g.ovf = ctx->q_ovf;
+        g.ovf_cap = (uint32_t)n;
+        g.ctr = ctx->q_ctr;
+        if (nw && (!check(ctx, launch_gapped_wide(g, (int)(lanes / 64), 64, st), "queue gapped") ||
+                   !check(ctx, hipMemcpyAsync(&c, ctx->q_ctr, sizeof(c), hipMemcpyDeviceToHost, st), "queue counters") ||
+                   !check(ctx, hipStreamSynchronize(st), "queue sync")))
+            return BLZ_EDEVICE;
+        const size_t nr = c.result_count, nh = std::min<size_t>(c.hsp_used, g.hsp_cap);
+        std::vector<ResultItem> rs(nr);
+        std::vector<Hsp> hs(nh);
+        if ((nr && !check(ctx, hipMemcpy(rs.data(), ctx->q_results, sizeof(ResultItem) * nr, hipMemcpyDeviceToHost),
+                          "queue results")) ||
+            (nh && !check(ctx, hipMemcpy(hs.data(), ctx->q_hsps, sizeof(Hsp) * nh, hipMemcpyDeviceToHost),
+                          "queue HSPs")))
+            return BLZ_EDEVICE;
+        for (ResultItem &x : rs) {
+            x.off += out.hs.size();
+            out.res.push_back(x);
+        }
+        out.hs.insert(out.hs.end(), hs.begin(), hs.end());
+        out.init_hits += c.init_hits;
+        out.gapped_ext += c.gapped_ext;
+        out.dp_cells += c.dp_cells;
+        out.work_items += nw;
+        // what is still over a cap goes round again with a larger stage
+        const size_t nleft = std::min<size_t>(c.overflow_count, n);
+        pairs.assign(nleft, 0);
+        if (nleft && !check(ctx, hipMemcpy(pairs.data(), ctx->q_ovf, sizeof(int64_t) * nleft, hipMemcpyDeviceToHost),
+                            "queue overflow"))
+            return BLZ_EDEVICE;
+        stage_cap *= 8;
+        pool_cap = std::max(pool_cap, stage_cap);
+    }
+    out.left = std::move(pairs);
+    return BLZ_OK;
+}
+
 // Counters and HSP lists of the run just enqueued, into the caller's sinks (caller order,
-// subjects ascending).
-blz_status collect(blz_ctx *ctx, int nq, const blz_db *db, blz_hsp_sink *sinks, blz_run_stats *stats,
-                   int n_launches) {
+// subjects ascending).  Returns kRunAgain, having raised the context's buffer sizes to what the
+// run counted, when a run-wide buffer (init-hit pool, work list, HSP pool, overflow list) filled
+// up: the counters keep counting past a full buffer, so one more run fits.  Pairs past a
+// per-subject cap go to the overflow queue (tuning.overflow_mode 0) or are handed back.
+int collect(blz_ctx *ctx, int nq, const blz_db *db, blz_hsp_sink *sinks, blz_run_stats *stats, int n_launches,
+            int reruns) {
     hipStream_t st = ctx->stream;
     DevCounters *ctr = ctx->ctr;
 
@@ -1242,7 +1484,35 @@ blz_status collect(blz_ctx *ctx, int nq, const blz_db *db, blz_hsp_sink *sinks, 
         !check(ctx, hipStreamSynchronize(st), "sync"))
         return BLZ_EDEVICE;
     const double t1 = log_time() ? now_ms() : 0;
-    const size_t n_res = c.result_count, n_hs = c.hsp_used <= ctx->hsp_cap ? c.hsp_used : 0;
+    const bool hand_back = ctx->tune.overflow_mode == 1;
+    {
+        // overflow records: those the retry prep could not move, plus those past the list now
+        const size_t lost = (size_t)c.ovf_lost + (c.overflow_count > ctx->run_ovf_cap ? c.overflow_count - ctx->run_ovf_cap : 0);
+        const size_t ovf_need = std::max<size_t>((size_t)c.retry_count + c.ovf_lost, c.overflow_count);
+        auto grow = [](size_t need) { return need + need / 4 + 1024; };
+        bool again = false;
+        if (lost > 0) {
+            ctx->grow_ovf = std::max(ctx->grow_ovf, grow(ovf_need));
+            again = true;
+        }
+        if (!hand_back) {
+            if (c.pool_used > ctx->run_pool_cap) {
+                ctx->grow_pool = std::max(ctx->grow_pool, grow((size_t)c.pool_used));
+                again = true;
+            }
+            if (c.work_count > ctx->run_work_cap) {
+                ctx->grow_work = std::max(ctx->grow_work, grow((size_t)c.work_count));
+                again = true;
+            }
+            if (c.hsp_used > ctx->run_hsp_cap) {
+                ctx->grow_hsps = std::max(ctx->grow_hsps, grow((size_t)c.hsp_used));
+                again = true;
+            }
+        }
+        if (again) return kRunAgain;
+    }
+    // results written before a pool filled up stay valid: each points below the pool's end
+    const size_t n_res = c.result_count, n_hs = std::min<size_t>(c.hsp_used, ctx->run_hsp_cap);
     const size_t res_bytes = (sizeof(ResultItem) * n_res + 255) & ~size_t(255);
     const size_t need = res_bytes + sizeof(Hsp) * n_hs;
     if (need > ctx->pin_cap) {  // grow the pinned staging (rarely: it keeps its size)
@@ -1265,43 +1535,69 @@ blz_status collect(blz_ctx *ctx, int nq, const blz_db *db, blz_hsp_sink *sinks, 
     (void)hipEventElapsedTime(&ms_seed, ctx->ev[0], ctx->ev[1]);
     (void)hipEventElapsedTime(&ms_gapped, ctx->ev[1], ctx->ev[2]);
 
+    // pairs past a per-subject cap (all of them are on the list: a run that lost some ran again)
+    std::vector<int64_t> ovf_pairs(std::min<size_t>(c.overflow_count, ctx->run_ovf_cap));
+    if (!ovf_pairs.empty() && !check(ctx, hipMemcpy(ovf_pairs.data(), ctx->ovf, sizeof(int64_t) * ovf_pairs.size(),
+                                                    hipMemcpyDeviceToHost), "overflow list"))
+        return BLZ_EDEVICE;
+    QueueOut qo;
+    double ms_queue = 0;
+    if (!ovf_pairs.empty() && !hand_back) {
+        const double q0 = now_ms();
+        const blz_status qs = run_overflow_queue(ctx, db, ovf_pairs, qo);
+        if (qs != BLZ_OK) return qs;
+        ms_queue = now_ms() - q0;
+    } else {
+        qo.left = ovf_pairs;
+    }
+
     // per query (in the caller's order), subjects ascending: the (query, subject) keys
     // query * nsubj + subject are distinct, so an LSD radix sort of key << 24 | item index gives
-    // the order (a comparison sort when that does not fit 64 bits); then gather
-    std::vector<ResultItem> res(n_res);
+    // the order (a comparison sort when that does not fit 64 bits); then gather.  The overflow
+    // queue's lists follow the run's, their HSP offsets moved past the run's HSPs.
+    const size_t n_all = n_res + qo.res.size();
+    auto item = [&](size_t i) -> ResultItem {
+        if (i < n_res) return res_pin[i];
+        ResultItem r = qo.res[i - n_res];
+        r.off += n_hs;
+        return r;
+    };
+    std::vector<ResultItem> res(n_all);
     {
-        std::vector<uint64_t> key(n_res), tmp(n_res);
+        std::vector<uint64_t> key(n_all), tmp(n_all);
         uint64_t kmax = 0;
-        for (size_t i = 0; i < n_res; i++) {
-            key[i] = (uint64_t)(uint32_t)res_pin[i].qi * (uint64_t)db->nsubj + (uint64_t)res_pin[i].sid;
+        for (size_t i = 0; i < n_all; i++) {
+            const ResultItem r = item(i);
+            key[i] = (uint64_t)(uint32_t)r.qi * (uint64_t)db->nsubj + (uint64_t)r.sid;
             kmax = std::max(kmax, key[i]);
         }
         const int kbits = kmax ? 64 - __builtin_clzll(kmax) : 1;
-        if (n_res < (1u << 24) && kbits + 24 <= 64) {
-            for (size_t i = 0; i < n_res; i++) key[i] = (key[i] << 24) | i;
+        if (n_all < (1u << 24) && kbits + 24 <= 64) {
+            for (size_t i = 0; i < n_all; i++) key[i] = (key[i] << 24) | i;
             constexpr int kDigit = 11;
             std::vector<uint32_t> cnt(1u << kDigit);
             for (int sh = 24; sh < 24 + kbits; sh += kDigit) {
                 std::fill(cnt.begin(), cnt.end(), 0u);
-                for (size_t i = 0; i < n_res; i++) cnt[(key[i] >> sh) & ((1u << kDigit) - 1)]++;
+                for (size_t i = 0; i < n_all; i++) cnt[(key[i] >> sh) & ((1u << kDigit) - 1)]++;
                 uint32_t acc = 0;
                 for (auto &c0 : cnt) {
                     const uint32_t v = c0;
                     c0 = acc;
                     acc += v;
                 }
-                for (size_t i = 0; i < n_res; i++) tmp[cnt[(key[i] >> sh) & ((1u << kDigit) - 1)]++] = key[i];
+                for (size_t i = 0; i < n_all; i++) tmp[cnt[(key[i] >> sh) & ((1u << kDigit) - 1)]++] = key[i];
                 key.swap(tmp);
             }
-            for (size_t i = 0; i < n_res; i++) res[i] = res_pin[key[i] & 0xFFFFFFu];
+            for (size_t i = 0; i < n_all; i++) res[i] = item(key[i] & 0xFFFFFFu);
         } else {
-            std::vector<std::pair<uint64_t, uint32_t>> kv(n_res);
-            for (size_t i = 0; i < n_res; i++) kv[i] = {key[i], (uint32_t)i};
+            std::vector<std::pair<uint64_t, uint32_t>> kv(n_all);
+            for (size_t i = 0; i < n_all; i++) kv[i] = {key[i], (uint32_t)i};
             std::sort(kv.begin(), kv.end());
-            for (size_t i = 0; i < n_res; i++) res[i] = res_pin[kv[i].second];
+            for (size_t i = 0; i < n_all; i++) res[i] = item(kv[i].second);
         }
     }
     std::vector<Hsp> hs(hs_pin, hs_pin + n_hs);
+    hs.insert(hs.end(), qo.hs.begin(), qo.hs.end());
     ctx->last_first.assign((size_t)nq + 1, 0);
     ctx->last_nh.assign((size_t)nq, 0);
     int64_t total = 0;
@@ -1319,12 +1615,7 @@ blz_status collect(blz_ctx *ctx, int nq, const blz_db *db, blz_hsp_sink *sinks, 
     ctx->last_res.swap(res);
     ctx->last_hs.swap(hs);
     ctx->last_ovf.clear();
-    if (c.overflow_count > 0) {
-        ctx->last_ovf.resize(std::min<size_t>(c.overflow_count, ctx->ovf_cap));
-        if (!check(ctx, hipMemcpy(ctx->last_ovf.data(), ctx->ovf, sizeof(int64_t) * ctx->last_ovf.size(),
-                                  hipMemcpyDeviceToHost), "overflow list"))
-            return BLZ_EDEVICE;
-    }
+    for (int64_t e : qo.left) ctx->last_ovf.push_back(e & ~(int64_t)kOvfGapped);
     blz_status rc = BLZ_OK;
     if (sinks)
         for (int qcaller = 0; qcaller < nq; qcaller++) {
@@ -1336,20 +1627,24 @@ blz_status collect(blz_ctx *ctx, int nq, const blz_db *db, blz_hsp_sink *sinks, 
         stats->residues += db->nres * (uint64_t)nq;
         stats->word_hits += c.word_hits;
         stats->ungapped_ext += c.ungapped_ext;
-        stats->init_hits += c.init_hits;
-        stats->gapped_subjects += c.work_count;
-        stats->gapped_ext += c.gapped_ext;
-        stats->dp_cells += c.dp_cells;
+        stats->init_hits += c.init_hits + qo.init_hits;
+        stats->gapped_subjects += std::min<uint64_t>(c.work_count, ctx->run_work_cap) + qo.work_items;
+        stats->gapped_ext += c.gapped_ext + qo.gapped_ext;
+        stats->dp_cells += c.dp_cells + qo.dp_cells;
         stats->hsps += (uint64_t)total;
-        stats->overflow_subjects += c.overflow_count;
+        stats->overflow_subjects += ovf_pairs.size();
+        stats->overflow_left += qo.left.size();
         stats->ms_seed += ms_seed;
         stats->ms_gapped += ms_gapped;
         stats->ms_total += ms_seed + ms_gapped;
         stats->seed_launches += (int32_t)n_launches;
         stats->gapped_launches += 1;
+        stats->reruns += reruns;
+        stats->drain_rounds += qo.rounds;
+        stats->ms_drain += ms_queue;
     }
-    if (c.overflow_count > 0 && rc == BLZ_OK) {
-        ctx->err = "subjects exceeded device caps: " + std::to_string(c.overflow_count);
+    if (!qo.left.empty() && rc == BLZ_OK) {
+        ctx->err = "pairs handed back past the device caps: " + std::to_string(qo.left.size());
         rc = BLZ_EOVERFLOW;
     }
     if (log_time())

@@ -587,7 +587,7 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
 // the item's pack that has any.  Query offsets become offsets in that query.
 template <class R>
 __device__ void emit_init_hits(const SeedArgs &a, const QueryDev &qd, WaveState<R> &w, const Ctx &c, int64_t sid,
-                               uint32_t nih) {
+                               uint32_t nih, bool count_init = true) {
     const int lane = lane_id();
     const int nm = qd.n_members;
     auto overflow = [&](int qi) {
@@ -630,7 +630,7 @@ __device__ void emit_init_hits(const SeedArgs &a, const QueryDev &qd, WaveState<
                 const int bucket = cost_bucket(best, (int)nih);
                 a.work[wi] = WorkItem{sid, (uint64_t)base, (int32_t)nih, qi, bucket, 0};
                 atomicAdd(&a.ctr->bucket_count[bucket], 1u);
-                atomicAdd(&a.ctr->init_hits, (unsigned long long)nih);
+                if (count_init) atomicAdd(&a.ctr->init_hits, (unsigned long long)nih);
             } else {
                 const uint32_t k = atomicAdd(&a.ctr->overflow_count, 1u);
                 if (k < a.ovf_cap) a.ovf[k] = ((int64_t)qi << kOvfShift) | sid;
@@ -1015,7 +1015,9 @@ size_t seed2_lds_bytes(const SeedArgs &a, int table, int waves) {
 
 // Second chance for subjects the seed launches recorded as past a cap (init hits beyond the
 // per-wave stage, pool or work list full).  One workgroup moves the overflow list to the retry
-// list and clears it (entries past its capacity stay counted as overflow).
+// list (as long as the overflow list) and restarts the overflow list at 0; records that did not
+// fit the overflow list are added to ovf_lost, and the host then runs the batch again with a
+// longer list.
 __global__ void __launch_bounds__(256) k_seed_retry_prep(DevCounters *ctr, const int64_t *ovf, uint32_t ovf_cap,
                                                          int64_t *retry) {
     const uint32_t total = ctr->overflow_count;
@@ -1025,7 +1027,8 @@ __global__ void __launch_bounds__(256) k_seed_retry_prep(DevCounters *ctr, const
     if (threadIdx.x == 0) {
         ctr->retry_count = n;
         ctr->next_retry = 0;
-        ctr->overflow_count = total - n;
+        ctr->ovf_lost += total - n;
+        ctr->overflow_count = 0;
     }
 }
 
@@ -1070,8 +1073,10 @@ __global__ void __launch_bounds__(256) k_seed_retry(SeedArgs a) {
         k = first_lane(k);
         if (k >= a.ctr->retry_count) break;
         const int64_t e = a.retry[k];
-        const int qi = (int)(e >> kOvfShift);
-        const int64_t sid = e & ((int64_t(1) << kOvfShift) - 1);
+        const int qi = (int)((e >> kOvfShift) & kOvfQiMask);
+        const int64_t sid = e & kOvfSidMask;
+        // a pair the gapped stage gave up on had its init hits counted when they were first found
+        const bool count_init = (e & (int64_t)kOvfGapped) == 0;
         const QueryDev &qd = a.qd[qi];
         c.Q = qd.query;
         c.qlen = qd.qlen;
@@ -1094,7 +1099,7 @@ __global__ void __launch_bounds__(256) k_seed_retry(SeedArgs a) {
         const uint32_t nih = w.cnt[2];
         wave_sync();
         if (nih > 0) {
-            emit_init_hits(a, qd, w, c, sid, nih);
+            emit_init_hits(a, qd, w, c, sid, nih, count_init);
             wave_sync();
             if (lane == 0) w.cnt[2] = 0;
             wave_sync();
@@ -1111,6 +1116,11 @@ hipError_t launch_seed_retry(const SeedArgs &a, int64_t *retry, int blocks, int 
     SeedArgs r = a;
     r.retry = retry;
     hipLaunchKernelGGL(k_seed_retry, dim3(blocks), dim3(64 * waves), seed_retry_lds_bytes(a, waves), st, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_seed_pairs(const SeedArgs &a, int blocks, int waves, hipStream_t st) {
+    hipLaunchKernelGGL(k_seed_retry, dim3(blocks), dim3(64 * waves), seed_retry_lds_bytes(a, waves), st, a);
     return hipGetLastError();
 }
 

@@ -33,7 +33,7 @@ typedef enum {
     BLZ_ENOMEM = 2,     /* host or device allocation failed */
     BLZ_EDEVICE = 3,    /* HIP runtime error or no usable gfx950 device */
     BLZ_ECAPACITY = 4,  /* caller's sink too small: counts are filled, *total says how many */
-    BLZ_EOVERFLOW = 5,  /* some subjects exceeded the device caps and were not processed */
+    BLZ_EOVERFLOW = 5,  /* some (query, subject) pairs were handed back unprocessed (blz_overflow_drain) */
     BLZ_ETIMEOUT = 6    /* blz_server_wait: the request is not done yet */
 } blz_status;
 
@@ -72,12 +72,17 @@ typedef struct {
     uint64_t gapped_ext;       /* semi-gapped extensions (left+right counted once) */
     uint64_t dp_cells;         /* DP cells evaluated inside the X-drop bands */
     uint64_t hsps;             /* HSPs returned */
-    uint64_t overflow_subjects;/* subjects past the device caps (retried on the wide path) */
+    uint64_t overflow_subjects;/* (query, subject) pairs past a per-subject cap of the main
+                                  passes: run by the overflow queue, or handed back */
     double ms_seed;            /* seed/two-hit/ungapped kernel time */
     double ms_gapped;          /* gapped-stage kernel time */
     double ms_total;           /* device time from first launch to results on device */
     int32_t seed_launches;     /* kernel launches of the seed stage */
     int32_t gapped_launches;
+    uint64_t overflow_left;    /* pairs handed back (blz_overflow_drain); 0 unless BLZ_EOVERFLOW */
+    int32_t reruns;            /* runs repeated because a work buffer filled up */
+    int32_t drain_rounds;      /* overflow-queue passes on the device */
+    double ms_drain;           /* wall time of the overflow queue's passes */
 } blz_run_stats;
 
 /* Result sink (caller-owned).  Subjects with no HSP are omitted, as NCBI skips the stream
@@ -133,6 +138,21 @@ typedef struct {
                                   subjects per wave) or 64; 0: 16 */
     int32_t subject_groups;    /* 1: a wave scans one subject at a time; else short subjects of a
                                   single-query scan share a wave's scan (up to 8, 1,024 letters) */
+    int32_t overflow_mode;     /* 0: (query, subject) pairs past a per-subject device cap go to the
+                                  overflow queue, which runs them again on the device with buffers
+                                  sized to them, and a run that filled one of its work buffers is
+                                  run again with larger ones; 1: such pairs are handed back
+                                  (BLZ_EOVERFLOW, blz_overflow_drain) and the buffers keep their
+                                  size (only the overflow list grows, so every pair is reported) */
+    /* Capacity limits, for tests of the overflow paths (0: built-in size) */
+    int32_t cap_stage;         /* init hits per subject staged by the seed pass (<= 4096) */
+    int32_t cap_retry;         /* init hits per pair staged by the retry pass (<= 65536) */
+    int32_t cap_ovf;           /* overflow-list entries */
+    int32_t cap_wide_ring;     /* DP band ring of the wide gapped pass (power of two, <= 4096) */
+    int32_t cap_wide_saved;    /* HSPs per subject before the purge in the wide gapped pass */
+    int64_t cap_work;          /* (query, subject) work items of a run */
+    int64_t cap_pool;          /* init hits of a run */
+    int64_t cap_hsps;          /* HSPs of a run */
 } blz_tuning;
 
 blz_status blz_ctx_set_tuning(blz_ctx *ctx, const blz_tuning *t);
@@ -176,9 +196,11 @@ blz_status blz_prelim_run_batch(blz_ctx *ctx, const blz_query *const *queries, i
 blz_status blz_result_sizes(const blz_ctx *ctx, int32_t qi, int64_t *n_subjects, int64_t *n_hsps);
 blz_status blz_result_copy(const blz_ctx *ctx, int32_t qi, blz_hsp_sink *sink);
 
-/* (query index << 40 | subject) of the pairs the last run could not finish within the device
- * caps (the run returned BLZ_EOVERFLOW); *n receives their number.  The caller re-runs them on
- * its own path, as BLAZE re-evaluates capped subjects on the CPU (README.md:36-37). */
+/* (query index << 40 | subject) of the pairs the last run handed back (it returned
+ * BLZ_EOVERFLOW: tuning.overflow_mode 1, or the overflow queue could not size its buffers);
+ * *n receives their number.  By default the overflow queue finishes such pairs on the device
+ * itself, where BLAZE flags capped subjects for re-evaluation on the CPU (README.md:36-37,
+ * flag 255 at gpuBLAZE.cu:1088,1243-1247). */
 blz_status blz_overflow_drain(const blz_ctx *ctx, int64_t *pairs, int64_t cap, int64_t *n);
 
 /* ---- serving: a request loop over a resident database ------------------------------

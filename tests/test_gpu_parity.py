@@ -209,24 +209,29 @@ def test_long_subject_rings(engine):
 def test_init_hit_cap(engine):
     """A subject with more init hits than the per-wave stage holds (6,704 > 4,096; low ungapped
     cutoff) and more HSPs than the wave pass keeps (1,799).  The batched kernel rescans it in its
-    retry launch and the wide pass finishes it: lists identical to the oracle's.  The first seed
-    kernel has no retry: it reports the pair (blz_overflow_drain) and every other list is exact."""
+    retry launch and the wide pass finishes it; the first seed kernel has no retry and the
+    overflow queue finishes it: lists identical to the oracle's either way.  With overflow_mode 1
+    the first kernel reports the pair instead (blz_overflow_drain) and every other list is exact."""
     q = blz.synth_query(77, 2000)
     subs = [blz.synth_query(78, 40000), blz.synth_query(79, 300), blz.synth_query(80, 5000)]
     offs = np.zeros(len(subs) + 1, dtype=np.uint64)
     offs[1:] = np.cumsum([len(s) for s in subs])
     res = np.concatenate(subs)
     kw = dict(ungapped_cutoff=12, gapped_cutoff=22, x_gapped=12)
+    stats, _ = compare(engine, q, res, offs, blz.default_params(**kw), oracle.params(**kw))
+    assert stats["overflow_left"] == 0
     if engine.get_tuning()["seed_variant"] != 1:
-        compare(engine, q, res, offs, blz.default_params(**kw), oracle.params(**kw))
         return
-    engine.upload_db(res, offs)
+    assert stats["overflow_subjects"] == 1
+    tune = engine.get_tuning()
+    engine.set_tuning(overflow_mode=1)
     qh = engine.prepare(q, blz.default_params(**kw))
     try:
         got, _ = engine.run(qh, allow_overflow=True)
         assert engine.overflow_pairs() == [(0, 0)]
     finally:
         engine.free_query(qh)
+        engine.set_tuning(**tune)
     counts, hsps, _ = oracle_run(q, res, offs, oracle.params(**kw))
     want = oracle.as_lists(counts, hsps)
     have = got.as_dict()
