@@ -535,6 +535,13 @@ static int purge_common_endpoints(ora_hsp *h, int n)
     return m;
 }
 
+int ora_purge(ora_hsp *h, int n) { return purge_common_endpoints(h, n); }
+
+void ora_sort_hsps(ora_hsp *h, int n)
+{
+    if (n > 1) qsort(h, n, sizeof(ora_hsp), cmp_score);
+}
+
 int ora_prelim_subject(const uint8_t *query, int qlen, const ora_lookup *lk, const uint8_t *subj,
                        int slen, const ora_params *p, ora_hsp *out, int cap, ora_stats *st)
 {
@@ -548,10 +555,18 @@ int ora_prelim_subject(const uint8_t *query, int qlen, const ora_lookup *lk, con
         ora_stats dummy = {0};
         ora_word_finder(query, qlen, lk, subj, slen, p, ih, icap, &dummy);
     }
-    if (n == 0) {
-        free(ih);
-        return 0;
-    }
+    int ns = ora_gapped_stage(query, qlen, subj, slen, ih, n, p, out, cap, st);
+    free(ih);
+    return ns;
+}
+
+int ora_gapped_stage(const uint8_t *query, int qlen, const uint8_t *subj, int slen,
+                     const ora_init_hit *hits, int n, const ora_params *p, ora_hsp *out, int cap,
+                     ora_stats *st)
+{
+    if (n <= 0) return 0;
+    ora_init_hit *ih = (ora_init_hit *)malloc(sizeof(ora_init_hit) * n);
+    memcpy(ih, hits, sizeof(ora_init_hit) * n);
     qsort(ih, n, sizeof(ora_init_hit), cmp_init_hit);
     ora_hsp *saved = (ora_hsp *)malloc(sizeof(ora_hsp) * n);
     int ns = 0;

@@ -107,6 +107,18 @@ int ora_semi_gapped(const uint8_t *A, const uint8_t *B, int M, int N, int x_drop
 int ora_prelim_subject(const uint8_t *query, int qlen, const ora_lookup *lk, const uint8_t *subj,
                        int slen, const ora_params *p, ora_hsp *out, int cap, ora_stats *st);
 
+/* The gapped stage of ora_prelim_subject on a given list of init hits (NCBI
+ * BLAST_GetGappedScore): init-hit order, containment, gapped start, both DPs, cutoff, purge,
+ * sort.  Returns the HSP count (the first cap are written). */
+int ora_gapped_stage(const uint8_t *query, int qlen, const uint8_t *subj, int slen,
+                     const ora_init_hit *hits, int n, const ora_params *p, ora_hsp *out, int cap,
+                     ora_stats *st);
+
+/* Common-endpoint purge (NCBI Blast_HSPListPurgeHSPsWithCommonEndpoints, purge = TRUE): returns
+ * the number kept, h[0..k) in query-end order.  Final order (NCBI ScoreCompareHSPs). */
+int ora_purge(ora_hsp *h, int n);
+void ora_sort_hsps(ora_hsp *h, int n);
+
 /* Prelim stage over a database of nsubj subjects (residues concatenated, offsets[nsubj+1]).
  * counts[i] receives subject i's HSP count; HSPs are written subject-major into out
  * (capacity cap). Returns the total HSP count, or -1 if cap was too small (counts are

@@ -83,6 +83,11 @@ def lib() -> C.CDLL:
     L.ora_prelim_db.argtypes = [vp, i32, vp, vp, i64, C.POINTER(Params), i32, vp, vp, i64,
                                 C.POINTER(Stats)]
     L.ora_prelim_db.restype = i64
+    L.ora_gapped_stage.argtypes = [vp, i32, vp, i32, vp, i32, C.POINTER(Params), vp, i32, C.POINTER(Stats)]
+    L.ora_gapped_stage.restype = i32
+    L.ora_purge.argtypes = [vp, i32]
+    L.ora_purge.restype = i32
+    L.ora_sort_hsps.argtypes = [vp, i32]
     _lib = L
     return L
 
@@ -185,6 +190,42 @@ def prelim_subject(query, subject, p: Params | None = None, lookup: LookupTable 
         if n <= cap:
             return out[:n], st.as_dict()
         cap = n
+
+
+def gapped_stage(query, subject, init_hits, p: Params | None = None):
+    """Gapped stage over given init hits (records or (q_start, s_start, len, score) tuples)."""
+    p = p or params()
+    query, subject = _u8(query), _u8(subject)
+    ih = np.zeros(len(init_hits), dtype=INIT_DTYPE)
+    for k, h in enumerate(init_hits):
+        if isinstance(h, tuple):
+            ih[k]["q_start"], ih[k]["s_start"], ih[k]["len"], ih[k]["score"] = h
+            ih[k]["q_hit"], ih[k]["s_hit"] = h[0], h[1]
+        else:
+            ih[k] = h
+    cap = max(1, len(init_hits))
+    st = Stats()
+    out = np.empty(cap, dtype=HSP_DTYPE)
+    n = lib().ora_gapped_stage(query.ctypes.data, len(query), subject.ctypes.data, len(subject), ih.ctypes.data,
+                               len(ih), C.byref(p), out.ctypes.data, cap, C.byref(st))
+    return out[:n], st.as_dict()
+
+
+def hsp_records(rows) -> np.ndarray:
+    """(score, q_off, q_end, s_off, s_end, q_gs, s_gs) tuples -> HSP records."""
+    return np.array([tuple(r) for r in rows], dtype=HSP_DTYPE)
+
+
+def purge(hsps) -> np.ndarray:
+    h = np.ascontiguousarray(hsps, dtype=HSP_DTYPE).copy()
+    n = lib().ora_purge(h.ctypes.data, len(h))
+    return h[:n]
+
+
+def sort_hsps(hsps) -> np.ndarray:
+    h = np.ascontiguousarray(hsps, dtype=HSP_DTYPE).copy()
+    lib().ora_sort_hsps(h.ctypes.data, len(h))
+    return h
 
 
 def prelim_db(query, residues, offsets, p: Params | None = None, nthreads: int = 1, hsp_cap: int = 1 << 20):
