@@ -214,7 +214,8 @@ constexpr int kRetryBlocks = 16, kRetryWaves = 4;
 constexpr size_t kOvfStart = 1 << 16;     // overflow-list entries a run starts with
 // overflow queue: init hits staged per pair in its first round (x8 each further round), and the
 // most device memory one round's per-pair staging and per-lane DP rings may take
-constexpr size_t kQueueStage0 = 1 << 20;
+constexpr size_t kQueueStage0 = 1 << 17;
+constexpr size_t kQueueWaves = 1024;  // waves of one round's scan (one pair each at a time)
 constexpr size_t kQueueBytes = 2ull << 30;
 constexpr int kQueueRounds = 5;
 
@@ -1327,7 +1328,8 @@ struct QueueOut {
 // Overflow queue.  Pairs past a per-subject cap of the main passes (more init hits than the
 // retry scan stages, a DP band wider than the wide pass's ring, more saved HSPs than it holds)
 // are run again from their letters with buffers sized to them: the retry scan with a stage of
-// kQueueStage0 init hits per pair (x8 per further round for the pairs that still do not fit),
+// kQueueStage0 init hits per pair (x8 per further round for the pairs that still do not fit, up to
+// kQueueWaves waves within kQueueBytes of staging),
 // then the wide gapped pass with a DP ring as wide as the longest of their subjects (a band
 // never has more than subject length + 1 cells) and room for every init hit as a saved HSP.
 // BLAZE flags such subjects for re-evaluation on the CPU (README.md:36-37; flag 255 at
@@ -1342,7 +1344,7 @@ blz_status run_overflow_queue(blz_ctx *ctx, const blz_db *db, std::vector<int64_
         out.rounds++;
         const size_t n = pairs.size();
         // one wave per pair at a time, as many waves as the staging budget allows
-        size_t waves = std::min<size_t>({n, (size_t)256, std::max<size_t>(1, kQueueBytes / (stage_cap * sizeof(InitHit)))});
+        size_t waves = std::min<size_t>({n, kQueueWaves, std::max<size_t>(1, kQueueBytes / (stage_cap * sizeof(InitHit)))});
         const int wpb = (int)std::min<size_t>(waves, 4);
         const int blocks = (int)((waves + wpb - 1) / wpb);
         waves = (size_t)blocks * wpb;
