@@ -37,7 +37,27 @@ class BlzError(RuntimeError):
 class Params(C.Structure):
     _fields_ = [("word_threshold", C.c_int32), ("window", C.c_int32), ("x_ungapped", C.c_int32),
                 ("x_gapped", C.c_int32), ("gap_open", C.c_int32), ("gap_extend", C.c_int32),
-                ("ungapped_cutoff", C.c_int32), ("gapped_cutoff", C.c_int32)]
+                ("ungapped_cutoff", C.c_int32), ("gapped_cutoff", C.c_int32), ("reap_score", C.c_int32)]
+
+
+class Karlin(C.Structure):
+    _fields_ = [("lambda_", C.c_double), ("K", C.c_double), ("logK", C.c_double), ("H", C.c_double)]
+
+
+class StatsOptions(C.Structure):
+    _fields_ = [("expect", C.c_double), ("prelim_evalue", C.c_double), ("x_ungapped_bits", C.c_double),
+                ("x_gapped_bits", C.c_double), ("x_final_bits", C.c_double), ("gap_trigger_bits", C.c_double),
+                ("gap_open", C.c_int32), ("gap_extend", C.c_int32), ("word_threshold", C.c_int32),
+                ("window", C.c_int32), ("query_composition", C.c_int32), ("max_target_seqs", C.c_int32),
+                ("composition_based_stats", C.c_int32), ("reserved", C.c_int32)]
+
+
+class SearchStats(C.Structure):
+    _fields_ = [("ungapped", Karlin), ("gapped", Karlin), ("alpha", C.c_double), ("beta", C.c_double),
+                ("length_adjustment", C.c_int32), ("gap_trigger", C.c_int32), ("eff_query_len", C.c_int64),
+                ("eff_db_len", C.c_int64), ("eff_searchsp", C.c_double), ("prelim_evalue", C.c_double),
+                ("reap_score", C.c_int32), ("x_final", C.c_int32), ("hitlist_size", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class RunStats(C.Structure):
@@ -104,6 +124,8 @@ EXPORTS = [
     "blz_ctx_set_tuning", "blz_ctx_get_tuning", "blz_result_sizes", "blz_result_copy", "blz_overflow_drain",
     "blz_server_default_config", "blz_server_create", "blz_server_destroy", "blz_server_submit",
     "blz_server_wait", "blz_server_copy", "blz_server_release", "blz_server_get_stats",
+    "blz_stats_default_options", "blz_karlin_ungapped", "blz_karlin_gapped", "blz_search_setup", "blz_evalue",
+    "blz_result_hitlist",
 ]
 
 _lib = None
@@ -171,9 +193,65 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.blz_synth_db_size.restype = C.c_int
     lib.blz_synth_db_fill.argtypes = [C.POINTER(SynthSpec), C.POINTER(vp), vp, i32, vp, vp, vp]
     lib.blz_synth_db_fill.restype = C.c_int
+    lib.blz_stats_default_options.argtypes = [C.POINTER(StatsOptions)]
+    lib.blz_karlin_ungapped.argtypes = [vp, C.POINTER(Karlin)]
+    lib.blz_karlin_ungapped.restype = C.c_int
+    lib.blz_karlin_gapped.argtypes = [i32, i32, C.POINTER(Karlin), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.blz_karlin_gapped.restype = C.c_int
+    lib.blz_search_setup.argtypes = [C.POINTER(StatsOptions), vp, i32, u64, i64, C.POINTER(Params),
+                                     C.POINTER(SearchStats)]
+    lib.blz_search_setup.restype = C.c_int
+    lib.blz_evalue.argtypes = [C.POINTER(SearchStats), i32]
+    lib.blz_evalue.restype = C.c_double
+    lib.blz_result_hitlist.argtypes = [vp, i32, C.POINTER(SearchStats), C.POINTER(Sink), vp]
+    lib.blz_result_hitlist.restype = C.c_int
     if path is None:
         _lib = lib
     return lib
+
+
+# ---------------------------------------------------------------------------------------
+# search statistics
+# ---------------------------------------------------------------------------------------
+def stats_options(**kw) -> StatsOptions:
+    o = StatsOptions()
+    load_library().blz_stats_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def karlin_ungapped(freq=None) -> Karlin:
+    k = Karlin()
+    f = None if freq is None else np.ascontiguousarray(freq, dtype=np.float64)
+    st = load_library().blz_karlin_ungapped(None if f is None else f.ctypes.data, C.byref(k))
+    if st:
+        raise BlzError(st, "blz_karlin_ungapped")
+    return k
+
+
+def karlin_gapped(gap_open: int, gap_extend: int):
+    k, a, b = Karlin(), C.c_double(), C.c_double()
+    st = load_library().blz_karlin_gapped(gap_open, gap_extend, C.byref(k), C.byref(a), C.byref(b))
+    if st:
+        raise BlzError(st, "blz_karlin_gapped")
+    return k, a.value, b.value
+
+
+def search_setup(query, db_residues: int, db_seqs: int, options: StatsOptions | None = None):
+    """(Params, SearchStats) of one query against a database of that size."""
+    q = np.ascontiguousarray(query, dtype=np.uint8)
+    o = options or stats_options()
+    p, s = Params(), SearchStats()
+    st = load_library().blz_search_setup(C.byref(o), q.ctypes.data, len(q), db_residues, db_seqs, C.byref(p),
+                                         C.byref(s))
+    if st:
+        raise BlzError(st, "blz_search_setup")
+    return p, s
+
+
+def evalue(stats: SearchStats, score: int) -> float:
+    return load_library().blz_evalue(C.byref(stats), int(score))
 
 
 def default_params(**kw) -> Params:
@@ -339,6 +417,21 @@ class Engine:
         if st:
             self._err(st, "blz_result_copy")
         return Result(oids[: ns.value], counts[: ns.value], hsps[: nh.value])
+
+    def hitlist(self, qi: int, stats: SearchStats):
+        """Query qi's hit list of the last run (best subjects by e-value, in that order) and the
+        best e-value of each listed subject."""
+        ns, nh = C.c_int64(), C.c_int64()
+        st = self.lib.blz_result_sizes(self.ctx, qi, C.byref(ns), C.byref(nh))
+        if st:
+            self._err(st, "blz_result_sizes")
+        sink, (oids, counts, hsps) = self._sink(ns.value, nh.value)
+        ev = np.empty(max(1, ns.value), dtype=np.float64)
+        st = self.lib.blz_result_hitlist(self.ctx, qi, C.byref(stats), C.byref(sink), ev.ctypes.data)
+        if st:
+            self._err(st, "blz_result_hitlist")
+        k, h = sink.n_subjects, sink.n_hsps
+        return Result(oids[:k], counts[:k], hsps[:h]), ev[:k]
 
     def overflow_pairs(self) -> list[tuple[int, int]]:
         """(query index, subject) pairs the last run handed back (BLZ_EOVERFLOW)."""

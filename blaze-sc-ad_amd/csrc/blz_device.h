@@ -26,6 +26,7 @@ struct QueryDev {
     uint32_t ring_len, ring_base, ring_magic;
     int32_t window, x_ungapped, ungapped_cutoff;
     int32_t x_gapped, gap_open, gap_extend, gapped_cutoff;
+    int32_t reap_score;        // HSPs scoring below it are dropped after the purge and sort (0: none)
     // a pack of queries scanned together (seed stage): query offsets of the concatenation
     // belong to member m when member_start[m] <= q < member_start[m+1]; member_idx[m] is the
     // member's index in the run's query table.  A single query is a pack of one.

@@ -513,6 +513,11 @@ __device__ int finish_hsps(const GappedArgs &a, const WorkItem &wi, Hsp *saved, 
         }
         ns = m;
         insertion_sort<cmp_score>(saved, ns);
+        // prelim e-value reap (blast_engine.c:1457): for one query the e-value falls as the
+        // score rises, so it keeps the HSPs scoring at least reap_score, a prefix of the order
+        const int reap = a.qd[wi.qi].reap_score;
+        while (ns > 0 && saved[ns - 1].score < reap) ns--;
+        if (ns == 0) return kDone;
         const unsigned long long base = atomicAdd(&a.ctr->hsp_used, (unsigned long long)ns);
         if (base + ns > a.hsp_cap) {
             outcome = kOverflow;

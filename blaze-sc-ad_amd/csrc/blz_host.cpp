@@ -447,6 +447,7 @@ void blz_default_params(blz_params *p) {
     p->gap_extend = 1;
     p->ungapped_cutoff = 41;
     p->gapped_cutoff = 67;
+    p->reap_score = 0;
 }
 
 blz_status blz_ctx_create(int32_t device, blz_ctx **out) {
@@ -621,7 +622,7 @@ blz_status blz_query_prepare(blz_ctx *ctx, const uint8_t *query, int32_t qlen, c
     // walks stop at sequence ends and pack separators on letter 0's score of -32768, which
     // must exceed any ungapped X-drop
     if (p.window < 1 || p.x_ungapped < 1 || p.x_ungapped > 16384 || p.x_gapped < 0 || p.gap_open < 0 ||
-        p.gap_extend < 1) {
+        p.gap_extend < 1 || p.reap_score < 0) {
         ctx->err = "invalid parameters";
         return BLZ_EINVAL;
     }
@@ -688,11 +689,17 @@ QueryDev query_dev(const blz_query *q) {
     d.gap_open = q->params.gap_open;
     d.gap_extend = q->params.gap_extend;
     d.gapped_cutoff = q->params.gapped_cutoff;
+    d.reap_score = q->params.reap_score;
     d.n_members = 1;
     return d;
 }
 
-bool same_params(const blz_params &a, const blz_params &b) { return std::memcmp(&a, &b, sizeof(a)) == 0; }
+// Queries may share a seed scan when the seed stage treats them alike; the gapped stage reads each
+// query's own parameters.
+bool same_params(const blz_params &a, const blz_params &b) {
+    return a.word_threshold == b.word_threshold && a.window == b.window && a.x_ungapped == b.x_ungapped &&
+           a.ungapped_cutoff == b.ungapped_cutoff;
+}
 
 // Concatenated letters of a pack: at most what keeps the diagonal ring at 1,024 words.
 int pack_len(int window) { return std::max(64, 1024 - 128 - window); }
@@ -1693,6 +1700,52 @@ blz_status blz_result_sizes(const blz_ctx *ctx, int32_t qi, int64_t *n_subjects,
 blz_status blz_result_copy(const blz_ctx *ctx, int32_t qi, blz_hsp_sink *sink) {
     if (!ctx || !sink || qi < 0 || (size_t)qi + 1 >= ctx->last_first.size()) return BLZ_EINVAL;
     return copy_result(ctx, qi, sink);
+}
+
+blz_status blz_result_hitlist(const blz_ctx *ctx, int32_t qi, const blz_search_stats *s, blz_hsp_sink *sink,
+                              double *best_evalue) {
+    if (!ctx || !s || !sink || qi < 0 || (size_t)qi + 1 >= ctx->last_first.size() || s->hitlist_size < 1)
+        return BLZ_EINVAL;
+    // one entry per subject list: its best e-value is its first HSP's (lists are in score order)
+    struct Entry {
+        double ev;
+        int32_t score;
+        int64_t sid;
+        size_t item;
+    };
+    const size_t r0 = ctx->last_first[qi], r1 = ctx->last_first[qi + 1];
+    std::vector<Entry> e;
+    e.reserve(r1 - r0);
+    for (size_t i = r0; i < r1; i++) {
+        const ResultItem &r = ctx->last_res[i];
+        const int32_t best = ctx->last_hs[r.off].score;
+        e.push_back({blz_evalue(s, best), best, r.sid, i});
+    }
+    // NCBI's list order (s_EvalueCompareHSPLists): best e-value ascending, e-values below 1e-180
+    // counting as equal; then best score descending; then the higher ordinal id first
+    auto before = [](const Entry &a, const Entry &b) {
+        constexpr double kTiny = 1.0e-180;
+        if (!(a.ev < kTiny && b.ev < kTiny) && a.ev != b.ev) return a.ev < b.ev;
+        if (a.score != b.score) return a.score > b.score;
+        return a.sid > b.sid;
+    };
+    const size_t k = std::min<size_t>(e.size(), (size_t)s->hitlist_size);
+    std::partial_sort(e.begin(), e.begin() + k, e.end(), before);
+    int64_t nh = 0;
+    for (size_t i = 0; i < k; i++) nh += ctx->last_res[e[i].item].n;
+    sink->n_subjects = (int64_t)k;
+    sink->n_hsps = nh;
+    if ((int64_t)k > sink->subject_cap || nh > sink->hsp_cap) return BLZ_ECAPACITY;
+    int64_t o = 0;
+    for (size_t i = 0; i < k; i++) {
+        const ResultItem &r = ctx->last_res[e[i].item];
+        sink->oids[i] = r.sid;
+        sink->counts[i] = r.n;
+        std::memcpy(sink->hsps + o, ctx->last_hs.data() + r.off, sizeof(Hsp) * r.n);
+        o += r.n;
+        if (best_evalue) best_evalue[i] = e[i].ev;
+    }
+    return BLZ_OK;
 }
 
 blz_status blz_overflow_drain(const blz_ctx *ctx, int64_t *pairs, int64_t cap, int64_t *n) {

@@ -49,6 +49,9 @@ typedef struct {
     int32_t gap_extend;      /* default 1 */
     int32_t ungapped_cutoff; /* init-hit cutoff (raw), default 41 */
     int32_t gapped_cutoff;   /* HSP cutoff (raw), default 67 */
+    int32_t reap_score;      /* prelim e-value reap as a score: HSPs scoring below it are dropped
+                                after the purge and sort (blast_engine.c:1436-1457,896-930);
+                                0 keeps every HSP.  blz_search_setup computes it. */
 } blz_params;
 
 /* One HSP.  0-based offsets, *_end one past the last aligned residue; q_gs/s_gs is the
@@ -202,6 +205,67 @@ blz_status blz_result_copy(const blz_ctx *ctx, int32_t qi, blz_hsp_sink *sink);
  * itself, where BLAZE flags capped subjects for re-evaluation on the CPU (README.md:36-37,
  * flag 255 at gpuBLAZE.cu:1088,1243-1247). */
 blz_status blz_overflow_drain(const blz_ctx *ctx, int64_t *pairs, int64_t cap, int64_t *n);
+
+/* ---- search statistics ---------------------------------------------------------------
+ * What NCBI computes before the prelim stage and after it, for the lists this engine returns:
+ * the cutoffs the reference passes to its hook (blast_engine.c:2971-2981; the ungapped X-drop 16,
+ * gapped X-drop 38 and gap trigger 41 it hard-codes at gpuBLAZE.cu:37,59-63 are 7, 15 and 22 bits),
+ * the e-values (Blast_HSPListGetEvalues, blast_engine.c:1441), the prelim e-value reap
+ * (s_Blast_HSPListReapByPrelimEvalue, blast_engine.c:896-930,1457) and the per-query hit list
+ * the HSP stream hands to traceback (prelim_stage.cpp:1462-1498). */
+typedef struct {
+    double lambda, K, logK, H;
+} blz_karlin;
+
+typedef struct {
+    double expect;              /* e-value threshold of the search, default 10 */
+    double prelim_evalue;       /* reap threshold after the prelim stage; 0: expect */
+    double x_ungapped_bits;     /* default 7 */
+    double x_gapped_bits;       /* default 15 (prelim gapped X-drop) */
+    double x_final_bits;        /* default 25 (traceback X-drop) */
+    double gap_trigger_bits;    /* default 22 */
+    int32_t gap_open, gap_extend;   /* default 11, 1 (BLOSUM62 costs with published statistics) */
+    int32_t word_threshold, window; /* default 11, 40 */
+    int32_t query_composition;  /* 1: ungapped statistics from the query's composition against the
+                                   background; 0: background on both sides */
+    int32_t max_target_seqs;    /* default 500 */
+    int32_t composition_based_stats; /* default 2; nonzero sizes the prelim hit list 2x + 50 */
+    int32_t reserved;
+} blz_stats_options;
+
+typedef struct {
+    blz_karlin ungapped, gapped;
+    double alpha, beta;         /* length-adjustment constants of the gapped statistics */
+    int32_t length_adjustment;
+    int32_t gap_trigger;        /* raw */
+    int64_t eff_query_len, eff_db_len;
+    double eff_searchsp;
+    double prelim_evalue;
+    int32_t reap_score;         /* lowest score whose e-value is within prelim_evalue */
+    int32_t x_final;            /* raw traceback X-drop */
+    int32_t hitlist_size;       /* prelim hit-list size */
+    int32_t reserved;
+} blz_search_stats;
+
+void blz_stats_default_options(blz_stats_options *o);
+/* Karlin-Altschul parameters of BLOSUM62 for letter frequencies freq[28] (NULL: the
+ * Robinson-Robinson background) against the background. */
+blz_status blz_karlin_ungapped(const double *freq, blz_karlin *out);
+/* Published gapped parameters of BLOSUM62 for the gap costs (BLZ_EINVAL if none are published). */
+blz_status blz_karlin_gapped(int32_t gap_open, int32_t gap_extend, blz_karlin *out, double *alpha, double *beta);
+/* Statistics of one query against a database of db_residues letters in db_seqs sequences, and
+ * the search parameters they imply (params may be NULL). */
+blz_status blz_search_setup(const blz_stats_options *o, const uint8_t *query, int32_t qlen, uint64_t db_residues,
+                            int64_t db_seqs, blz_params *params, blz_search_stats *stats);
+/* E-value of a raw gapped score. */
+double blz_evalue(const blz_search_stats *s, int32_t score);
+/* The hit list of query qi of the last run (NCBI Blast_HSPResultsFromHSPStreamWithLimitEx and
+ * Blast_HSPResultsSortByEvalue): at most s->hitlist_size subjects, the best by best e-value
+ * (e-values below 1e-180 equal), then best score, then higher subject id first, in that order;
+ * each subject's HSPs keep their score order.  best_evalue[subject_cap] (optional) receives each
+ * listed subject's best e-value.  BLZ_ECAPACITY (sizes reported) if the sink is too small. */
+blz_status blz_result_hitlist(const blz_ctx *ctx, int32_t qi, const blz_search_stats *s, blz_hsp_sink *sink,
+                              double *best_evalue);
 
 /* ---- serving: a request loop over a resident database ------------------------------
  * No reference counterpart (the reference is a per-search hook, blast_engine.c:2983-2996).
