@@ -125,7 +125,7 @@ EXPORTS = [
     "blz_server_default_config", "blz_server_create", "blz_server_destroy", "blz_server_submit",
     "blz_server_wait", "blz_server_copy", "blz_server_release", "blz_server_get_stats",
     "blz_stats_default_options", "blz_karlin_ungapped", "blz_karlin_gapped", "blz_search_setup", "blz_evalue",
-    "blz_result_hitlist",
+    "blz_result_hitlist", "blz_prelim_run_noalloc",
 ]
 
 _lib = None
@@ -205,6 +205,8 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.blz_evalue.restype = C.c_double
     lib.blz_result_hitlist.argtypes = [vp, i32, C.POINTER(SearchStats), C.POINTER(Sink), vp]
     lib.blz_result_hitlist.restype = C.c_int
+    lib.blz_prelim_run_noalloc.argtypes = [vp, vp, vp, C.POINTER(Sink), C.POINTER(RunStats)]
+    lib.blz_prelim_run_noalloc.restype = C.c_int
     if path is None:
         _lib = lib
     return lib

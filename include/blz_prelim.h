@@ -192,6 +192,20 @@ blz_status blz_prelim_run(blz_ctx *ctx, const blz_query *q, const blz_db *db,
 blz_status blz_prelim_run_batch(blz_ctx *ctx, const blz_query *const *queries, int32_t nq,
                                 const blz_db *db, blz_hsp_sink *sinks, blz_run_stats *stats);
 
+/* blz_prelim_run without any allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Every device work buffer lies in one fixed 64 MiB array that the
+ * library holds on each GPU it runs on until the process ends; the call makes no hipMalloc,
+ * hipFree, pinned or host-heap allocation.  Databases of more than 131,072 subjects are rejected
+ * (BLZ_EINVAL).  The lists go straight into the caller's sink (BLZ_ECAPACITY, sizes reported, if
+ * it is too small) and are not kept for blz_result_*.  Subjects past its per-subject caps (256
+ * init hits, 1,024 HSPs) or its run caps (524,288 init hits, 262,144 HSPs) have no list: the call
+ * returns BLZ_EOVERFLOW with stats->overflow_left set, and those need blz_prelim_run.  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (from several threads or
+ * contexts) race and must be serialised by the caller. */
+blz_status blz_prelim_run_noalloc(blz_ctx *ctx, const blz_query *q, const blz_db *db, blz_hsp_sink *sink,
+                                  blz_run_stats *stats);
+
 /* Lists of query qi (its index in the last run's batch; 0 after blz_prelim_run) kept by the
  * context until its next run: their sizes, and a copy into a caller-sized sink (BLZ_ECAPACITY,
  * sizes still reported, if it is too small).  No reference counterpart: the reference hands

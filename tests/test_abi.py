@@ -42,10 +42,13 @@ def header_struct_fields(name):
                                           ("blz_run_stats", "RunStats"), ("blz_hsp_sink", "Sink"),
                                           ("blz_server_config", "ServerConfig"),
                                           ("blz_request_info", "RequestInfo"),
-                                          ("blz_server_stats", "ServerStats")])
+                                          ("blz_server_stats", "ServerStats"), ("blz_karlin", "Karlin"),
+                                          ("blz_stats_options", "StatsOptions"),
+                                          ("blz_search_stats", "SearchStats")])
 def test_binding_structs_match_header(cname, pytype):
-    """The ctypes mirrors declare the header's fields, in the header's order."""
-    assert header_struct_fields(cname) == [f for f, _ in getattr(blz, pytype)._fields_]
+    """The ctypes mirrors declare the header's fields, in the header's order (lambda_ stands for
+    the C field lambda, a Python keyword)."""
+    assert header_struct_fields(cname) == [f.rstrip("_") for f, _ in getattr(blz, pytype)._fields_]
 
 
 def test_no_device_is_an_error_not_a_fallback():
@@ -106,3 +109,4 @@ def test_bad_inputs_rejected_without_device():
     assert lib.blz_db_upload(None, None, None, 0, C.byref(h)) == 1
     assert lib.blz_query_prepare(None, None, 0, None, C.byref(h)) == 1
     assert lib.blz_prelim_run(None, None, None, None, None) == 1
+    assert lib.blz_prelim_run_noalloc(None, None, None, None, None) == 1
