@@ -87,17 +87,19 @@ class Tuning(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("slots", C.c_int32), ("max_batch", C.c_int32), ("max_batch_letters", C.c_int64),
-                ("reserved", C.c_int32)]
+                ("reserved", C.c_int32), ("max_batch_work", C.c_int64)]
 
 
 class RequestInfo(C.Structure):
     _fields_ = [("status", C.c_int), ("batch_queries", C.c_int32), ("latency_us", C.c_double),
-                ("queue_us", C.c_double), ("n_subjects", C.c_int64), ("n_hsps", C.c_int64)]
+                ("queue_us", C.c_double), ("n_subjects", C.c_int64), ("n_hsps", C.c_int64),
+                ("n_overflow", C.c_int64)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("batches", C.c_uint64),
-                ("residues", C.c_uint64), ("device_ms", C.c_double)]
+                ("residues", C.c_uint64), ("device_ms", C.c_double), ("reruns", C.c_uint64),
+                ("work_rate", C.c_double)]
 
 
 class Sink(C.Structure):
@@ -125,7 +127,7 @@ EXPORTS = [
     "blz_server_default_config", "blz_server_create", "blz_server_destroy", "blz_server_submit",
     "blz_server_wait", "blz_server_copy", "blz_server_release", "blz_server_get_stats",
     "blz_stats_default_options", "blz_karlin_ungapped", "blz_karlin_gapped", "blz_search_setup", "blz_evalue",
-    "blz_result_hitlist", "blz_prelim_run_noalloc",
+    "blz_result_hitlist", "blz_prelim_run_noalloc", "blz_server_overflow",
 ]
 
 _lib = None
@@ -183,6 +185,8 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.blz_server_wait.restype = C.c_int
     lib.blz_server_copy.argtypes = [vp, u64, C.POINTER(Sink)]
     lib.blz_server_copy.restype = C.c_int
+    lib.blz_server_overflow.argtypes = [vp, u64, vp, i64, C.POINTER(i64)]
+    lib.blz_server_overflow.restype = C.c_int
     lib.blz_server_release.argtypes = [vp, u64]
     lib.blz_server_release.restype = C.c_int
     lib.blz_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
@@ -469,7 +473,7 @@ class Server:
     thread, wait for each, fetch its lists."""
 
     def __init__(self, engine: "Engine", slots: int = 0, max_batch: int = 0, max_batch_letters: int = 0,
-                 tuning: dict | None = None):
+                 tuning: dict | None = None, max_batch_work: int = 0):
         self.lib = engine.lib
         cfg = ServerConfig()
         self.lib.blz_server_default_config(C.byref(cfg))
@@ -479,6 +483,8 @@ class Server:
             cfg.max_batch = max_batch
         if max_batch_letters:
             cfg.max_batch_letters = max_batch_letters
+        if max_batch_work:
+            cfg.max_batch_work = max_batch_work
         t = None
         if tuning:
             t = Tuning()
@@ -515,6 +521,16 @@ class Server:
         if st:
             raise BlzError(st, "blz_server_copy")
         return Result(oids[: info["n_subjects"]], counts[: info["n_subjects"]], hsps[: info["n_hsps"]])
+
+    def overflow(self, ticket: int) -> list[int]:
+        """Subjects the request's batch handed back."""
+        n = C.c_int64()
+        self.lib.blz_server_overflow(self.s, ticket, None, 0, C.byref(n))
+        buf = np.empty(max(1, n.value), dtype=np.int64)
+        st = self.lib.blz_server_overflow(self.s, ticket, buf.ctypes.data, n.value, C.byref(n))
+        if st:
+            raise BlzError(st, "blz_server_overflow")
+        return [int(v) for v in buf[: n.value]]
 
     def release(self, ticket: int) -> None:
         self.lib.blz_server_release(self.s, ticket)

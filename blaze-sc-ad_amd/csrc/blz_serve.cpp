@@ -42,7 +42,15 @@ struct Request {
     std::vector<int64_t> oids;
     std::vector<int32_t> counts;
     std::vector<blz_hsp> hsps;
+    std::vector<int64_t> overflow;  // subjects its batch handed back
 };
+
+// Work items a query is expected to make: (query, subject) pairs with init hits, about one per
+// 10,000 letter-subject pairs at the default parameters (99,759 for 500 letters x 2,000,000
+// subjects); the server starts from half again as much and then learns from its batches.
+constexpr double kPriorWorkRate = 1.5e-4;
+constexpr int64_t kSlotWorkItems = 48ll << 20;   // a slot context's work list at start
+constexpr int64_t kBytesPerWorkItem = 100;       // work item, two init hits, order entry, result
 
 // Query-size class of a request: the batch former keeps a batch within one class, so the
 // batch's seed launches are sized for similar queries.
@@ -70,6 +78,9 @@ struct blz_server {
     bool stopping = false;
     std::thread prep_th, dispatch_th;
     blz_server_stats stats{};
+    int64_t nsubj = 0;
+    double work_rate = kPriorWorkRate;  // work items per query letter and subject
+    int64_t work_budget = 0;            // estimated work items a batch may hold
 };
 
 namespace {
@@ -99,18 +110,23 @@ void prepare_loop(blz_server *s) {
     }
 }
 
-// Oldest waiting request plus same-class requests behind it, within the caps.
+// Oldest waiting request plus same-class requests behind it, within the caps: queries, query
+// letters, and the estimated device footprint (work items, each request's at most one per subject).
 std::vector<Request *> form_batch(blz_server *s) {
     std::vector<Request *> b;
     if (s->waiting.empty()) return b;
     const int cls = size_class((int32_t)s->waiting.front()->letters.size());
     int64_t letters = 0;
+    double work = 0;
     for (auto it = s->waiting.begin(); it != s->waiting.end() && (int)b.size() < s->cfg.max_batch;) {
         Request *r = *it;
         const int64_t L = (int64_t)r->letters.size();
-        if (size_class((int32_t)L) == cls && (b.empty() || letters + L <= s->cfg.max_batch_letters)) {
+        const double w = std::min((double)s->nsubj, s->work_rate * (double)L * (double)s->nsubj);
+        if (size_class((int32_t)L) == cls &&
+            (b.empty() || (letters + L <= s->cfg.max_batch_letters && work + w <= (double)s->work_budget))) {
             b.push_back(r);
             letters += L;
+            work += w;
             it = s->waiting.erase(it);
         } else {
             ++it;
@@ -154,8 +170,20 @@ void slot_loop(blz_server *s, Slot *sl) {
         for (Request *r : batch) qs.push_back(r->q);
         blz_run_stats st{};
         const blz_status rc = blz_prelim_run_batch(sl->ctx, qs.data(), (int32_t)qs.size(), s->db, nullptr, &st);
+        // pairs handed back, per request
+        std::vector<int64_t> handed;
+        if (rc == BLZ_EOVERFLOW) {
+            int64_t n = 0;
+            blz_overflow_drain(sl->ctx, nullptr, 0, &n);
+            handed.resize((size_t)n);
+            if (n > 0 && blz_overflow_drain(sl->ctx, handed.data(), n, &n) != BLZ_OK) handed.clear();
+        }
+        int64_t letters = 0;
         for (size_t i = 0; i < batch.size(); i++) {
             Request *r = batch[i];
+            letters += (int64_t)r->letters.size();
+            for (int64_t e : handed)
+                if ((e >> 40) == (int64_t)i) r->overflow.push_back(e & ((int64_t(1) << 40) - 1));
             r->status = rc;
             if (rc == BLZ_OK || rc == BLZ_EOVERFLOW) {
                 int64_t ns = 0, nh = 0;
@@ -184,6 +212,13 @@ void slot_loop(blz_server *s, Slot *sl) {
         s->stats.completed += batch.size();
         s->stats.residues += st.residues;
         s->stats.device_ms += st.ms_total;
+        s->stats.reruns += st.reruns > 0 ? 1 : 0;
+        // the admission's rate follows the batches: a quarter above the highest rate seen lately
+        if (letters > 0 && s->nsubj > 0) {
+            const double seen = (double)st.gapped_subjects / ((double)letters * (double)s->nsubj);
+            s->work_rate = std::max(1.25 * seen, std::max(0.95 * s->work_rate, kPriorWorkRate / 16));
+            s->stats.work_rate = s->work_rate;
+        }
         sl->busy = false;
         s->cv.notify_all();
     }
@@ -198,6 +233,7 @@ void blz_server_default_config(blz_server_config *c) {
     c->max_batch = 64;
     c->max_batch_letters = 32768;
     c->reserved = 0;
+    c->max_batch_work = 0;
 }
 
 blz_status blz_server_create(int32_t device, const blz_db *db, const blz_server_config *cfg,
@@ -210,10 +246,24 @@ blz_status blz_server_create(int32_t device, const blz_db *db, const blz_server_
         if (cfg->slots > 0) s->cfg.slots = cfg->slots;
         if (cfg->max_batch > 0) s->cfg.max_batch = cfg->max_batch;
         if (cfg->max_batch_letters > 0) s->cfg.max_batch_letters = cfg->max_batch_letters;
+        if (cfg->max_batch_work > 0) s->cfg.max_batch_work = cfg->max_batch_work;
     }
     if (s->cfg.slots > 16) s->cfg.slots = 16;
     s->db = db;
+    s->nsubj = blz_db_num_subjects(db);
+    s->stats.work_rate = s->work_rate;
     blz_status st = blz_ctx_create(device, &s->prep_ctx);
+    // a batch's footprint budget: what a slot's buffers hold at start, within its share of half
+    // the free device memory
+    s->work_budget = s->cfg.max_batch_work;
+    if (s->work_budget <= 0) {
+        size_t free_b = 0, total_b = 0;
+        if (st == BLZ_OK && hipSetDevice(device) == hipSuccess && hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+            s->work_budget = std::min<int64_t>(kSlotWorkItems, (int64_t)(free_b / 2 / (size_t)s->cfg.slots) /
+                                                                   kBytesPerWorkItem);
+        else
+            s->work_budget = kSlotWorkItems;
+    }
     s->slots.resize((size_t)s->cfg.slots);
     for (auto &sl : s->slots) {
         if (st != BLZ_OK) break;
@@ -301,7 +351,20 @@ blz_status blz_server_wait(blz_server *s, uint64_t ticket, int32_t timeout_ms, b
         info->queue_us = us(r->t_start - r->t_submit);
         info->n_subjects = (int64_t)r->oids.size();
         info->n_hsps = (int64_t)r->hsps.size();
+        info->n_overflow = (int64_t)r->overflow.size();
     }
+    return BLZ_OK;
+}
+
+blz_status blz_server_overflow(blz_server *s, uint64_t ticket, int64_t *subjects, int64_t cap, int64_t *n) {
+    if (!s || !n || cap < 0 || (cap > 0 && !subjects)) return BLZ_EINVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->requests.find(ticket);
+    if (it == s->requests.end() || !it->second->done) return BLZ_EINVAL;
+    const std::vector<int64_t> &v = it->second->overflow;
+    *n = (int64_t)v.size();
+    if (*n > cap) return BLZ_ECAPACITY;
+    std::copy(v.begin(), v.end(), subjects);
     return BLZ_OK;
 }
 

@@ -285,10 +285,12 @@ blz_status blz_result_hitlist(const blz_ctx *ctx, int32_t qi, const blz_search_s
  * No reference counterpart (the reference is a per-search hook, blast_engine.c:2983-2996).
  * Requests are submitted from any thread.  A preparation thread builds their lookup tables;
  * whenever one of `slots` contexts (each its own stream and buffers) is free, the oldest waiting
- * request and the same-size-class requests behind it (up to max_batch queries and
- * max_batch_letters query letters) run there as one batch (blz_prelim_run_batch).  Batches of
- * different slots overlap on the device, and requests arriving meanwhile join the next batch.
- * A request's lists are exactly those blz_prelim_run gives for it alone. */
+ * request and the same-size-class requests behind it run there as one batch
+ * (blz_prelim_run_batch), admitted by footprint: up to max_batch queries, max_batch_letters query
+ * letters and max_batch_work estimated work items, the estimate per request being its letters x
+ * the database's subjects x a rate learned from the batches already run.  Batches of different
+ * slots overlap on the device, and requests arriving meanwhile join the next batch.  A request's
+ * lists are exactly those blz_prelim_run gives for it alone. */
 typedef struct blz_server blz_server;
 
 typedef struct {
@@ -296,6 +298,10 @@ typedef struct {
     int32_t max_batch;         /* queries per batch; 0: 64 (1: one query per launch) */
     int64_t max_batch_letters; /* query letters per batch; 0: 32768 */
     int32_t reserved;
+    int64_t max_batch_work;    /* device footprint of a batch, as estimated (query, subject) work
+                                  items (each with its init hits, order entry and result: about
+                                  100 bytes); 0: what a slot's buffers hold at start (48 Mi items),
+                                  within its share of the free HBM.  A request alone always runs. */
 } blz_server_config;
 
 typedef struct {
@@ -304,12 +310,15 @@ typedef struct {
     double latency_us;         /* submit -> lists on the host */
     double queue_us;           /* submit -> its batch started */
     int64_t n_subjects, n_hsps;
+    int64_t n_overflow;        /* subjects handed back for this request (blz_server_overflow) */
 } blz_request_info;
 
 typedef struct {
     uint64_t submitted, completed, batches;
     uint64_t residues;         /* database residues scanned, summed over requests */
     double device_ms;          /* device time of the batches, summed over slots */
+    uint64_t reruns;           /* batches run again with grown buffers (a footprint estimate fell short) */
+    double work_rate;          /* work items per query letter and subject the admission assumes now */
 } blz_server_stats;
 
 void blz_server_default_config(blz_server_config *cfg);
@@ -323,6 +332,9 @@ blz_status blz_server_submit(blz_server *s, const uint8_t *query, int32_t qlen, 
 /* Blocks until the request is done (timeout_ms < 0: no limit; BLZ_ETIMEOUT if it ran out). */
 blz_status blz_server_wait(blz_server *s, uint64_t ticket, int32_t timeout_ms, blz_request_info *info);
 blz_status blz_server_copy(blz_server *s, uint64_t ticket, blz_hsp_sink *sink);
+/* Subjects of a finished request whose lists its batch handed back (status BLZ_EOVERFLOW; only
+ * with tuning.overflow_mode 1 or when the overflow queue could not size its buffers). */
+blz_status blz_server_overflow(blz_server *s, uint64_t ticket, int64_t *subjects, int64_t cap, int64_t *n);
 blz_status blz_server_release(blz_server *s, uint64_t ticket);
 blz_status blz_server_get_stats(blz_server *s, blz_server_stats *out);
 

@@ -63,3 +63,71 @@ def test_serving_alone_vs_mix():
         have = alone[i].as_dict()
         assert sorted(want) == sorted(have)
         assert all(np.array_equal(want[k], have[k]) for k in want)
+
+
+def _requests(seed, n, db_subjects):
+    rng = np.random.default_rng(seed)
+    lens = [int(x) for x in np.clip(np.exp(rng.normal(np.log(350), 0.8, n)), 30, 2500)]
+    qs = [blz.synth_query(seed * 1000 + i, L) for i, L in enumerate(lens)]
+    spec = blz.synth_spec(seed=seed, n_subjects=db_subjects, homolog_frac=0.1, ident_min=0.3, ident_max=0.8,
+                          repeat_frac=0.05)
+    res, offs, _ = blz.synth_db(spec, qs)
+    return qs, res, offs
+
+
+def test_admission_by_footprint():
+    """A footprint budget of one work item admits one request per batch although max_batch is 8;
+    a generous budget batches them; the lists are the solo runs' either way."""
+    qs, res, offs = _requests(31, 16, 3000)
+    with blz.Engine(0) as eng:
+        eng.upload_db(res, offs)
+        alone = []
+        for q in qs:
+            h = eng.prepare(q)
+            alone.append(eng.run(h)[0])
+            eng.free_query(h)
+        for budget, want_single in ((1, True), (10 ** 9, False)):
+            with blz.Server(eng, slots=2, max_batch=8, max_batch_work=budget) as srv:
+                tickets = [srv.submit(q) for q in qs]
+                sizes = []
+                for i, tk in enumerate(tickets):
+                    info = srv.wait(tk)
+                    assert info["status"] == 0
+                    got = srv.result(tk)
+                    srv.release(tk)
+                    sizes.append(info["batch_queries"])
+                    assert np.array_equal(got.hsps, alone[i].hsps) and np.array_equal(got.oids, alone[i].oids)
+                st = srv.stats()
+            print(f"\n[case] budget {budget}: batch sizes {sorted(set(sizes))}, rate {st['work_rate']:.3g}", flush=True)
+            assert (max(sizes) == 1) == want_single
+            assert st["work_rate"] > 0
+
+
+def test_served_overflow_subjects_per_ticket():
+    """Hand-back mode with tiny caps: each request reports the subjects its batch could not finish
+    (blz_server_overflow); its lists equal the oracle's for every other subject, and the oracle's
+    subjects missing from them are among the reported ones."""
+    qs, res, offs = _requests(32, 6, 1500)
+    tune = dict(overflow_mode=1, cap_stage=2, cap_retry=4, cap_ovf=8, cap_wide_ring=64, cap_wide_saved=3)
+    with blz.Engine(0) as eng:
+        eng.upload_db(res, offs)
+        with blz.Server(eng, slots=1, max_batch=6, tuning=tune) as srv:
+            tickets = [srv.submit(q) for q in qs]
+            out = []
+            for tk in tickets:
+                info = srv.wait(tk)
+                assert info["status"] in (0, 5)
+                got = srv.result(tk)
+                out.append((got, set(srv.overflow(tk)), info))
+                srv.release(tk)
+    assert sum(len(o) for _, o, _ in out) > 0
+    for i in (0, 3):
+        got, ovf, info = out[i]
+        assert info["n_overflow"] == len(ovf)
+        counts, hsps, _ = oracle.prelim_db(qs[i], res, offs, nthreads=NTHREADS)
+        want = oracle.as_lists(counts, hsps)
+        have = got.as_dict()
+        assert not ovf & set(have)
+        assert set(want) - set(have) <= ovf
+        for k in have:
+            assert np.array_equal(want[k], have[k]), (i, k)
