@@ -128,6 +128,9 @@ EXPORTS = [
     "blz_server_wait", "blz_server_copy", "blz_server_release", "blz_server_get_stats",
     "blz_stats_default_options", "blz_karlin_ungapped", "blz_karlin_gapped", "blz_search_setup", "blz_evalue",
     "blz_result_hitlist", "blz_prelim_run_noalloc", "blz_server_overflow",
+    "blz_blastdb_open", "blz_blastdb_close", "blz_blastdb_last_error", "blz_blastdb_num_seqs",
+    "blz_blastdb_num_residues", "blz_blastdb_max_length", "blz_blastdb_title", "blz_blastdb_sequence",
+    "blz_blastdb_header", "blz_blastdb_gather", "blz_db_upload_blastdb",
 ]
 
 _lib = None
@@ -211,9 +214,94 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.blz_result_hitlist.restype = C.c_int
     lib.blz_prelim_run_noalloc.argtypes = [vp, vp, vp, C.POINTER(Sink), C.POINTER(RunStats)]
     lib.blz_prelim_run_noalloc.restype = C.c_int
+    lib.blz_blastdb_open.argtypes = [C.c_char_p, C.POINTER(vp)]
+    lib.blz_blastdb_open.restype = C.c_int
+    lib.blz_blastdb_close.argtypes = [vp]
+    lib.blz_blastdb_last_error.restype = C.c_char_p
+    lib.blz_blastdb_num_seqs.argtypes = [vp]
+    lib.blz_blastdb_num_seqs.restype = i64
+    lib.blz_blastdb_num_residues.argtypes = [vp]
+    lib.blz_blastdb_num_residues.restype = u64
+    lib.blz_blastdb_max_length.argtypes = [vp]
+    lib.blz_blastdb_max_length.restype = i32
+    lib.blz_blastdb_title.argtypes = [vp]
+    lib.blz_blastdb_title.restype = C.c_char_p
+    lib.blz_blastdb_sequence.argtypes = [vp, i64, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(i32)]
+    lib.blz_blastdb_sequence.restype = C.c_int
+    lib.blz_blastdb_header.argtypes = [vp, i64, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(i64)]
+    lib.blz_blastdb_header.restype = C.c_int
+    lib.blz_blastdb_gather.argtypes = [vp, i32, vp, vp, vp]
+    lib.blz_blastdb_gather.restype = C.c_int
+    lib.blz_db_upload_blastdb.argtypes = [vp, vp, i32, C.POINTER(vp), vp]
+    lib.blz_db_upload_blastdb.restype = C.c_int
     if path is None:
         _lib = lib
     return lib
+
+
+class BlastDB:
+    """A BLAST protein database (makeblastdb format 4 or 5) opened with blz_blastdb_open."""
+
+    def __init__(self, path):
+        self.lib = load_library()
+        h = C.c_void_p()
+        st = self.lib.blz_blastdb_open(str(path).encode(), C.byref(h))
+        if st:
+            raise BlzError(st, f"blz_blastdb_open: {self.lib.blz_blastdb_last_error().decode()}")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.blz_blastdb_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def num_seqs(self) -> int:
+        return self.lib.blz_blastdb_num_seqs(self.h)
+
+    @property
+    def num_residues(self) -> int:
+        return self.lib.blz_blastdb_num_residues(self.h)
+
+    @property
+    def max_length(self) -> int:
+        return self.lib.blz_blastdb_max_length(self.h)
+
+    @property
+    def title(self) -> str:
+        return self.lib.blz_blastdb_title(self.h).decode()
+
+    def sequence(self, oid: int) -> np.ndarray:
+        p, n = C.POINTER(C.c_uint8)(), C.c_int32()
+        st = self.lib.blz_blastdb_sequence(self.h, oid, C.byref(p), C.byref(n))
+        if st:
+            raise BlzError(st, "blz_blastdb_sequence")
+        return np.ctypeslib.as_array(p, shape=(max(1, n.value),))[: n.value].copy()
+
+    def header(self, oid: int) -> bytes:
+        p, n = C.POINTER(C.c_uint8)(), C.c_int64()
+        st = self.lib.blz_blastdb_header(self.h, oid, C.byref(p), C.byref(n))
+        if st:
+            raise BlzError(st, "blz_blastdb_header")
+        return bytes(np.ctypeslib.as_array(p, shape=(max(1, n.value),))[: n.value]) if n.value else b""
+
+    def gather(self, sort_by_length: bool = False):
+        """(residues, offsets, perm): the sequences in OID order or ascending length."""
+        n = self.num_seqs
+        res = np.empty(self.num_residues + 1, np.uint8)
+        offs = np.empty(n + 1, np.uint64)
+        perm = np.empty(max(1, n), np.int64)
+        st = self.lib.blz_blastdb_gather(self.h, int(sort_by_length), res.ctypes.data, offs.ctypes.data,
+                                         perm.ctypes.data)
+        if st:
+            raise BlzError(st, f"blz_blastdb_gather: {self.lib.blz_blastdb_last_error().decode()}")
+        return res[: self.num_residues], offs, perm[:n]
 
 
 # ---------------------------------------------------------------------------------------
@@ -391,6 +479,20 @@ class Engine:
             self._err(st, "blz_db_upload")
         self.db = h
         self.db_n = n
+
+    def upload_blastdb(self, bdb: "BlastDB", sort_by_length: bool = True) -> np.ndarray:
+        """Makes a BLAST database resident; returns the OID of each uploaded position."""
+        if self.db:
+            self.lib.blz_db_free(self.db)
+            self.db = None
+        h = C.c_void_p()
+        perm = np.empty(max(1, bdb.num_seqs), np.int64)
+        st = self.lib.blz_db_upload_blastdb(self.ctx, bdb.h, int(sort_by_length), C.byref(h), perm.ctypes.data)
+        if st:
+            self._err(st, "blz_db_upload_blastdb")
+        self.db = h
+        self.db_n = bdb.num_seqs
+        return perm[: bdb.num_seqs]
 
     def prepare(self, query: np.ndarray, params: Params | None = None):
         query = np.ascontiguousarray(query, dtype=np.uint8)

@@ -338,6 +338,34 @@ blz_status blz_server_overflow(blz_server *s, uint64_t ticket, int64_t *subjects
 blz_status blz_server_release(blz_server *s, uint64_t ticket);
 blz_status blz_server_get_stats(blz_server *s, blz_server_stats *out);
 
+/* ---- BLAST protein databases (makeblastdb format versions 4 and 5) -------------------
+ * Replaces the database access of the hook (CSeqDB, seqdbimpl.cpp:716-730; gathered per chunk at
+ * gpuBLAZE.cu:1587-1618).  A database name is a volume's base path (<name>.pin, .psq and .phr,
+ * memory-mapped) or an alias file <name>.pal whose DBLIST line names the volumes.  The reference
+ * requires the FASTA sorted by length before makeblastdb (README.md:138); here gathering can sort
+ * by length instead, giving the OID of each position. */
+typedef struct blz_blastdb blz_blastdb;
+blz_status blz_blastdb_open(const char *path, blz_blastdb **out);
+void blz_blastdb_close(blz_blastdb *db);
+/* Message of the last failed blz_blastdb_* call on this thread. */
+const char *blz_blastdb_last_error(void);
+int64_t blz_blastdb_num_seqs(const blz_blastdb *db);
+uint64_t blz_blastdb_num_residues(const blz_blastdb *db);
+int32_t blz_blastdb_max_length(const blz_blastdb *db);
+const char *blz_blastdb_title(const blz_blastdb *db);
+/* Letters (NCBIstdaa, inside the mapped file) of one sequence. */
+blz_status blz_blastdb_sequence(const blz_blastdb *db, int64_t oid, const uint8_t **letters, int32_t *len);
+/* Header bytes (ASN.1 BER Blast-def-line-set, as stored) of one sequence. */
+blz_status blz_blastdb_header(const blz_blastdb *db, int64_t oid, const uint8_t **bytes, int64_t *len);
+/* All sequences into residues[num_residues] and offsets[num_seqs + 1]: in OID order, or with
+ * sort_by_length 1 in ascending length (OID order within a length); perm[num_seqs] (optional)
+ * receives the OID at each position. */
+blz_status blz_blastdb_gather(const blz_blastdb *db, int32_t sort_by_length, uint8_t *residues, uint64_t *offsets,
+                              int64_t *perm);
+/* blz_blastdb_gather then blz_db_upload: the database resident in HBM. */
+blz_status blz_db_upload_blastdb(blz_ctx *ctx, const blz_blastdb *bdb, int32_t sort_by_length, blz_db **out,
+                                 int64_t *perm);
+
 /* ---- synthetic data (NCBIstdaa, Robinson-Robinson background) ----------------------- */
 
 typedef struct {
