@@ -119,6 +119,28 @@ int ora_gapped_stage(const uint8_t *query, int qlen, const uint8_t *subj, int sl
 int ora_purge(ora_hsp *h, int n);
 void ora_sort_hsps(ora_hsp *h, int n);
 
+/* ---- traceback stage (traceback.c) ---- */
+enum { ORA_OP_SUB = 0, ORA_OP_GAP_Q = 1, ORA_OP_GAP_S = 2 };  /* aligned pair, subject letter alone, query letter alone */
+
+typedef struct {
+    int32_t score;
+    int32_t q_off, q_end, s_off, s_end;  /* ends exclusive */
+    int32_t q_gs, s_gs;                  /* where the alignment was anchored */
+    int32_t num_ident, align_len, gaps, gap_opens;
+    int32_t op_first, op_count;          /* its runs (op << 28 | length), left to right */
+} ora_tb_hsp;
+
+/* One direction of NCBI's gapped alignment with traceback (ALIGN_EX): score, best cell, and
+ * the path's operations from the best cell back to the origin. */
+int ora_align_ex(const uint8_t *A, const uint8_t *B, int M, int N, int x_dropoff, int gap_open, int gap_extend,
+                 int reverse, int *a_off, int *b_off, uint8_t *ops, int *nops, uint64_t *cells);
+/* Traceback of one subject's prelim HSPs (score order): containment against the HSPs traced so
+ * far, alignment from each gapped start with the final X-drop, cutoff, purge and final order.
+ * *nruns receives the runs written (negative: that many needed, runs_cap too small). */
+int ora_traceback_subject(const uint8_t *q, int qlen, const uint8_t *s, int slen, const ora_hsp *prelim, int n,
+                          int x_final, int gap_open, int gap_extend, int cutoff, ora_tb_hsp *out, int cap,
+                          uint32_t *runs, int runs_cap, int *nruns, uint64_t *cells);
+
 /* Prelim stage over a database of nsubj subjects (residues concatenated, offsets[nsubj+1]).
  * counts[i] receives subject i's HSP count; HSPs are written subject-major into out
  * (capacity cap). Returns the total HSP count, or -1 if cap was too small (counts are

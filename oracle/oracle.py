@@ -88,6 +88,12 @@ def lib() -> C.CDLL:
     L.ora_purge.argtypes = [vp, i32]
     L.ora_purge.restype = i32
     L.ora_sort_hsps.argtypes = [vp, i32]
+    L.ora_align_ex.argtypes = [vp, vp, i32, i32, i32, i32, i32, i32, C.POINTER(i32), C.POINTER(i32), vp,
+                               C.POINTER(i32), C.POINTER(C.c_uint64)]
+    L.ora_align_ex.restype = i32
+    L.ora_traceback_subject.argtypes = [vp, i32, vp, i32, vp, i32, i32, i32, i32, i32, vp, i32, vp, i32,
+                                        C.POINTER(i32), C.POINTER(C.c_uint64)]
+    L.ora_traceback_subject.restype = i32
     _lib = L
     return L
 
@@ -226,6 +232,44 @@ def sort_hsps(hsps) -> np.ndarray:
     h = np.ascontiguousarray(hsps, dtype=HSP_DTYPE).copy()
     lib().ora_sort_hsps(h.ctypes.data, len(h))
     return h
+
+
+TB_DTYPE = np.dtype(
+    [("score", "<i4"), ("q_off", "<i4"), ("q_end", "<i4"), ("s_off", "<i4"), ("s_end", "<i4"), ("q_gs", "<i4"),
+     ("s_gs", "<i4"), ("num_ident", "<i4"), ("align_len", "<i4"), ("gaps", "<i4"), ("gap_opens", "<i4"),
+     ("op_first", "<i4"), ("op_count", "<i4")]
+)
+OP_SUB, OP_GAP_Q, OP_GAP_S = 0, 1, 2
+
+
+def align_ex(A, B, x_dropoff, gap_open=11, gap_extend=1, reverse=False, M=None, N=None):
+    """(score, a_off, b_off, ops from the best cell back to the origin, cells) of one direction."""
+    A, B = _u8(A), _u8(B)
+    if M is None:
+        M = len(A) if reverse else len(A) - 1
+    if N is None:
+        N = len(B) if reverse else len(B) - 1
+    a, b, n, cells = C.c_int(), C.c_int(), C.c_int(), C.c_uint64(0)
+    ops = np.empty(M + N + 8, np.uint8)
+    sc = lib().ora_align_ex(A.ctypes.data, B.ctypes.data, M, N, x_dropoff, gap_open, gap_extend, 1 if reverse else 0,
+                            C.byref(a), C.byref(b), ops.ctypes.data, C.byref(n), C.byref(cells))
+    return sc, a.value, b.value, ops[: n.value].copy(), cells.value
+
+
+def traceback_subject(query, subject, prelim_hsps, x_final=64, gap_open=11, gap_extend=1, cutoff=0):
+    """Traced HSPs (TB_DTYPE, final order) of one subject, and their runs (op << 28 | length)."""
+    query, subject = _u8(query), _u8(subject)
+    pre = np.ascontiguousarray(prelim_hsps, dtype=HSP_DTYPE)
+    cap = max(1, len(pre))
+    out = np.empty(cap, TB_DTYPE)
+    runs_cap = 4 * (len(query) + len(subject)) * cap + 16
+    runs = np.empty(runs_cap, np.uint32)
+    nr, cells = C.c_int(), C.c_uint64(0)
+    n = lib().ora_traceback_subject(query.ctypes.data, len(query), subject.ctypes.data, len(subject), pre.ctypes.data,
+                                    len(pre), x_final, gap_open, gap_extend, cutoff, out.ctypes.data, cap,
+                                    runs.ctypes.data, runs_cap, C.byref(nr), C.byref(cells))
+    assert nr.value >= 0
+    return out[:n], runs[: nr.value].copy()
 
 
 def prelim_db(query, residues, offsets, p: Params | None = None, nthreads: int = 1, hsp_cap: int = 1 << 20):
