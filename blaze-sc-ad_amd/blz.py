@@ -24,6 +24,15 @@ HSP_DTYPE = np.dtype(
      ("q_gs", "<i4"), ("s_gs", "<i4")]
 )
 
+# blz_tb_hsp: a traced HSP; op_first/op_count index its edit script (runs op << 28 | length)
+TB_DTYPE = np.dtype(
+    [("score", "<i4"), ("q_off", "<i4"), ("q_end", "<i4"), ("s_off", "<i4"), ("s_end", "<i4"),
+     ("q_gs", "<i4"), ("s_gs", "<i4"), ("num_ident", "<i4"), ("align_len", "<i4"), ("gaps", "<i4"),
+     ("gap_opens", "<i4"), ("op_first", "<i4"), ("op_count", "<i4")]
+)
+OP_SUB, OP_GAP_Q, OP_GAP_S = 0, 1, 2  # aligned pair; subject letter vs gap; query letter vs gap
+TB_PAIR_DTYPE = np.dtype([("query", "<i4"), ("n_hsps", "<i4"), ("oid", "<i8")])
+
 STATUS = {0: "BLZ_OK", 1: "BLZ_EINVAL", 2: "BLZ_ENOMEM", 3: "BLZ_EDEVICE", 4: "BLZ_ECAPACITY",
           5: "BLZ_EOVERFLOW", 6: "BLZ_ETIMEOUT"}
 
@@ -82,7 +91,7 @@ class Tuning(C.Structure):
                 ("subject_groups", C.c_int32), ("overflow_mode", C.c_int32), ("cap_stage", C.c_int32),
                 ("cap_retry", C.c_int32), ("cap_ovf", C.c_int32), ("cap_wide_ring", C.c_int32),
                 ("cap_wide_saved", C.c_int32), ("cap_work", C.c_int64), ("cap_pool", C.c_int64),
-                ("cap_hsps", C.c_int64)]
+                ("cap_hsps", C.c_int64), ("cap_tb_bytes", C.c_int64), ("cap_tb_runs", C.c_int64)]
 
 
 class ServerConfig(C.Structure):
@@ -100,6 +109,21 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("batches", C.c_uint64),
                 ("residues", C.c_uint64), ("device_ms", C.c_double), ("reruns", C.c_uint64),
                 ("work_rate", C.c_double)]
+
+
+class TbParams(C.Structure):
+    _fields_ = [("x_dropoff", C.c_int32), ("gap_open", C.c_int32), ("gap_extend", C.c_int32),
+                ("cutoff", C.c_int32)]
+
+
+class TbStats(C.Structure):
+    _fields_ = [("pairs", C.c_uint64), ("prelim_hsps", C.c_uint64), ("hsps", C.c_uint64),
+                ("alignments", C.c_uint64), ("dp_cells", C.c_uint64), ("runs", C.c_uint64),
+                ("wide_pairs", C.c_uint64), ("reruns", C.c_int32), ("reserved", C.c_int32),
+                ("ms_main", C.c_double), ("ms_wide", C.c_double), ("ms_total", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 class Sink(C.Structure):
@@ -131,6 +155,7 @@ EXPORTS = [
     "blz_blastdb_open", "blz_blastdb_close", "blz_blastdb_last_error", "blz_blastdb_num_seqs",
     "blz_blastdb_num_residues", "blz_blastdb_max_length", "blz_blastdb_title", "blz_blastdb_sequence",
     "blz_blastdb_header", "blz_blastdb_gather", "blz_db_upload_blastdb",
+    "blz_tb_default_params", "blz_traceback", "blz_traceback_sizes", "blz_traceback_copy",
 ]
 
 _lib = None
@@ -234,6 +259,13 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.blz_blastdb_gather.restype = C.c_int
     lib.blz_db_upload_blastdb.argtypes = [vp, vp, i32, C.POINTER(vp), vp]
     lib.blz_db_upload_blastdb.restype = C.c_int
+    lib.blz_tb_default_params.argtypes = [C.POINTER(TbParams)]
+    lib.blz_traceback.argtypes = [vp, vp, C.POINTER(vp), i32, vp, i64, vp, C.POINTER(TbParams), C.POINTER(TbStats)]
+    lib.blz_traceback.restype = C.c_int
+    lib.blz_traceback_sizes.argtypes = [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]
+    lib.blz_traceback_sizes.restype = C.c_int
+    lib.blz_traceback_copy.argtypes = [vp, vp, i64, vp, i64, vp, i64]
+    lib.blz_traceback_copy.restype = C.c_int
     if path is None:
         _lib = lib
     return lib
@@ -568,6 +600,56 @@ class Engine:
         if st and not (st == 5 and allow_overflow):
             self._err(st, "blz_prelim_run_batch")
         return ([self._fetch(i) for i in range(n)] if fetch else None), stats.as_dict()
+
+    def traceback(self, qhs, lists, params: TbParams | None = None):
+        """Traceback of the listed pairs: lists[i] is a Result (e.g. a hit list) of prepared query
+        qhs[i].  Returns, per query, {oid: (traced HSPs as TB_DTYPE in score order, their runs)}
+        (op_first indexing that pair's runs), and the call's stats."""
+        nq = len(qhs)
+        pairs, pre = [], []
+        for qi, r in enumerate(lists):
+            for oid, c in zip(np.asarray(r.oids).tolist(), np.asarray(r.counts).tolist()):
+                pairs.append((qi, c, oid))
+            pre.append(np.asarray(r.hsps, dtype=HSP_DTYPE))
+        pa = np.array(pairs, dtype=TB_PAIR_DTYPE) if pairs else np.empty(0, TB_PAIR_DTYPE)
+        hs = np.ascontiguousarray(np.concatenate(pre) if pre else np.empty(0, HSP_DTYPE))
+        arr = (C.c_void_p * max(1, nq))(*[q.value if isinstance(q, C.c_void_p) else q for q in qhs])
+        p = params if params is not None else tb_params()
+        stats = TbStats()
+        st = self.lib.blz_traceback(self.ctx, self.db, arr, nq, pa.ctypes.data, len(pa), hs.ctypes.data, C.byref(p),
+                                    C.byref(stats))
+        if st:
+            self._err(st, "blz_traceback")
+        n_p, n_h, n_r = C.c_int64(), C.c_int64(), C.c_int64()
+        self.lib.blz_traceback_sizes(self.ctx, C.byref(n_p), C.byref(n_h), C.byref(n_r))
+        counts = np.empty(max(1, n_p.value), np.int32)
+        tb = np.empty(max(1, n_h.value), TB_DTYPE)
+        runs = np.empty(max(1, n_r.value), np.uint32)
+        st = self.lib.blz_traceback_copy(self.ctx, counts.ctypes.data, n_p.value, tb.ctypes.data, n_h.value,
+                                         runs.ctypes.data, n_r.value)
+        if st:
+            self._err(st, "blz_traceback_copy")
+        out = [dict() for _ in range(nq)]
+        o = 0
+        for (qi, _, oid), c in zip(pairs, counts[: n_p.value].tolist()):
+            t = tb[o:o + c].copy()
+            o += c
+            if c:
+                r0 = int(t["op_first"][0])
+                r1 = int(t["op_first"][-1] + t["op_count"][-1])
+                t["op_first"] -= r0
+                out[qi][oid] = (t, runs[r0:r1].copy())
+            else:
+                out[qi][oid] = (t, np.empty(0, np.uint32))
+        return out, stats.as_dict()
+
+
+def tb_params(**kw) -> TbParams:
+    p = TbParams()
+    load_library().blz_tb_default_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
 
 
 class Server:

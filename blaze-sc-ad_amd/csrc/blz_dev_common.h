@@ -155,4 +155,56 @@ __device__ __forceinline__ int two_hit(const int16_t (*mat)[28], const uint8_t *
     return left_score > right_score ? left_score : right_score;
 }
 
+// ---------------------------------------------------------------------------------------
+// purge orders and the final order (NCBI s_QueryOffsetCompareHSPs, s_QueryEndCompareHSPs,
+// ScoreCompareHSPs; the gapped start breaks remaining ties)
+__device__ __forceinline__ int cmp_start(const Hsp &a, const Hsp &b) {
+    if (a.q_off != b.q_off) return a.q_off < b.q_off ? -1 : 1;
+    if (a.s_off != b.s_off) return a.s_off < b.s_off ? -1 : 1;
+    if (a.score != b.score) return a.score > b.score ? -1 : 1;
+    if (a.q_end != b.q_end) return a.q_end > b.q_end ? -1 : 1;
+    if (a.s_end != b.s_end) return a.s_end > b.s_end ? -1 : 1;
+    if (a.q_gs != b.q_gs) return a.q_gs < b.q_gs ? -1 : 1;
+    if (a.s_gs != b.s_gs) return a.s_gs < b.s_gs ? -1 : 1;
+    return 0;
+}
+__device__ __forceinline__ int cmp_end(const Hsp &a, const Hsp &b) {
+    if (a.q_end != b.q_end) return a.q_end < b.q_end ? -1 : 1;
+    if (a.s_end != b.s_end) return a.s_end < b.s_end ? -1 : 1;
+    if (a.score != b.score) return a.score > b.score ? -1 : 1;
+    if (a.q_off != b.q_off) return a.q_off > b.q_off ? -1 : 1;
+    if (a.s_off != b.s_off) return a.s_off > b.s_off ? -1 : 1;
+    if (a.q_gs != b.q_gs) return a.q_gs < b.q_gs ? -1 : 1;
+    if (a.s_gs != b.s_gs) return a.s_gs < b.s_gs ? -1 : 1;
+    return 0;
+}
+__device__ __forceinline__ int cmp_score(const Hsp &a, const Hsp &b) {
+    if (a.score != b.score) return a.score > b.score ? -1 : 1;
+    if (a.s_off != b.s_off) return a.s_off < b.s_off ? -1 : 1;
+    if (a.s_end != b.s_end) return a.s_end > b.s_end ? -1 : 1;
+    if (a.q_off != b.q_off) return a.q_off < b.q_off ? -1 : 1;
+    if (a.q_end != b.q_end) return a.q_end > b.q_end ? -1 : 1;
+    if (a.q_gs != b.q_gs) return a.q_gs < b.q_gs ? -1 : 1;
+    if (a.s_gs != b.s_gs) return a.s_gs < b.s_gs ? -1 : 1;
+    return 0;
+}
+
+// records sorted by their HSP fields: Hsp itself, or a record holding one as member h
+__device__ __forceinline__ const Hsp &hsp_key(const Hsp &h) { return h; }
+template <class R>
+__device__ __forceinline__ const Hsp &hsp_key(const R &r) { return r.h; }
+
+template <int (*CMP)(const Hsp &, const Hsp &), class R>
+__device__ void insertion_sort(R *h, int n) {
+    for (int i = 1; i < n; i++) {
+        R x = h[i];
+        int j = i - 1;
+        while (j >= 0 && CMP(hsp_key(x), hsp_key(h[j])) < 0) {
+            h[j + 1] = h[j];
+            j--;
+        }
+        h[j + 1] = x;
+    }
+}
+
 }  // namespace blz

@@ -140,4 +140,62 @@ int gapped_wave_blocks_per_cu();
 hipError_t launch_gapped_seg(const GappedArgs &a, int blocks, hipStream_t st);
 int gapped_seg_blocks_per_cu();
 
+// ---- traceback stage ----------------------------------------------------------------------
+// Traced HSP, same layout as blz_tb_hsp in blz_prelim.h: the HSP fields, then identities,
+// alignment columns, gap letters, gap openings and the edit script runs[op_first, +op_count).
+struct TbHsp {
+    Hsp h;
+    int32_t num_ident, align_len, gaps, gap_opens, op_first, op_count;
+};
+// One (query, subject) pair of a traceback call: prelim HSPs pre[off, off + n) in score order;
+// its traced HSPs go to out[off, off + n) (out_n[pair] of them, -1 if the pass gave it back).
+struct TbPair {
+    int64_t sid;
+    uint64_t off;
+    int32_t qi, n;
+};
+struct TbQuery {
+    const uint8_t *q;
+    int32_t qlen, pad;
+};
+struct TbCounters {
+    unsigned long long runs_used;  // edit-script runs reserved (may pass runs_cap: then the call runs again)
+    unsigned long long cells;      // DP cells inside the X-drop bands
+    unsigned long long dps;        // one-direction alignments
+    unsigned int next_pair;        // work-stealing cursor
+    unsigned int back_count;       // pairs handed to the wide pass (back[])
+    unsigned int bad_walk;         // paths that left the stored rows (never expected)
+    unsigned int pad;
+};
+struct TbArgs {
+    const uint8_t *res;
+    const uint64_t *offs;
+    const TbQuery *qs;
+    const TbPair *pairs;
+    uint32_t n_pairs;
+    const uint32_t *list;      // wide pass: pair indices back[0, n_list); main pass: null
+    uint32_t n_list;
+    const Hsp *pre;
+    TbHsp *out;
+    int32_t *out_n;
+    uint32_t *runs;
+    unsigned long long runs_cap;
+    uint32_t *back;            // main pass: pairs whose DP outgrew a per-wave buffer
+    uint32_t back_cap;
+    TbCounters *ctr;
+    // per-wave storage (wave w at w * cap)
+    uint8_t *tb;               // traceback bytes of one DP
+    unsigned long long tb_cap;
+    int4 *rows;                // per row: first column, byte offset, length
+    int32_t rows_cap;
+    uint32_t *wruns;           // runs of the current HSP before they are placed
+    int32_t wruns_cap;
+    int2 *ring;                // wide pass: DP band ring, ring_mask + 1 entries per wave
+    int32_t ring_mask;
+    int32_t x_final, gap_open, gap_extend, cutoff;
+};
+constexpr int kTbLdsRing = 1024;  // main pass: DP band ring per wave in LDS; wider bands go to the wide pass
+hipError_t launch_traceback(const TbArgs &a, int blocks, bool wide, hipStream_t st);
+int traceback_blocks_per_cu(bool wide);
+
 }  // namespace blz

@@ -205,52 +205,6 @@ __device__ __forceinline__ int gapped_start(const int16_t (*mat)[kStride], const
     return max_score > 0 ? q_start + max_k - kHspMaxWindow / 2 : q_start;
 }
 
-// purge orders and the final order (NCBI s_QueryOffsetCompareHSPs, s_QueryEndCompareHSPs,
-// ScoreCompareHSPs; the gapped start breaks remaining ties)
-__device__ __forceinline__ int cmp_start(const Hsp &a, const Hsp &b) {
-    if (a.q_off != b.q_off) return a.q_off < b.q_off ? -1 : 1;
-    if (a.s_off != b.s_off) return a.s_off < b.s_off ? -1 : 1;
-    if (a.score != b.score) return a.score > b.score ? -1 : 1;
-    if (a.q_end != b.q_end) return a.q_end > b.q_end ? -1 : 1;
-    if (a.s_end != b.s_end) return a.s_end > b.s_end ? -1 : 1;
-    if (a.q_gs != b.q_gs) return a.q_gs < b.q_gs ? -1 : 1;
-    if (a.s_gs != b.s_gs) return a.s_gs < b.s_gs ? -1 : 1;
-    return 0;
-}
-__device__ __forceinline__ int cmp_end(const Hsp &a, const Hsp &b) {
-    if (a.q_end != b.q_end) return a.q_end < b.q_end ? -1 : 1;
-    if (a.s_end != b.s_end) return a.s_end < b.s_end ? -1 : 1;
-    if (a.score != b.score) return a.score > b.score ? -1 : 1;
-    if (a.q_off != b.q_off) return a.q_off > b.q_off ? -1 : 1;
-    if (a.s_off != b.s_off) return a.s_off > b.s_off ? -1 : 1;
-    if (a.q_gs != b.q_gs) return a.q_gs < b.q_gs ? -1 : 1;
-    if (a.s_gs != b.s_gs) return a.s_gs < b.s_gs ? -1 : 1;
-    return 0;
-}
-__device__ __forceinline__ int cmp_score(const Hsp &a, const Hsp &b) {
-    if (a.score != b.score) return a.score > b.score ? -1 : 1;
-    if (a.s_off != b.s_off) return a.s_off < b.s_off ? -1 : 1;
-    if (a.s_end != b.s_end) return a.s_end > b.s_end ? -1 : 1;
-    if (a.q_off != b.q_off) return a.q_off < b.q_off ? -1 : 1;
-    if (a.q_end != b.q_end) return a.q_end > b.q_end ? -1 : 1;
-    if (a.q_gs != b.q_gs) return a.q_gs < b.q_gs ? -1 : 1;
-    if (a.s_gs != b.s_gs) return a.s_gs < b.s_gs ? -1 : 1;
-    return 0;
-}
-
-template <int (*CMP)(const Hsp &, const Hsp &)>
-__device__ void insertion_sort(Hsp *h, int n) {
-    for (int i = 1; i < n; i++) {
-        Hsp x = h[i];
-        int j = i - 1;
-        while (j >= 0 && CMP(x, h[j]) < 0) {
-            h[j + 1] = h[j];
-            j--;
-        }
-        h[j + 1] = x;
-    }
-}
-
 enum ItemOutcome { kDone = 0, kRedo = 1, kOverflow = 2 };  // kRedo: band or HSP count past this pass's caps
 
 // Cross-lane steps within a segment of kSeg lanes: the whole wave (64), or one DPP row (16), so

@@ -335,6 +335,7 @@ void blz_ctx_destroy(blz_ctx *ctx) {
                     ctx->q_work,     ctx->q_order,     ctx->q_ovf,      ctx->q_dp,      ctx->q_saved,
                     ctx->q_hsps,     ctx->q_results};
     ctx->drop_graph();
+    tb_release(ctx->tb);
     for (auto &pk : ctx->packs) pk.t.release();
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -360,7 +361,8 @@ blz_status blz_ctx_set_tuning(blz_ctx *ctx, const blz_tuning *t) {
         (t->gapped_lanes != 0 && t->gapped_lanes != 16 && t->gapped_lanes != 64) || t->subject_groups < 0 ||
         t->subject_groups > 1 || t->overflow_mode < 0 || t->overflow_mode > 1 || t->cap_stage < 0 ||
         t->cap_retry < 0 || t->cap_ovf < 0 || t->cap_wide_ring < 0 || t->cap_wide_saved < 0 || t->cap_work < 0 ||
-        t->cap_work > 0xFFFFFFFFll || t->cap_pool < 0 || t->cap_hsps < 0)
+        t->cap_work > 0xFFFFFFFFll || t->cap_pool < 0 || t->cap_hsps < 0 ||
+        t->cap_tb_bytes < 0 || t->cap_tb_runs < 0)
         return BLZ_EINVAL;
     if (t->query_packing != ctx->tune.query_packing || t->pack_len != ctx->tune.pack_len) {
         // cached packs were made under the old choice

@@ -34,6 +34,10 @@ struct DevTables {
     }
 };
 
+// Device buffers and the last result of the traceback stage (blz_traceback.cpp).
+struct TbState;
+void tb_release(TbState *s);
+
 // Several short queries scanned as one concatenation (their lookup tables merged); the
 // gapped stage still runs per query.
 struct Pack {
@@ -141,6 +145,7 @@ struct blz_ctx {
     std::vector<size_t> last_first;      // [nq + 1] first result item of each query
     std::vector<int64_t> last_nh;        // [nq] HSPs of each query
     std::vector<int64_t> last_ovf;       // overflow pairs: query << 40 | subject
+    TbState *tb = nullptr;               // traceback stage, created by its first call
     // launch sequence of the last batch, captured; valid while the same batch comes again and
     // no other run has rewritten the device tables in between
     hipGraph_t graph[2] = {nullptr, nullptr};          // seed stage, gapped stage

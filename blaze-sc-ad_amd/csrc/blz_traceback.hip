@@ -1,0 +1,466 @@
+// blz_traceback.hip -- traceback stage: the gapped alignment with traceback of the prelim HSPs of
+// the (query, subject) pairs a caller lists (normally each query's hit list), one wave per pair.
+//
+// Per pair (NCBI BLAST+ 2.13.0 Blast_TracebackFromHSPList for blastp without composition-based
+// rescoring: the stage that consumes the lists BLAZE's prelim stage writes to the HSP stream at
+// BLAZE_ncbi/blast_engine.c:2499, which the reference leaves to NCBI's CPU code):
+//   the prelim HSPs in their score order; one contained in an HSP already traced (score no higher,
+//   both end points inside its box; NCBI s_HSPIsContained) is skipped; otherwise the alignment
+//   with traceback (BLAST_GappedAlignmentWithTraceback) runs ALIGN_EX left over the reversed
+//   prefixes (the gapped start included) and right over the suffixes with the final X-drop and
+//   reads each path back from its best cell.  An HSP scoring at least the cutoff is kept with its
+//   edit script (runs of one operation: an aligned pair, a subject letter against a gap, a query
+//   letter against a gap), identities, columns, gap letters and gap openings.  Then the
+//   common-endpoint purge and the score sort of the prelim stage.
+//
+// ALIGN_EX here is row-parallel: a row's band is spread over the 64 lanes, and the row gap and the
+// running best are prefix maxima across the lanes (as in the prelim stage's wave DP).  Each cell
+// stores the serial loop's traceback byte: the operation that gave the cell its score (the
+// diagonal unless a gap score is strictly greater, the row gap last) and, per gap direction,
+// whether the gap leaving the cell extends the one entering it.  The prefix-max form has the
+// serial loop's gap values at every cell the X-drop keeps, except where the serial loop carries a
+// gap past a dropped cell; such a gap lies below the X-drop bound, so it can neither give a kept
+// cell its score nor lie on the path read back, and every byte the path reads is the serial one.
+//
+// Two passes share the code: the main pass keeps the DP band ring in LDS and bounds each wave's
+// traceback rows and edit-script scratch; a pair outgrowing them goes to the wide pass, whose
+// waves have a global ring and storage sized to the largest such pair.
+#include <hip/hip_runtime.h>
+
+#include "blz_common.h"
+#include "blz_dev_common.h"
+#include "blz_device.h"
+
+namespace blz {
+namespace {
+
+// traceback byte of a cell
+enum : int { kTbSub = 0, kTbGapA = 1, kTbGapB = 2, kTbOpMask = 3, kTbExtA = 4, kTbExtB = 8 };
+constexpr int kNoCell = 0xFF;  // window byte outside the stored rows
+
+constexpr int kBWin = 2048;     // B letters of the band (LDS window over columns)
+constexpr int kAWin = 1024;     // A letters of the coming rows
+constexpr int kWinRows = 32, kWinCols = 32;  // path-walk window
+constexpr uint32_t kRunLen = 0x0FFFFFFFu;
+
+enum PairOutcome { kPairDone = 0, kPairBack = 1, kPairBad = 2 };
+
+struct TbLds {
+    int16_t mat[28][28];
+    uint8_t bwin[kBWin];
+    uint8_t awin[kAWin];
+    uint8_t win[kWinRows * kWinCols];
+};
+
+// Column b's letter of B: B[N - b] on reversed prefixes, else B[b] (column 0 has none).
+__device__ __forceinline__ void load_bwin(TbLds &L, const uint8_t *B, int N, bool reverse, int bw0) {
+    for (int i = lane_id(); i < kBWin; i += 64) {
+        const int b = bw0 + i;
+        L.bwin[i] = (b >= 1 && b <= N) ? (reverse ? B[N - b] : B[b]) : 0;
+    }
+}
+__device__ __forceinline__ void load_awin(TbLds &L, const uint8_t *A, int M, bool reverse, int aw0) {
+    for (int i = lane_id(); i < kAWin; i += 64) {
+        const int a = aw0 + i;
+        L.awin[i] = (a >= 1 && a <= M) ? (reverse ? A[M - a] : A[a]) : 0;
+    }
+}
+
+// ALIGN_EX of one direction by the whole wave (uniform arguments).  Rows walk A (a = 1..M), the
+// X-drop band walks B; row 0 is the gap run in A.  Row a's traceback bytes, columns
+// [rows[a].x, rows[a].x + rows[a].z), lie at tb[rows[a].y ...].  Returns the best score, its cell
+// in (*a_off, *b_off); sets *ovf when the band outgrows the ring or the rows outgrow tb_cap or
+// rows_cap (nothing usable is left then).
+__device__ int align_ex_wave(TbLds &L, int2 *ring, int wmask, const uint8_t *A, const uint8_t *B, int M, int N,
+                             int X, int go, int ge, bool reverse, uint8_t *tb, unsigned long long tb_cap, int4 *rows,
+                             int rows_cap, int *a_off, int *b_off, bool *ovf, unsigned long long *cells) {
+    const int lane = lane_id();
+    const int goe = go + ge;
+    *a_off = 0;
+    *b_off = 0;
+    if (X < goe) X = goe;
+    if (N <= 0 || M <= 0) return 0;
+    const int W = wmask + 1;
+    const int k0 = (X - goe) / ge + 1;
+    int size = (N < k0 ? N : k0) + 1;
+    if (M + 1 > rows_cap || size > W || (unsigned long long)size > tb_cap) {
+        *ovf = true;
+        return 0;
+    }
+    for (int b = lane; b < size; b += 64) {
+        const int h = b == 0 ? 0 : -goe - (b - 1) * ge;
+        ring[b & wmask] = make_int2(h, h - goe);
+        tb[b] = kTbGapA;
+    }
+    if (lane == 0) rows[0] = make_int4(0, 0, size, 0);
+    unsigned long long used = (unsigned long long)size;
+    int bw0 = 0, aw0 = 1;
+    wave_sync();
+    load_bwin(L, B, N, reverse, bw0);
+    load_awin(L, A, M, reverse, aw0);
+    wave_sync();
+    int best = 0, first = 0, ba = 0, bb = 0;
+    unsigned long long nc = 0;
+    for (int a = 1; a <= M; a++) {
+        if (a >= aw0 + kAWin) {
+            aw0 = a;
+            load_awin(L, A, M, reverse, aw0);
+            wave_sync();
+        }
+        const int16_t *mrow = L.mat[L.awin[a - aw0]];
+        const int rs = first;
+        if (used + (unsigned long long)(size - rs) > tb_cap) {
+            *ovf = true;
+            return 0;
+        }
+        uint8_t *trow = tb + ((long long)used - rs);  // trow[b] for b >= rs
+        int carry_g = kMinIntDp;   // max of max(D,E)(b') + b' * extend over earlier steps
+        int carry_h = best;        // running best over earlier steps
+        int carry_hp = kMinIntDp;  // previous row's H left of this step (none for the first cell)
+        int fa = -1, la = -1, rmax = kMinIntDp, rmax_b = -1;
+        for (int c0 = first; c0 < size; c0 += 64) {
+            const int c1 = min(c0 + 64, size);
+            if (c0 < bw0 || c1 > bw0 + kBWin) {  // the window moves to the row's band, or this step
+                bw0 = c1 - first <= kBWin ? first : c0;
+                load_bwin(L, B, N, reverse, bw0);
+                wave_sync();
+            }
+            const int b = c0 + lane;
+            const bool valid = b < size;
+            const int2 prev = valid ? ring[b & wmask] : make_int2(kMinIntDp, kMinIntDp);
+            const int hp_left = wave_shift_up1(prev.x, carry_hp);
+            const int sb = (valid && b >= 1) ? mrow[L.bwin[b - bw0]] : 0;
+            const int D = hp_left + sb;
+            const int E = prev.y;
+            const int de = valid ? max(D, E) : kMinIntDp;
+            const int g = valid ? de + b * ge : kMinIntDp;
+            const int gin = wave_inclusive_max(g);
+            const int gex = max(wave_shift_up1(gin, kMinIntDp), carry_g);
+            const int f = gex - goe - (b - 1) * ge;  // row gap entering the cell
+            const int h = valid ? max(de, f) : kMinIntDp;
+            const int hin = wave_inclusive_max(h);
+            const int hex = max(wave_shift_up1(hin, kMinIntDp), carry_h);
+            const bool alive = valid && hex - h <= X;
+            const uint64_t am = __ballot(alive);
+            if (am) {
+                if (fa < 0) fa = c0 + (int)__builtin_ctzll(am);
+                la = c0 + 63 - (int)__builtin_clzll(am);
+            }
+            const int cm = (int)last_lane((uint32_t)hin);
+            if (cm > carry_h) {
+                rmax = cm;
+                rmax_b = c0 + (int)__builtin_ctzll(__ballot(valid && h == cm));
+            }
+            if (valid) {
+                int t = f > de ? kTbGapA : (E > D ? kTbGapB : kTbSub);
+                if (alive) {
+                    if (E - ge >= h - goe) t |= kTbExtB;  // the column gap leaving the cell extends
+                    if (f - ge >= h - goe) t |= kTbExtA;  // the row gap leaving the cell extends
+                }
+                trow[b] = (uint8_t)t;
+                ring[b & wmask] = alive ? make_int2(h, max(h - goe, E - ge)) : make_int2(kMinIntDp, kMinIntDp);
+            }
+            carry_g = max(carry_g, (int)last_lane((uint32_t)gin));
+            carry_h = max(carry_h, cm);
+            carry_hp = (int)last_lane((uint32_t)prev.x);
+        }
+        nc += (unsigned long long)(size - first);
+        int row_end = size;
+        if (fa < 0) {
+            if (lane == 0) rows[a] = make_int4(rs, (int)used, row_end - rs, 0);
+            break;
+        }
+        if (rmax > best) {
+            best = rmax;
+            ba = a;
+            bb = rmax_b;
+        }
+        first = fa;
+        if (la < size - 1) {
+            size = la + 1;
+        } else {
+            const int gr = carry_g - goe - (size - 1) * ge;  // row gap one past the last cell
+            if (gr >= best - X) {
+                int k = (ge == 1 ? gr - (best - X) : (gr - (best - X)) / ge) + 1;
+                if (k > N + 1 - size) k = N + 1 - size;
+                if (size + k - first > W || used + (unsigned long long)(size + k - rs) > tb_cap) {
+                    *ovf = true;
+                    return 0;
+                }
+                for (int i = lane; i < k; i += 64) {
+                    const int hv = gr - i * ge;
+                    ring[(size + i) & wmask] = make_int2(hv, hv - goe);
+                    trow[size + i] = kTbGapA;
+                }
+                size += k;
+                row_end = size;
+            }
+        }
+        if (lane == 0) rows[a] = make_int4(rs, (int)used, row_end - rs, 0);
+        used += (unsigned long long)(row_end - rs);
+        if (size <= N) {
+            if (size + 1 - first > W) {
+                *ovf = true;
+                return 0;
+            }
+            if (lane == 0) ring[size & wmask] = make_int2(kMinIntDp, kMinIntDp);
+            size++;
+        }
+        wave_sync();
+    }
+    *a_off = ba;
+    *b_off = bb;
+    *cells += nc;
+    return best;
+}
+
+enum WalkOutcome { kWalkOk = 0, kWalkFull = 1, kWalkBad = 2 };
+
+// Reads the path back from cell (a, b) to the origin (NCBI ALIGN_EX's traceback: a cell reached
+// through a gap continues that gap while its extension bit for that direction is set, else takes
+// the cell's own operation) and appends its operations, as runs op << 28 | length in walk order,
+// to runs[*n, cap).  The rows come through an LDS window of kWinRows x kWinCols cells that the
+// wave fetches around the current cell whenever the walk leaves it.  Operation codes: the byte's
+// kTbGapA (a B letter against a gap in A) and kTbGapB (an A letter against a gap in B).
+__device__ int walk_path(TbLds &L, const uint8_t *tb, const int4 *rows, int a, int b, uint32_t *runs, int cap, int *n) {
+    const int lane = lane_id();
+    int cur = kTbSub, run_op = -1, run_len = 0, nr = *n;
+    int wa = -1, wb = -1;  // window: rows (wa - kWinRows, wa], columns (wb - kWinCols, wb]
+    while (a > 0 || b > 0) {
+        if (a > wa || a <= wa - kWinRows || b > wb || b <= wb - kWinCols) {
+            wa = a;
+            wb = b;
+            wave_sync();
+            const int r = wa - (lane >> 1);  // lane: one row, 16 of its columns
+            const int4 m = r >= 0 ? rows[r] : make_int4(0, 0, 0, 0);
+            uint8_t *w = &L.win[(lane >> 1) * kWinCols + (lane & 1) * 16];
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const int i = wb - (lane & 1) * 16 - j - m.x;
+                w[j] = (r >= 0 && i >= 0 && i < m.z) ? tb[(unsigned)m.y + (unsigned)i] : (uint8_t)kNoCell;
+            }
+            wave_sync();
+        }
+        const int next = __builtin_amdgcn_readfirstlane(L.win[(wa - a) * kWinCols + (wb - b)]);
+        if (next == kNoCell) return kWalkBad;
+        if (cur == kTbGapA)
+            cur = (next & kTbExtA) ? kTbGapA : (next & kTbOpMask);
+        else if (cur == kTbGapB)
+            cur = (next & kTbExtB) ? kTbGapB : (next & kTbOpMask);
+        else
+            cur = next & kTbOpMask;
+        if (cur == kTbGapA) {
+            b--;
+        } else if (cur == kTbGapB) {
+            a--;
+        } else {
+            a--;
+            b--;
+        }
+        if (a < 0 || b < 0 || cur == kTbOpMask) return kWalkBad;
+        if (cur == run_op) {
+            run_len++;
+        } else {
+            if (run_op >= 0) {
+                if (nr >= cap) return kWalkFull;
+                if (lane == 0) runs[nr] = ((uint32_t)run_op << 28) | (uint32_t)run_len;
+                nr++;
+            }
+            run_op = cur;
+            run_len = 1;
+        }
+    }
+    if (run_op >= 0) {
+        if (nr >= cap) return kWalkFull;
+        if (lane == 0) runs[nr] = ((uint32_t)run_op << 28) | (uint32_t)run_len;
+        nr++;
+    }
+    *n = nr;
+    return kWalkOk;
+}
+
+// Run i of the HSP's edit script left to right: the left walk's runs wr[0, nl) as they are, then
+// the right walk's wr[nl, nl + nr) backwards, the two runs at the junction merged when they hold
+// the same operation.
+__device__ __forceinline__ uint32_t final_run(const uint32_t *wr, int nl, int nr, bool merge, int i) {
+    if (i < nl) {
+        const uint32_t v = wr[i];
+        return (merge && i == nl - 1) ? v + (wr[nl + nr - 1] & kRunLen) : v;
+    }
+    return wr[nl + nr - 1 - (i - nl + (merge ? 1 : 0))];
+}
+
+__device__ __forceinline__ bool contained(const Hsp &h, const Hsp &o) {
+    return h.score <= o.score && o.q_off <= h.q_off && h.q_off <= o.q_end && o.s_off <= h.s_off &&
+           h.s_off <= o.s_end && o.q_off <= h.q_end && h.q_end <= o.q_end && o.s_off <= h.s_end && h.s_end <= o.s_end;
+}
+
+// The traceback of one pair by the whole wave.  Traced HSPs are built in out[0, n) (lane 0
+// writes), purged and sorted there; returns the outcome and the kept count in *kept.
+__device__ int trace_pair(const TbArgs &a, TbLds &L, int2 *ring, int wmask, uint8_t *tb, int4 *rows, uint32_t *wr,
+                          uint32_t pi, int *kept, unsigned long long *cells, unsigned long long *dps) {
+    const int lane = lane_id();
+    const TbPair p = a.pairs[pi];
+    const TbQuery Q = a.qs[p.qi];
+    const uint64_t s0 = a.offs[p.sid];
+    const uint8_t *s = a.res + s0;
+    const int slen = (int)(a.offs[p.sid + 1] - s0);
+    TbHsp *out = a.out + p.off;
+    const Hsp *pre = a.pre + p.off;
+    int nt = 0;
+    for (int k = 0; k < p.n; k++) {
+        const Hsp h = pre[k];
+        bool skip = false;
+        for (int t0 = 0; t0 < nt && !skip; t0 += 64) {
+            const int t = t0 + lane;
+            skip = __ballot(t < nt && contained(h, out[t].h)) != 0;
+        }
+        if (skip) continue;
+        bool ovf = false;
+        int la, lb, ra, rb;
+        const int left = align_ex_wave(L, ring, wmask, Q.q, s, h.q_gs + 1, h.s_gs + 1, a.x_final, a.gap_open,
+                                       a.gap_extend, true, tb, a.tb_cap, rows, a.rows_cap, &la, &lb, &ovf, cells);
+        if (ovf) return kPairBack;
+        int nl = 0;
+        int ws = walk_path(L, tb, rows, la, lb, wr, a.wruns_cap, &nl);
+        if (ws != kWalkOk) return ws == kWalkFull ? kPairBack : kPairBad;
+        const int right = align_ex_wave(L, ring, wmask, Q.q + h.q_gs, s + h.s_gs, Q.qlen - h.q_gs - 1,
+                                        slen - h.s_gs - 1, a.x_final, a.gap_open, a.gap_extend, false, tb, a.tb_cap,
+                                        rows, a.rows_cap, &ra, &rb, &ovf, cells);
+        if (ovf) return kPairBack;
+        *dps += 2;
+        const int score = left + right;
+        if (score < a.cutoff) continue;
+        int nall = nl;
+        ws = walk_path(L, tb, rows, ra, rb, wr, a.wruns_cap, &nall);
+        if (ws != kWalkOk) return ws == kWalkFull ? kPairBack : kPairBad;
+        wave_sync();  // lane 0 wrote wr[]
+        const int nr = nall - nl;
+        const bool merge = nl > 0 && nr > 0 && (wr[nl - 1] >> 28) == (wr[nl + nr - 1] >> 28);
+        const int cnt = nl + nr - (merge ? 1 : 0);
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(&a.ctr->runs_used, (unsigned long long)cnt);
+        base = first_lane64(base);
+        const bool fits = base + (unsigned long long)cnt <= a.runs_cap;
+        // columns, gaps and identities along the script; the script places the runs as it goes
+        const int q_off = h.q_gs + 1 - la, s_off = h.s_gs + 1 - lb;
+        int qi = q_off, si = s_off, ident = 0, columns = 0, gaps = 0, opens = 0;
+        for (int i = 0; i < cnt; i++) {
+            const uint32_t v = final_run(wr, nl, nr, merge, i);
+            if (fits && lane == 0) a.runs[base + i] = v;
+            const int op = (int)(v >> 28), len = (int)(v & kRunLen);
+            columns += len;
+            if (op == kTbSub) {
+                for (int c0 = 0; c0 < len; c0 += 64) {
+                    const int c = c0 + lane;
+                    ident += __popcll(__ballot(c < len && Q.q[qi + c] == s[si + c]));
+                }
+                qi += len;
+                si += len;
+            } else {
+                gaps += len;
+                opens++;
+                if (op == kTbGapA)
+                    si += len;
+                else
+                    qi += len;
+            }
+        }
+        if (qi != h.q_gs + 1 + ra || si != h.s_gs + 1 + rb) return kPairBad;
+        if (lane == 0) {
+            TbHsp o;
+            o.h = Hsp{score, q_off, h.q_gs + 1 + ra, s_off, h.s_gs + 1 + rb, h.q_gs, h.s_gs};
+            o.num_ident = ident;
+            o.align_len = columns;
+            o.gaps = gaps;
+            o.gap_opens = opens;
+            o.op_first = (int32_t)(fits ? base : 0);
+            o.op_count = cnt;
+            out[nt] = o;
+        }
+        nt++;
+        wave_sync();  // out[] is read by every lane in the containment test
+    }
+    // common-endpoint purge and score order (lane 0; lists are short)
+    int m = 0;
+    if (lane == 0 && nt > 0) {
+        insertion_sort<cmp_start>(out, nt);
+        for (int i = 0; i < nt; i++) {
+            if (m > 0 && out[m - 1].h.q_off == out[i].h.q_off && out[m - 1].h.s_off == out[i].h.s_off) continue;
+            out[m++] = out[i];
+        }
+        int m2 = 0;
+        insertion_sort<cmp_end>(out, m);
+        for (int i = 0; i < m; i++) {
+            if (m2 > 0 && out[m2 - 1].h.q_end == out[i].h.q_end && out[m2 - 1].h.s_end == out[i].h.s_end) continue;
+            out[m2++] = out[i];
+        }
+        m = m2;
+        insertion_sort<cmp_score>(out, m);
+    }
+    *kept = (int)first_lane((uint32_t)m);
+    return kPairDone;
+}
+
+template <bool kWide>
+__global__ void __launch_bounds__(64) k_traceback(TbArgs a) {
+    __shared__ TbLds L;
+    __shared__ int2 lring[kWide ? 1 : kTbLdsRing];
+    const int lane = lane_id();
+    for (int i = lane; i < 28 * 28; i += 64) L.mat[i / 28][i % 28] = kBlosum62Dev[i / 28][i % 28];
+    wave_sync();
+    const size_t w = blockIdx.x;
+    uint8_t *tb = a.tb + w * a.tb_cap;
+    int4 *rows = a.rows + w * (size_t)a.rows_cap;
+    uint32_t *wr = a.wruns + w * (size_t)a.wruns_cap;
+    int2 *ring = kWide ? a.ring + w * ((size_t)a.ring_mask + 1) : lring;
+    const int wmask = kWide ? a.ring_mask : kTbLdsRing - 1;
+    const uint32_t n_items = a.list ? a.n_list : a.n_pairs;
+    unsigned long long cells = 0, dps = 0;
+    for (;;) {
+        uint32_t k = 0;
+        if (lane == 0) k = atomicAdd(&a.ctr->next_pair, 1u);
+        k = first_lane(k);
+        if (k >= n_items) break;
+        const uint32_t pi = a.list ? a.list[k] : k;
+        int kept = 0;
+        const int r = trace_pair(a, L, ring, wmask, tb, rows, wr, pi, &kept, &cells, &dps);
+        if (lane == 0) {
+            if (r == kPairDone) {
+                a.out_n[pi] = kept;
+            } else if (r == kPairBack) {  // the wide pass counts the pairs it could not take either
+                a.out_n[pi] = -1;
+                const uint32_t j = atomicAdd(&a.ctr->back_count, 1u);
+                if (!kWide && j < a.back_cap) a.back[j] = pi;
+            } else {
+                a.out_n[pi] = -2;
+                atomicAdd(&a.ctr->bad_walk, 1u);
+            }
+        }
+        wave_sync();
+    }
+    if (lane == 0) {
+        atomicAdd(&a.ctr->cells, cells);
+        atomicAdd(&a.ctr->dps, dps);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_traceback(const TbArgs &a, int blocks, bool wide, hipStream_t st) {
+    if (blocks <= 0) return hipSuccess;
+    if (wide)
+        hipLaunchKernelGGL(k_traceback<true>, dim3(blocks), dim3(64), 0, st, a);
+    else
+        hipLaunchKernelGGL(k_traceback<false>, dim3(blocks), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+int traceback_blocks_per_cu(bool wide) {
+    int n = 0;
+    const hipError_t e = wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_traceback<true>, 64, 0)
+                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_traceback<false>, 64, 0);
+    return e == hipSuccess ? n : 0;
+}
+
+}  // namespace blz

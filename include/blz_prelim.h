@@ -156,6 +156,9 @@ typedef struct {
     int64_t cap_work;          /* (query, subject) work items of a run */
     int64_t cap_pool;          /* init hits of a run */
     int64_t cap_hsps;          /* HSPs of a run */
+    int64_t cap_tb_bytes;      /* traceback main pass: traceback bytes per wave (pairs needing
+                                  more go to the wide pass) */
+    int64_t cap_tb_runs;       /* traceback: edit-script runs a call starts with */
 } blz_tuning;
 
 blz_status blz_ctx_set_tuning(blz_ctx *ctx, const blz_tuning *t);
@@ -280,6 +283,72 @@ double blz_evalue(const blz_search_stats *s, int32_t score);
  * listed subject's best e-value.  BLZ_ECAPACITY (sizes reported) if the sink is too small. */
 blz_status blz_result_hitlist(const blz_ctx *ctx, int32_t qi, const blz_search_stats *s, blz_hsp_sink *sink,
                               double *best_evalue);
+
+/* ---- traceback stage ---------------------------------------------------------------------
+ * The gapped alignment with traceback of the prelim HSPs of listed (query, subject) pairs,
+ * normally each query's hit list (blz_result_hitlist): NCBI Blast_TracebackFromHSPList for blastp
+ * without composition-based rescoring (NCBI's -comp_based_stats 0), the stage downstream of the
+ * lists the reference's prelim stage writes (blast_engine.c:2499) and which the reference leaves
+ * to NCBI's CPU code.  Per pair, the HSPs in their score order: one contained in an HSP already
+ * traced (score no higher, both end points inside it) is skipped; otherwise ALIGN_EX runs left
+ * and right from its gapped start with the final X-drop and the path is read back.  HSPs scoring
+ * at least the cutoff are kept, then the common-endpoint purge and the score sort. */
+enum { BLZ_OP_SUB = 0, BLZ_OP_GAP_Q = 1, BLZ_OP_GAP_S = 2 };  /* aligned pair; subject letter
+                                      against a gap in the query; query letter against a gap */
+
+/* One traced HSP: the blz_hsp fields (score and extent of the final alignment, the gapped start
+ * it was anchored at), identical pairs, alignment columns, gap letters, gap openings, and its
+ * edit script, runs[op_first, op_first + op_count) left to right, each run op << 28 | length. */
+typedef struct {
+    int32_t score;
+    int32_t q_off, q_end;
+    int32_t s_off, s_end;
+    int32_t q_gs, s_gs;
+    int32_t num_ident, align_len, gaps, gap_opens;
+    int32_t op_first, op_count;
+} blz_tb_hsp;
+
+/* A pair to trace: query index into the call's query array, subject oid, and how many of the
+ * call's prelim HSPs (consecutive, pair after pair, each pair's in score order) are its. */
+typedef struct {
+    int32_t query;
+    int32_t n_hsps;
+    int64_t oid;
+} blz_tb_pair;
+
+typedef struct {
+    int32_t x_dropoff;   /* raw final X-drop, default 64 (25 bits; blz_search_stats.x_final) */
+    int32_t gap_open, gap_extend;  /* default 11, 1 */
+    int32_t cutoff;      /* HSPs scoring below it are dropped (0 keeps all; for the e-value
+                            threshold E, the lowest score whose blz_evalue is within E) */
+} blz_tb_params;
+
+typedef struct {
+    uint64_t pairs, prelim_hsps;
+    uint64_t hsps;        /* traced HSPs returned */
+    uint64_t alignments;  /* one-direction ALIGN_EX runs */
+    uint64_t dp_cells;    /* cells inside the X-drop bands */
+    uint64_t runs;        /* edit-script runs returned */
+    uint64_t wide_pairs;  /* pairs whose band or storage outgrew the main pass */
+    int32_t reruns;       /* calls repeated because the edit-script buffer filled */
+    int32_t reserved;
+    double ms_main, ms_wide, ms_total;  /* device time of the passes; wall time of the call */
+} blz_tb_stats;
+
+void blz_tb_default_params(blz_tb_params *p);
+/* Trace n_pairs pairs against the resident database; queries[p.query] are prepared queries of
+ * this device.  The result stays in the context until its next traceback call
+ * (blz_traceback_sizes / blz_traceback_copy).  BLZ_EINVAL for a pair or HSP outside its query or
+ * subject; BLZ_EOVERFLOW if a pair's alignment outgrows even the wide pass (2^31 traceback cells
+ * in one direction). */
+blz_status blz_traceback(blz_ctx *ctx, const blz_db *db, const blz_query *const *queries, int32_t nq,
+                         const blz_tb_pair *pairs, int64_t n_pairs, const blz_hsp *hsps, const blz_tb_params *p,
+                         blz_tb_stats *stats);
+blz_status blz_traceback_sizes(const blz_ctx *ctx, int64_t *n_pairs, int64_t *n_hsps, int64_t *n_runs);
+/* counts[n_pairs]: traced HSPs per pair; hsps: pair after pair, score order; runs: the edit
+ * scripts (op_first indexes runs).  BLZ_ECAPACITY if a capacity is short. */
+blz_status blz_traceback_copy(const blz_ctx *ctx, int32_t *counts, int64_t n_pairs, blz_tb_hsp *hsps,
+                              int64_t hsp_cap, uint32_t *runs, int64_t runs_cap);
 
 /* ---- serving: a request loop over a resident database ------------------------------
  * No reference counterpart (the reference is a per-search hook, blast_engine.c:2983-2996).
