@@ -9,6 +9,10 @@
 --workload stress: 64 queries of 1,000-2,000 aa against 500,000 long subjects (median 800, up
   to 35,000) with 20% homologs at 30-45% identity and low-complexity repeat blocks: GCUPS of the
   semi-gapped stage, and bit-exact lists on a sample against the CPU restatement.
+--workload traceback: 256 queries (log-uniform 50-3,000 aa) with ~500 homologs each among 2,000,000
+  subjects; their hit lists (expect 10, 500 subjects each) traced (gapped alignment with
+  traceback, edit scripts back on the host) in one call per step: pairs/s, traceback GCUPS, a
+  sample bit-exact against the CPU restatement.
 --workload serve: Poisson query streams (lengths log-normal, median 350, sigma 0.8, clipped
   30-4,000) at several rates against a resident DB, through the serving loop with continuous
   batching, and through the same loop with one query per launch: p50/p99 latency, residues/s.
@@ -142,7 +146,7 @@ def copy_calibration(nbytes: int) -> int:
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="scan", choices=["scan", "batch", "stress", "serve"])
+    ap.add_argument("--workload", default="scan", choices=["scan", "batch", "stress", "serve", "traceback"])
     ap.add_argument("--rates", default="20,100,200", help="serve: arrival rates, queries/s")
     ap.add_argument("--duration", type=float, default=4.0, help="serve: seconds of arrivals per rate")
     ap.add_argument("--autotune", action="store_true", help="batch: tune on a sample batch first")
@@ -172,6 +176,8 @@ def main() -> int:
         return run_stress(args)
     if args.workload == "serve":
         return run_serve(args)
+    if args.workload == "traceback":
+        return run_traceback(args)
     return run_scan(args)
 
 
@@ -465,6 +471,160 @@ def run_stress(args) -> int:
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+TB_SOURCES = ["blz_traceback.hip", "blz_dev_common.h", "blz_device.h", "blz_common.h"]
+
+
+def tb_source_hash() -> str:
+    import hashlib
+
+    h = hashlib.sha256()
+    for name in TB_SOURCES:
+        h.update((ROOT / "blaze-sc-ad_amd" / "csrc" / name).read_bytes())
+    return h.hexdigest()[:16]
+
+
+def tb_roofline(kernel_ms: float, cells: int, pairs: int) -> dict:
+    """k_traceback's roofline: VALU instruction issue, priced with the instruction count of its
+    PMC profile on this workload and build (profiles/k_traceback_issue.json), when one matches."""
+    peak, basis = valu_peak()
+    out = {"kernel": "k_traceback", "bound": "valu", "unit": "G VALU wave-instructions/s", "achieved": None,
+           "peak": peak, "peak_basis": basis, "frac": None, "traffic": None, "cells": cells, "pairs": pairs}
+    prof_path = ROOT / "profiles" / "k_traceback_issue.json"
+    if prof_path.exists():
+        prof = json.loads(prof_path.read_text())
+        if prof.get("source_sha") == tb_source_hash() and prof.get("pairs") == pairs:
+            ach = prof["valu_insts_per_call"] / (kernel_ms * 1e-3) / 1e9
+            out.update(achieved=ach, frac=ach / peak, valu_insts_per_call=prof["valu_insts_per_call"],
+                       valu_per_cell=prof["valu_insts_per_call"] / max(1, cells), profile=prof.get("source"),
+                       traffic=prof.get("bytes_per_call"))
+    return out
+
+
+def run_traceback(args) -> int:
+    """Traceback of 256 queries' hit lists: queries log-uniform 50-3,000 aa, each with about 500
+    planted homologs (35-70% identity) among 2,000,000 subjects; the prelim batch runs once with
+    each query's e-value reap, then per step one blz_traceback call traces every query's hit list
+    (expect 10, 500 subjects, no composition-based statistics) with the query's own cutoff and the
+    final X-drop (25 bits)."""
+    nq = args.queries or 256
+    rng = np.random.default_rng(args.seed)
+    lens = np.exp(rng.uniform(np.log(50), np.log(3000), nq)).astype(int)
+    queries = [blz.synth_query(1000 + i, int(L)) for i, L in enumerate(lens)]
+    t0 = time.perf_counter()
+    spec = blz.synth_spec(seed=args.seed, n_subjects=args.subjects, len_median=300, len_sigma=0.8, len_min=30,
+                          len_max=5000, homolog_frac=min(1.0, 500.0 * nq / args.subjects), ident_min=0.35,
+                          ident_max=0.70)
+    res, offs, hom = blz.synth_db(spec, queries)
+    n_sub, n_res = len(offs) - 1, int(offs[-1])
+    log(f"workload built: {nq} queries, {n_sub} subjects, {int((hom >= 0).sum())} homologs "
+        f"({time.perf_counter() - t0:.1f} s)")
+    opts = blz.stats_options(expect=10.0, composition_based_stats=0)
+    stats, params = [], []
+    for q in queries:
+        p, ss = blz.search_setup(q, n_res, n_sub, opts)
+        params.append(p)
+        stats.append(ss)
+    with blz.Engine(0) as eng:
+        if args.tune:
+            eng.set_tuning(**json.loads(args.tune))
+        eng.upload_db(res, offs)
+        qhs = [eng.prepare(q, p) for q, p in zip(queries, params)]
+        eng.run_batch(qhs, fetch=False)
+        lists = [eng.hitlist(i, ss)[0] for i, ss in enumerate(stats)]
+        cutoffs = [s.reap_score for s in stats]
+        tbp = blz.tb_params(x_dropoff=stats[0].x_final)
+        n_pairs = sum(len(r.oids) for r in lists)
+        n_pre = sum(len(r.hsps) for r in lists)
+        log(f"hit lists: {n_pairs} pairs, {n_pre} prelim HSPs; final X-drop {tbp.x_dropoff}")
+        for i in range(args.warmup):
+            got, st = eng.traceback(qhs, lists, tbp, cutoffs)
+        calls, kms, wrapper = [], [], []
+        for i in range(args.steps):
+            t = time.perf_counter()
+            got, st = eng.traceback(qhs, lists, tbp, cutoffs)
+            wrapper.append(time.perf_counter() - t)
+            calls.append(st["ms_total"])
+            kms.append(st["ms_main"] + st["ms_wide"])
+            log(f"step {i}: {st['ms_total']:.1f} ms call ({1000 * wrapper[-1]:.1f} ms with the Python wrapper), "
+                f"{kms[-1]:.2f} ms kernels ({st['ms_main']:.2f} main + {st['ms_wide']:.2f} wide, {st['wide_pairs']} "
+                f"wide pairs), {st['hsps']} HSPs, {st['dp_cells']} cells, {st['runs']} runs")
+        for qh in qhs:
+            eng.free_query(qh)
+    call_ms = float(np.median(calls))
+    kernel_ms = float(np.median(kms))
+    cpu, exact = None, None
+    if not args.no_cpu:
+        cpu, exact = tb_cpu_baseline(queries, res, offs, lists, got, tbp, cutoffs, args.cpu_budget)
+        log(f"cpu baseline {cpu['value']:.1f} pairs/s on {cpu['cores']} threads; sample bit-exact: {exact}")
+    out = {
+        "metric": "traceback pairs/s (gapped alignment with traceback and edit scripts, bit-exact)",
+        "value": n_pairs / (call_ms * 1e-3), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": call_ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"hit lists (expect 10, 500 subjects, no composition-based statistics) of {nq} "
+                               f"queries, log-uniform 50-3,000 aa (seed {args.seed}), each with ~500 planted homologs "
+                               f"at 35-70% identity among {n_sub} subjects; one blz_traceback call per step (pairs "
+                               "and prelim HSPs up, edit scripts back on the host)",
+                   "queries": nq, "pairs": n_pairs, "prelim_hsps": n_pre, "x_dropoff": tbp.x_dropoff},
+        "step": "wall time of the blz_traceback C call (upload, kernels, download, compaction); "
+                "python_wrapper_ms adds the ctypes wrapper's per-pair dicts",
+        "python_wrapper_ms": 1000 * float(np.median(wrapper)),
+        "traced_hsps": st["hsps"], "dp_cells": st["dp_cells"], "runs": st["runs"], "kernel_ms": kernel_ms,
+        "gcups": st["dp_cells"] / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else None,
+        "wide_pairs": st["wide_pairs"], "bit_exact_sample": exact,
+        "roofline": tb_roofline(kernel_ms, st["dp_cells"], n_pairs), "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def tb_cpu_baseline(queries, res, offs, lists, got, params, cutoffs, budget_s: float, runs: int = 5):
+    """The oracle's traceback (oracle/traceback.c) on a stride sample of the same pairs, on all the
+    host's cores and on one thread (median of `runs` after a warm-up each); the sample's GPU
+    results are checked against it."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from concurrent.futures import ThreadPoolExecutor
+
+    import oracle
+
+    host = host_cpu()
+    threads = host["nproc"]
+    jobs = [(qi, oid, hs) for qi, r in enumerate(lists) for oid, hs in r.as_dict().items()]
+
+    def one(job):
+        qi, oid, hs = job
+        s = res[int(offs[oid]):int(offs[oid + 1])]
+        return oracle.traceback_subject(queries[qi], s, hs, x_final=params.x_dropoff, cutoff=cutoffs[qi])
+
+    def run(sample, th):
+        with ThreadPoolExecutor(th) as ex:
+            return list(ex.map(one, sample))
+
+    t = time.perf_counter()
+    probe = jobs[:: max(1, len(jobs) // 200)]
+    run(probe, 1)
+    rate1 = len(probe) / max(1e-6, time.perf_counter() - t)
+    per_run = budget_s / 2 / (runs + 1)
+    s1 = jobs[:: max(1, int(len(jobs) / max(1.0, rate1 * per_run)))]
+    run(s1, 1)
+    t1, _ = _timed_runs(lambda: run(s1, 1), runs)
+    sa = jobs[:: max(1, int(len(jobs) / max(1.0, rate1 * threads * 0.8 * per_run)))]
+    run(sa, threads)
+    ta, outs = _timed_runs(lambda: run(sa, threads), runs)
+    exact = True
+    for (qi, oid, _), (wt, wr) in zip(sa, outs):
+        gt, gr = got[qi][oid]
+        exact &= gt.tolist() == wt.tolist() and gr.tolist() == wr.tolist()
+    return {
+        "value": len(sa) / ta, "unit": "pairs/s", "cores": threads, "kind": "port", "cpu_model": host["model"],
+        "nproc": host["nproc"], "runs": runs,
+        "sample": f"every {len(jobs) // max(1, len(sa))}th of the {len(jobs)} pairs ({len(sa)} pairs), median "
+                  f"{ta:.2f} s of {runs} runs on {threads} threads; CPU restatement of NCBI's traceback (oracle/)",
+        "single_thread": {"value": len(s1) / t1, "unit": "pairs/s", "cores": 1,
+                          "sample": f"{len(s1)} pairs, median {t1:.2f} s of {runs} runs"},
+    }, exact
 
 
 def _play(srv, queries, arrivals):

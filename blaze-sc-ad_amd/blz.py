@@ -260,7 +260,8 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.blz_db_upload_blastdb.argtypes = [vp, vp, i32, C.POINTER(vp), vp]
     lib.blz_db_upload_blastdb.restype = C.c_int
     lib.blz_tb_default_params.argtypes = [C.POINTER(TbParams)]
-    lib.blz_traceback.argtypes = [vp, vp, C.POINTER(vp), i32, vp, i64, vp, C.POINTER(TbParams), C.POINTER(TbStats)]
+    lib.blz_traceback.argtypes = [vp, vp, C.POINTER(vp), i32, vp, vp, i64, vp, C.POINTER(TbParams),
+                                  C.POINTER(TbStats)]
     lib.blz_traceback.restype = C.c_int
     lib.blz_traceback_sizes.argtypes = [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]
     lib.blz_traceback_sizes.restype = C.c_int
@@ -601,10 +602,11 @@ class Engine:
             self._err(st, "blz_prelim_run_batch")
         return ([self._fetch(i) for i in range(n)] if fetch else None), stats.as_dict()
 
-    def traceback(self, qhs, lists, params: TbParams | None = None):
+    def traceback(self, qhs, lists, params: TbParams | None = None, cutoffs=None):
         """Traceback of the listed pairs: lists[i] is a Result (e.g. a hit list) of prepared query
-        qhs[i].  Returns, per query, {oid: (traced HSPs as TB_DTYPE in score order, their runs)}
-        (op_first indexing that pair's runs), and the call's stats."""
+        qhs[i]; cutoffs[i] (optional) its score cutoff in place of params.cutoff.  Returns, per
+        query, {oid: (traced HSPs as TB_DTYPE in score order, their runs)} (op_first indexing that
+        pair's runs), and the call's stats."""
         nq = len(qhs)
         pairs, pre = [], []
         for qi, r in enumerate(lists):
@@ -616,8 +618,9 @@ class Engine:
         arr = (C.c_void_p * max(1, nq))(*[q.value if isinstance(q, C.c_void_p) else q for q in qhs])
         p = params if params is not None else tb_params()
         stats = TbStats()
-        st = self.lib.blz_traceback(self.ctx, self.db, arr, nq, pa.ctypes.data, len(pa), hs.ctypes.data, C.byref(p),
-                                    C.byref(stats))
+        cut = None if cutoffs is None else np.ascontiguousarray(cutoffs, dtype=np.int32)
+        st = self.lib.blz_traceback(self.ctx, self.db, arr, nq, None if cut is None else cut.ctypes.data,
+                                    pa.ctypes.data, len(pa), hs.ctypes.data, C.byref(p), C.byref(stats))
         if st:
             self._err(st, "blz_traceback")
         n_p, n_h, n_r = C.c_int64(), C.c_int64(), C.c_int64()

@@ -156,7 +156,8 @@ struct TbPair {
 };
 struct TbQuery {
     const uint8_t *q;
-    int32_t qlen, pad;
+    int32_t qlen;
+    int32_t cutoff;  // traced HSPs scoring below it are dropped
 };
 struct TbCounters {
     unsigned long long runs_used;  // edit-script runs reserved (may pass runs_cap: then the call runs again)
@@ -192,7 +193,7 @@ struct TbArgs {
     int32_t wruns_cap;
     int2 *ring;                // wide pass: DP band ring, ring_mask + 1 entries per wave
     int32_t ring_mask;
-    int32_t x_final, gap_open, gap_extend, cutoff;
+    int32_t x_final, gap_open, gap_extend;
 };
 constexpr int kTbLdsRing = 1024;  // main pass: DP band ring per wave in LDS; wider bands go to the wide pass
 hipError_t launch_traceback(const TbArgs &a, int blocks, bool wide, hipStream_t st);

@@ -108,8 +108,8 @@ void blz_tb_default_params(blz_tb_params *p) {
 }
 
 blz_status blz_traceback(blz_ctx *ctx, const blz_db *db, const blz_query *const *queries, int32_t nq,
-                         const blz_tb_pair *pairs, int64_t n_pairs, const blz_hsp *hsps, const blz_tb_params *p,
-                         blz_tb_stats *stats) {
+                         const int32_t *query_cutoffs, const blz_tb_pair *pairs, int64_t n_pairs,
+                         const blz_hsp *hsps, const blz_tb_params *p, blz_tb_stats *stats) {
     if (!ctx || !db || !p || n_pairs < 0 || nq < 0 || (n_pairs > 0 && (!pairs || !queries)) ||
         n_pairs > (int64_t)0xFFFFFFF0 || db->ctx == nullptr || db->ctx->device != ctx->device)
         return BLZ_EINVAL;
@@ -126,7 +126,7 @@ blz_status blz_traceback(blz_ctx *ctx, const blz_db *db, const blz_query *const 
     for (int i = 0; i < nq; i++) {
         const blz_query *q = queries[i];
         if (!q || !q->t.d_query || q->ctx == nullptr || q->ctx->device != ctx->device) return BLZ_EINVAL;
-        qh[i] = TbQuery{q->t.d_query, q->qlen, 0};
+        qh[i] = TbQuery{q->t.d_query, q->qlen, query_cutoffs ? query_cutoffs[i] : p->cutoff};
         max_qlen = std::max(max_qlen, q->qlen);
     }
     std::vector<TbPair> ph((size_t)n_pairs);
@@ -199,7 +199,6 @@ blz_status blz_traceback(blz_ctx *ctx, const blz_db *db, const blz_query *const 
     a.x_final = p->x_dropoff;
     a.gap_open = p->gap_open;
     a.gap_extend = p->gap_extend;
-    a.cutoff = p->cutoff;
     size_t runs_need =
         ctx->tune.cap_tb_runs > 0 ? (size_t)ctx->tune.cap_tb_runs : std::max<size_t>(1u << 20, 32 * total);
     TbCounters hc{};

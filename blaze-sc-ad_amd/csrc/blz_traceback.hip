@@ -330,7 +330,7 @@ __device__ int trace_pair(const TbArgs &a, TbLds &L, int2 *ring, int wmask, uint
         if (ovf) return kPairBack;
         *dps += 2;
         const int score = left + right;
-        if (score < a.cutoff) continue;
+        if (score < Q.cutoff) continue;
         int nall = nl;
         ws = walk_path(L, tb, rows, ra, rb, wr, a.wruns_cap, &nall);
         if (ws != kWalkOk) return ws == kWalkFull ? kPairBack : kPairBad;

@@ -337,13 +337,14 @@ typedef struct {
 
 void blz_tb_default_params(blz_tb_params *p);
 /* Trace n_pairs pairs against the resident database; queries[p.query] are prepared queries of
- * this device.  The result stays in the context until its next traceback call
- * (blz_traceback_sizes / blz_traceback_copy).  BLZ_EINVAL for a pair or HSP outside its query or
- * subject; BLZ_EOVERFLOW if a pair's alignment outgrows even the wide pass (2^31 traceback cells
- * in one direction). */
+ * this device, query_cutoffs[nq] (optional) their score cutoffs in place of p->cutoff (NCBI's
+ * cutoff is per query: its e-value threshold over the query's search space).  The result stays in
+ * the context until its next traceback call (blz_traceback_sizes / blz_traceback_copy).
+ * BLZ_EINVAL for a pair or HSP outside its query or subject; BLZ_EOVERFLOW if a pair's alignment
+ * outgrows even the wide pass (2^31 traceback cells in one direction). */
 blz_status blz_traceback(blz_ctx *ctx, const blz_db *db, const blz_query *const *queries, int32_t nq,
-                         const blz_tb_pair *pairs, int64_t n_pairs, const blz_hsp *hsps, const blz_tb_params *p,
-                         blz_tb_stats *stats);
+                         const int32_t *query_cutoffs, const blz_tb_pair *pairs, int64_t n_pairs,
+                         const blz_hsp *hsps, const blz_tb_params *p, blz_tb_stats *stats);
 blz_status blz_traceback_sizes(const blz_ctx *ctx, int64_t *n_pairs, int64_t *n_hsps, int64_t *n_runs);
 /* counts[n_pairs]: traced HSPs per pair; hsps: pair after pair, score order; runs: the edit
  * scripts (op_first indexes runs).  BLZ_ECAPACITY if a capacity is short. */

@@ -100,7 +100,7 @@ def test_config0_hitlist_equals_oracle():
         eng.upload_db(res, offs)
         hs, lists, stats = prelim_and_hitlists(eng, [q], res, offs)
         x_final, cutoff = stats[0].x_final, stats[0].reap_score
-        got, st = eng.traceback(hs, lists, blz.tb_params(x_dropoff=x_final, cutoff=cutoff))
+        got, st = eng.traceback(hs, lists, blz.tb_params(x_dropoff=x_final), cutoffs=[cutoff])
         for h in hs:
             eng.free_query(h)
     log(f"config 0: {st['pairs']} pairs, {st['hsps']} traced HSPs, {st['dp_cells']} cells, "
