@@ -604,23 +604,25 @@ class Engine:
 
     def traceback(self, qhs, lists, params: TbParams | None = None, cutoffs=None):
         """Traceback of the listed pairs: lists[i] is a Result (e.g. a hit list) of prepared query
-        qhs[i]; cutoffs[i] (optional) its score cutoff in place of params.cutoff.  Returns, per
-        query, {oid: (traced HSPs as TB_DTYPE in score order, their runs)} (op_first indexing that
-        pair's runs), and the call's stats."""
+        qhs[i]; cutoffs[i] (optional) its score cutoff in place of params.cutoff.  Returns a
+        TbResult (per query, {oid: (traced HSPs as TB_DTYPE in score order, their runs)}, op_first
+        indexing that pair's runs) and the call's stats."""
         nq = len(qhs)
-        pairs, pre = [], []
-        for qi, r in enumerate(lists):
-            for oid, c in zip(np.asarray(r.oids).tolist(), np.asarray(r.counts).tolist()):
-                pairs.append((qi, c, oid))
-            pre.append(np.asarray(r.hsps, dtype=HSP_DTYPE))
-        pa = np.array(pairs, dtype=TB_PAIR_DTYPE) if pairs else np.empty(0, TB_PAIR_DTYPE)
-        hs = np.ascontiguousarray(np.concatenate(pre) if pre else np.empty(0, HSP_DTYPE))
+        per = [len(r.oids) for r in lists]
+        n = int(sum(per))
+        pa = np.empty(n, TB_PAIR_DTYPE)
+        if n:
+            pa["query"] = np.repeat(np.arange(len(lists), dtype=np.int32), per)
+            pa["n_hsps"] = np.concatenate([np.asarray(r.counts, np.int32) for r in lists])
+            pa["oid"] = np.concatenate([np.asarray(r.oids, np.int64) for r in lists])
+        hs = np.ascontiguousarray(np.concatenate([np.asarray(r.hsps, dtype=HSP_DTYPE) for r in lists])
+                                  if lists else np.empty(0, HSP_DTYPE))
         arr = (C.c_void_p * max(1, nq))(*[q.value if isinstance(q, C.c_void_p) else q for q in qhs])
         p = params if params is not None else tb_params()
         stats = TbStats()
         cut = None if cutoffs is None else np.ascontiguousarray(cutoffs, dtype=np.int32)
         st = self.lib.blz_traceback(self.ctx, self.db, arr, nq, None if cut is None else cut.ctypes.data,
-                                    pa.ctypes.data, len(pa), hs.ctypes.data, C.byref(p), C.byref(stats))
+                                    pa.ctypes.data, n, hs.ctypes.data, C.byref(p), C.byref(stats))
         if st:
             self._err(st, "blz_traceback")
         n_p, n_h, n_r = C.c_int64(), C.c_int64(), C.c_int64()
@@ -632,19 +634,57 @@ class Engine:
                                          runs.ctypes.data, n_r.value)
         if st:
             self._err(st, "blz_traceback_copy")
-        out = [dict() for _ in range(nq)]
-        o = 0
-        for (qi, _, oid), c in zip(pairs, counts[: n_p.value].tolist()):
-            t = tb[o:o + c].copy()
-            o += c
-            if c:
-                r0 = int(t["op_first"][0])
-                r1 = int(t["op_first"][-1] + t["op_count"][-1])
-                t["op_first"] -= r0
-                out[qi][oid] = (t, runs[r0:r1].copy())
-            else:
-                out[qi][oid] = (t, np.empty(0, np.uint32))
-        return out, stats.as_dict()
+        return TbResult(nq, pa, counts[: n_p.value], tb[: n_h.value], runs[: n_r.value]), stats.as_dict()
+
+class TbResult:
+    """Result of one traceback call: pair i (in call order) has counts[i] traced HSPs; result[qi]
+    maps a subject oid of query qi to (its traced HSPs, their runs) with op_first rebased."""
+
+    def __init__(self, nq, pairs, counts, tb, runs):
+        self.pairs, self.counts, self.tb, self.runs = pairs, counts, tb, runs
+        self.first = np.concatenate([[0], np.cumsum(counts, dtype=np.int64)])
+        self._views = [None] * nq
+
+    def pair(self, i: int):
+        t = self.tb[self.first[i]:self.first[i + 1]].copy()
+        if len(t) == 0:
+            return t, np.empty(0, np.uint32)
+        r0 = int(t["op_first"][0])
+        r1 = int(t["op_first"][-1] + t["op_count"][-1])
+        t["op_first"] -= r0
+        return t, self.runs[r0:r1].copy()
+
+    def __len__(self):
+        return len(self._views)
+
+    def __getitem__(self, qi: int) -> "TbQueryView":
+        if self._views[qi] is None:
+            idx = np.nonzero(self.pairs["query"] == qi)[0]
+            self._views[qi] = TbQueryView(self, dict(zip(self.pairs["oid"][idx].tolist(), idx.tolist())))
+        return self._views[qi]
+
+
+class TbQueryView:
+    def __init__(self, res: TbResult, index: dict):
+        self.res, self.index = res, index
+
+    def __getitem__(self, oid):
+        return self.res.pair(self.index[oid])
+
+    def __contains__(self, oid):
+        return oid in self.index
+
+    def __len__(self):
+        return len(self.index)
+
+    def __iter__(self):
+        return iter(self.index)
+
+    def keys(self):
+        return self.index.keys()
+
+    def items(self):
+        return ((oid, self.res.pair(i)) for oid, i in self.index.items())
 
 
 def tb_params(**kw) -> TbParams:

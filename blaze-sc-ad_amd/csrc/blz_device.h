@@ -195,7 +195,7 @@ struct TbArgs {
     int32_t ring_mask;
     int32_t x_final, gap_open, gap_extend;
 };
-constexpr int kTbLdsRing = 1024;  // main pass: DP band ring per wave in LDS; wider bands go to the wide pass
+constexpr int kTbLdsRing = 256;   // main pass: DP band ring per wave in LDS; wider bands go to the wide pass
 hipError_t launch_traceback(const TbArgs &a, int blocks, bool wide, hipStream_t st);
 int traceback_blocks_per_cu(bool wide);
 

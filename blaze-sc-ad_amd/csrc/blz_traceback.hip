@@ -38,12 +38,22 @@ namespace {
 enum : int { kTbSub = 0, kTbGapA = 1, kTbGapB = 2, kTbOpMask = 3, kTbExtA = 4, kTbExtB = 8 };
 constexpr int kNoCell = 0xFF;  // window byte outside the stored rows
 
-constexpr int kBWin = 2048;     // B letters of the band (LDS window over columns)
-constexpr int kAWin = 1024;     // A letters of the coming rows
-constexpr int kWinRows = 32, kWinCols = 32;  // path-walk window
+constexpr int kBWin = 512;      // B letters of the band (LDS window over columns)
+constexpr int kAWin = 256;      // A letters of the coming rows
+constexpr int kWinRows = 16, kWinCols = 32;  // path-walk window
+constexpr int kWinLanesPerRow = 64 / kWinRows, kWinBytesPerLane = kWinCols / kWinLanesPerRow;
+static_assert(kBWin >= kTbLdsRing + 64, "a row's band and its last step fit the letter window");
 constexpr uint32_t kRunLen = 0x0FFFFFFFu;
 
 enum PairOutcome { kPairDone = 0, kPairBack = 1, kPairBad = 2 };
+
+// Values every lane holds alike, moved to scalar registers: loads through plain pointers land in
+// vector registers, and control flow on them would otherwise be compiled as divergent.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+template <class T>
+__device__ __forceinline__ T *uni_ptr(T *p) {
+    return reinterpret_cast<T *>(first_lane64(reinterpret_cast<uint64_t>(p)));
+}
 
 struct TbLds {
     int16_t mat[28][28];
@@ -76,6 +86,7 @@ __device__ int align_ex_wave(TbLds &L, int2 *ring, int wmask, const uint8_t *A, 
                              int rows_cap, int *a_off, int *b_off, bool *ovf, unsigned long long *cells) {
     const int lane = lane_id();
     const int goe = go + ge;
+    const int lane_ge = lane * ge;
     *a_off = 0;
     *b_off = 0;
     if (X < goe) X = goe;
@@ -102,6 +113,10 @@ __device__ int align_ex_wave(TbLds &L, int2 *ring, int wmask, const uint8_t *A, 
     int best = 0, first = 0, ba = 0, bb = 0;
     unsigned long long nc = 0;
     for (int a = 1; a <= M; a++) {
+        first = uni(first);
+        size = uni(size);
+        bw0 = uni(bw0);
+        best = uni(best);
         if (a >= aw0 + kAWin) {
             aw0 = a;
             load_awin(L, A, M, reverse, aw0);
@@ -127,16 +142,17 @@ __device__ int align_ex_wave(TbLds &L, int2 *ring, int wmask, const uint8_t *A, 
             }
             const int b = c0 + lane;
             const bool valid = b < size;
+            const int bge = c0 * ge + lane_ge;  // b * extend
             const int2 prev = valid ? ring[b & wmask] : make_int2(kMinIntDp, kMinIntDp);
             const int hp_left = wave_shift_up1(prev.x, carry_hp);
             const int sb = (valid && b >= 1) ? mrow[L.bwin[b - bw0]] : 0;
             const int D = hp_left + sb;
             const int E = prev.y;
             const int de = valid ? max(D, E) : kMinIntDp;
-            const int g = valid ? de + b * ge : kMinIntDp;
+            const int g = valid ? de + bge : kMinIntDp;
             const int gin = wave_inclusive_max(g);
             const int gex = max(wave_shift_up1(gin, kMinIntDp), carry_g);
-            const int f = gex - goe - (b - 1) * ge;  // row gap entering the cell
+            const int f = gex - go - bge;  // row gap entering the cell: gex - open - extend * (b - 1) - extend
             const int h = valid ? max(de, f) : kMinIntDp;
             const int hin = wave_inclusive_max(h);
             const int hex = max(wave_shift_up1(hin, kMinIntDp), carry_h);
@@ -231,12 +247,13 @@ __device__ int walk_path(TbLds &L, const uint8_t *tb, const int4 *rows, int a, i
             wa = a;
             wb = b;
             wave_sync();
-            const int r = wa - (lane >> 1);  // lane: one row, 16 of its columns
+            const int wr = lane / kWinLanesPerRow, wc = (lane % kWinLanesPerRow) * kWinBytesPerLane;
+            const int r = wa - wr;  // lane: one row, kWinBytesPerLane of its columns
             const int4 m = r >= 0 ? rows[r] : make_int4(0, 0, 0, 0);
-            uint8_t *w = &L.win[(lane >> 1) * kWinCols + (lane & 1) * 16];
+            uint8_t *w = &L.win[wr * kWinCols + wc];
 #pragma unroll
-            for (int j = 0; j < 16; j++) {
-                const int i = wb - (lane & 1) * 16 - j - m.x;
+            for (int j = 0; j < kWinBytesPerLane; j++) {
+                const int i = wb - wc - j - m.x;
                 w[j] = (r >= 0 && i >= 0 && i < m.z) ? tb[(unsigned)m.y + (unsigned)i] : (uint8_t)kNoCell;
             }
             wave_sync();
@@ -300,16 +317,23 @@ __device__ __forceinline__ bool contained(const Hsp &h, const Hsp &o) {
 __device__ int trace_pair(const TbArgs &a, TbLds &L, int2 *ring, int wmask, uint8_t *tb, int4 *rows, uint32_t *wr,
                           uint32_t pi, int *kept, unsigned long long *cells, unsigned long long *dps) {
     const int lane = lane_id();
-    const TbPair p = a.pairs[pi];
-    const TbQuery Q = a.qs[p.qi];
-    const uint64_t s0 = a.offs[p.sid];
+    const TbPair pv = a.pairs[pi];
+    const int64_t sid = (int64_t)first_lane64((uint64_t)pv.sid);
+    const uint64_t off = first_lane64(pv.off);
+    const int qi = uni(pv.qi), n_pre = uni(pv.n);
+    const TbQuery Qv = a.qs[qi];
+    const uint8_t *qs = uni_ptr(Qv.q);
+    const int qlen = uni(Qv.qlen), cutoff = uni(Qv.cutoff);
+    const uint64_t s0 = first_lane64(a.offs[sid]);
     const uint8_t *s = a.res + s0;
-    const int slen = (int)(a.offs[p.sid + 1] - s0);
-    TbHsp *out = a.out + p.off;
-    const Hsp *pre = a.pre + p.off;
+    const int slen = uni((int)(a.offs[sid + 1] - s0));
+    TbHsp *out = a.out + off;
+    const Hsp *pre = a.pre + off;
     int nt = 0;
-    for (int k = 0; k < p.n; k++) {
-        const Hsp h = pre[k];
+    for (int k = 0; k < n_pre; k++) {
+        const Hsp hv = pre[k];
+        const Hsp h{uni(hv.score), uni(hv.q_off), uni(hv.q_end), uni(hv.s_off), uni(hv.s_end), uni(hv.q_gs),
+                    uni(hv.s_gs)};
         bool skip = false;
         for (int t0 = 0; t0 < nt && !skip; t0 += 64) {
             const int t = t0 + lane;
@@ -318,19 +342,19 @@ __device__ int trace_pair(const TbArgs &a, TbLds &L, int2 *ring, int wmask, uint
         if (skip) continue;
         bool ovf = false;
         int la, lb, ra, rb;
-        const int left = align_ex_wave(L, ring, wmask, Q.q, s, h.q_gs + 1, h.s_gs + 1, a.x_final, a.gap_open,
+        const int left = align_ex_wave(L, ring, wmask, qs, s, h.q_gs + 1, h.s_gs + 1, a.x_final, a.gap_open,
                                        a.gap_extend, true, tb, a.tb_cap, rows, a.rows_cap, &la, &lb, &ovf, cells);
         if (ovf) return kPairBack;
         int nl = 0;
         int ws = walk_path(L, tb, rows, la, lb, wr, a.wruns_cap, &nl);
         if (ws != kWalkOk) return ws == kWalkFull ? kPairBack : kPairBad;
-        const int right = align_ex_wave(L, ring, wmask, Q.q + h.q_gs, s + h.s_gs, Q.qlen - h.q_gs - 1,
+        const int right = align_ex_wave(L, ring, wmask, qs + h.q_gs, s + h.s_gs, qlen - h.q_gs - 1,
                                         slen - h.s_gs - 1, a.x_final, a.gap_open, a.gap_extend, false, tb, a.tb_cap,
                                         rows, a.rows_cap, &ra, &rb, &ovf, cells);
         if (ovf) return kPairBack;
         *dps += 2;
         const int score = left + right;
-        if (score < Q.cutoff) continue;
+        if (score < cutoff) continue;
         int nall = nl;
         ws = walk_path(L, tb, rows, ra, rb, wr, a.wruns_cap, &nall);
         if (ws != kWalkOk) return ws == kWalkFull ? kPairBack : kPairBad;
@@ -344,29 +368,29 @@ __device__ int trace_pair(const TbArgs &a, TbLds &L, int2 *ring, int wmask, uint
         const bool fits = base + (unsigned long long)cnt <= a.runs_cap;
         // columns, gaps and identities along the script; the script places the runs as it goes
         const int q_off = h.q_gs + 1 - la, s_off = h.s_gs + 1 - lb;
-        int qi = q_off, si = s_off, ident = 0, columns = 0, gaps = 0, opens = 0;
+        int qp = q_off, sp = s_off, ident = 0, columns = 0, gaps = 0, opens = 0;
         for (int i = 0; i < cnt; i++) {
-            const uint32_t v = final_run(wr, nl, nr, merge, i);
+            const uint32_t v = (uint32_t)uni((int)final_run(wr, nl, nr, merge, i));
             if (fits && lane == 0) a.runs[base + i] = v;
             const int op = (int)(v >> 28), len = (int)(v & kRunLen);
             columns += len;
             if (op == kTbSub) {
                 for (int c0 = 0; c0 < len; c0 += 64) {
                     const int c = c0 + lane;
-                    ident += __popcll(__ballot(c < len && Q.q[qi + c] == s[si + c]));
+                    ident += __popcll(__ballot(c < len && qs[qp + c] == s[sp + c]));
                 }
-                qi += len;
-                si += len;
+                qp += len;
+                sp += len;
             } else {
                 gaps += len;
                 opens++;
                 if (op == kTbGapA)
-                    si += len;
+                    sp += len;
                 else
-                    qi += len;
+                    qp += len;
             }
         }
-        if (qi != h.q_gs + 1 + ra || si != h.s_gs + 1 + rb) return kPairBad;
+        if (qp != h.q_gs + 1 + ra || sp != h.s_gs + 1 + rb) return kPairBad;
         if (lane == 0) {
             TbHsp o;
             o.h = Hsp{score, q_off, h.q_gs + 1 + ra, s_off, h.s_gs + 1 + rb, h.q_gs, h.s_gs};

@@ -110,3 +110,18 @@ def test_bad_inputs_rejected_without_device():
     assert lib.blz_query_prepare(None, None, 0, None, C.byref(h)) == 1
     assert lib.blz_prelim_run(None, None, None, None, None) == 1
     assert lib.blz_prelim_run_noalloc(None, None, None, None, None) == 1
+
+
+def test_traceback_result_views():
+    """TbResult: pairs in call order, each pair's HSPs with op_first rebased to its own runs."""
+    pa = np.zeros(3, blz.TB_PAIR_DTYPE)
+    pa["query"], pa["oid"], pa["n_hsps"] = [0, 0, 1], [5, 9, 5], [1, 0, 2]
+    tb = np.zeros(3, blz.TB_DTYPE)
+    tb["op_first"], tb["op_count"], tb["score"] = [0, 2, 3], [2, 1, 2], [10, 20, 15]
+    r = blz.TbResult(2, pa, np.array([1, 0, 2], np.int32), tb, np.arange(5, dtype=np.uint32))
+    t, runs = r[0][5]
+    assert t["score"].tolist() == [10] and runs.tolist() == [0, 1]
+    assert len(r[0][9][0]) == 0 and len(r[0][9][1]) == 0
+    t, runs = r[1][5]
+    assert t["op_first"].tolist() == [0, 1] and runs.tolist() == [2, 3, 4]
+    assert sorted(r[0].keys()) == [5, 9] and 5 in r[1] and len(r) == 2
