@@ -256,8 +256,10 @@ def align_ex(A, B, x_dropoff, gap_open=11, gap_extend=1, reverse=False, M=None, 
     return sc, a.value, b.value, ops[: n.value].copy(), cells.value
 
 
-def traceback_subject(query, subject, prelim_hsps, x_final=64, gap_open=11, gap_extend=1, cutoff=0):
-    """Traced HSPs (TB_DTYPE, final order) of one subject, and their runs (op << 28 | length)."""
+def traceback_subject(query, subject, prelim_hsps, x_final=64, gap_open=11, gap_extend=1, cutoff=0,
+                      return_cells=False):
+    """Traced HSPs (TB_DTYPE, final order) of one subject, and their runs (op << 28 | length);
+    with return_cells also the DP cells the subject's alignments evaluated."""
     query, subject = _u8(query), _u8(subject)
     pre = np.ascontiguousarray(prelim_hsps, dtype=HSP_DTYPE)
     cap = max(1, len(pre))
@@ -269,6 +271,8 @@ def traceback_subject(query, subject, prelim_hsps, x_final=64, gap_open=11, gap_
                                     len(pre), x_final, gap_open, gap_extend, cutoff, out.ctypes.data, cap,
                                     runs.ctypes.data, runs_cap, C.byref(nr), C.byref(cells))
     assert nr.value >= 0
+    if return_cells:
+        return out[:n], runs[: nr.value].copy(), cells.value
     return out[:n], runs[: nr.value].copy()
 
 
