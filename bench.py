@@ -149,6 +149,7 @@ def main() -> int:
     ap.add_argument("--workload", default="scan", choices=["scan", "batch", "stress", "serve", "traceback"])
     ap.add_argument("--rates", default="20,100,200", help="serve: arrival rates, queries/s")
     ap.add_argument("--duration", type=float, default=4.0, help="serve: seconds of arrivals per rate")
+    ap.add_argument("--slots", type=int, default=2, help="serve: contexts of the batched form")
     ap.add_argument("--autotune", action="store_true", help="batch: tune on a sample batch first")
     ap.add_argument("--queries", type=int, default=0, help="batch/stress: number of queries (0: 256 / 64)")
     ap.add_argument("--gpus", type=int, default=1)
@@ -670,7 +671,7 @@ def run_serve(args) -> int:
     res, offs, hom = blz.synth_db(spec, traces[0][2][:50])
     n_sub, n_res = len(offs) - 1, int(offs[-1])
     log(f"DB built: {n_sub} subjects, {n_res} residues ({time.perf_counter() - t0:.1f} s)")
-    forms = {"batched": dict(slots=3, max_batch=64, max_batch_letters=32768),
+    forms = {"batched": dict(slots=args.slots, max_batch=64, max_batch_letters=32768),
              "plain": dict(slots=1, max_batch=1)}
     results = []
     with blz.Engine(0) as eng:
@@ -699,7 +700,7 @@ def run_serve(args) -> int:
         "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
         "config": {"workload": f"Poisson query streams at {args.rates} q/s for {args.duration:g} s each (lengths "
                                f"log-normal median 350, sigma 0.8, clip 30-4,000) vs a resident {n_sub}-subject DB "
-                               f"({n_res} residues); serving loop with continuous batching (3 slots, <= 64 queries) "
+                               f"({n_res} residues); serving loop with continuous batching ({args.slots} slots, <= 64 queries) "
                                "against the same loop with one query per launch",
                    "subjects": n_sub, "residues": n_res},
         "latency_ms": {"p50": top["batched"]["p50_ms"], "p99": top["batched"]["p99_ms"]},

@@ -91,12 +91,13 @@ class Tuning(C.Structure):
                 ("subject_groups", C.c_int32), ("overflow_mode", C.c_int32), ("cap_stage", C.c_int32),
                 ("cap_retry", C.c_int32), ("cap_ovf", C.c_int32), ("cap_wide_ring", C.c_int32),
                 ("cap_wide_saved", C.c_int32), ("cap_work", C.c_int64), ("cap_pool", C.c_int64),
-                ("cap_hsps", C.c_int64), ("cap_tb_bytes", C.c_int64), ("cap_tb_runs", C.c_int64)]
+                ("cap_hsps", C.c_int64), ("cap_tb_bytes", C.c_int64), ("cap_tb_runs", C.c_int64),
+                ("tb_lanes", C.c_int32), ("reserved_tb", C.c_int32)]
 
 
 class ServerConfig(C.Structure):
     _fields_ = [("slots", C.c_int32), ("max_batch", C.c_int32), ("max_batch_letters", C.c_int64),
-                ("reserved", C.c_int32), ("max_batch_work", C.c_int64)]
+                ("linger_us", C.c_int32), ("max_batch_work", C.c_int64), ("overlap_ms", C.c_double)]
 
 
 class RequestInfo(C.Structure):
@@ -108,7 +109,7 @@ class RequestInfo(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("batches", C.c_uint64),
                 ("residues", C.c_uint64), ("device_ms", C.c_double), ("reruns", C.c_uint64),
-                ("work_rate", C.c_double)]
+                ("work_rate", C.c_double), ("pair_rate", C.c_double)]
 
 
 class TbParams(C.Structure):
@@ -700,7 +701,7 @@ class Server:
     thread, wait for each, fetch its lists."""
 
     def __init__(self, engine: "Engine", slots: int = 0, max_batch: int = 0, max_batch_letters: int = 0,
-                 tuning: dict | None = None, max_batch_work: int = 0):
+                 tuning: dict | None = None, max_batch_work: int = 0, overlap_ms: float = 0.0, linger_us: int = 0):
         self.lib = engine.lib
         cfg = ServerConfig()
         self.lib.blz_server_default_config(C.byref(cfg))
@@ -712,6 +713,8 @@ class Server:
             cfg.max_batch_letters = max_batch_letters
         if max_batch_work:
             cfg.max_batch_work = max_batch_work
+        cfg.overlap_ms = overlap_ms
+        cfg.linger_us = linger_us
         t = None
         if tuning:
             t = Tuning()

@@ -207,4 +207,43 @@ __device__ void insertion_sort(R *h, int n) {
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Cross-lane steps within a segment of kSeg lanes: the whole wave (64), or one DPP row (16), so
+// that a wave runs four independent DPs side by side.  Values are per lane and uniform within a
+// segment.
+template <int kSeg>
+struct Seg;
+template <>
+struct Seg<64> {
+    static __device__ __forceinline__ int lane() { return lane_id(); }
+    static __device__ __forceinline__ int incl_max(int x) { return wave_inclusive_max(x); }
+    static __device__ __forceinline__ int shift_up1(int x, int in) { return wave_shift_up1(x, in); }
+    static __device__ __forceinline__ int last(int x) { return (int)last_lane((uint32_t)x); }
+    static __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+    static __device__ __forceinline__ int first(int x) { return (int)first_lane((uint32_t)x); }
+};
+template <>
+struct Seg<16> {
+    static __device__ __forceinline__ int lane() { return lane_id() & 15; }
+    static __device__ __forceinline__ int incl_max(int x) {  // row_shr 1/2/4/8 inside the row
+        constexpr int kMin = -2147483647 - 1;
+        x = max(x, __builtin_amdgcn_update_dpp(kMin, x, 0x111, 0xf, 0xf, false));
+        x = max(x, __builtin_amdgcn_update_dpp(kMin, x, 0x112, 0xf, 0xf, false));
+        x = max(x, __builtin_amdgcn_update_dpp(kMin, x, 0x114, 0xf, 0xf, false));
+        x = max(x, __builtin_amdgcn_update_dpp(kMin, x, 0x118, 0xf, 0xf, false));
+        return x;
+    }
+    static __device__ __forceinline__ int shift_up1(int x, int in) {  // row_shr:1, the row's lane 0 gets in
+        return __builtin_amdgcn_update_dpp(in, x, 0x111, 0xf, 0xf, false);
+    }
+    // DPP row_share:n (gfx90a and later): every lane of the row reads the row's lane n.  last()
+    // folds into a following max (v_max_i32_dpp); bcast() writes every lane and needs no old
+    // value (update_dpp's would cost a move per use).
+    static __device__ __forceinline__ int last(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x15F, 0xf, 0xf, false); }
+    static __device__ __forceinline__ int bcast(int x) { return __builtin_amdgcn_mov_dpp(x, 0x15F, 0xf, 0xf, false); }
+    static __device__ __forceinline__ uint64_t ballot(bool p) { return (__ballot(p) >> (lane_id() & 48)) & 0xFFFFull; }
+    static __device__ __forceinline__ int first(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x150, 0xf, 0xf, false); }
+    static __device__ __forceinline__ int reduce_max(int x) { return bcast(incl_max(x)); }
+};
+
 }  // namespace blz

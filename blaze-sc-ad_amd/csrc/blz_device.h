@@ -195,8 +195,13 @@ struct TbArgs {
     int32_t ring_mask;
     int32_t x_final, gap_open, gap_extend;
 };
-constexpr int kTbLdsRing = 256;   // main pass: DP band ring per wave in LDS; wider bands go to the wide pass
-hipError_t launch_traceback(const TbArgs &a, int blocks, bool wide, hipStream_t st);
-int traceback_blocks_per_cu(bool wide);
+// main pass: DP band ring in LDS per pair (a 64-lane wave, or a 16-lane segment of four per
+// wave); wider bands go to the wide pass
+constexpr int kTbWaveRing = 256, kTbSegRing = 128;
+// lanes: 16 (four pairs per wave) or 64 (one pair per wave) in the main pass; the wide pass
+// runs one pair per wave.  Each workgroup is one wave; per-pair storage areas are indexed by
+// workgroup x pairs per wave + segment.
+hipError_t launch_traceback(const TbArgs &a, int blocks, int lanes, bool wide, hipStream_t st);
+int traceback_blocks_per_cu(int lanes, bool wide);
 
 }  // namespace blz

@@ -362,7 +362,7 @@ blz_status blz_ctx_set_tuning(blz_ctx *ctx, const blz_tuning *t) {
         t->subject_groups > 1 || t->overflow_mode < 0 || t->overflow_mode > 1 || t->cap_stage < 0 ||
         t->cap_retry < 0 || t->cap_ovf < 0 || t->cap_wide_ring < 0 || t->cap_wide_saved < 0 || t->cap_work < 0 ||
         t->cap_work > 0xFFFFFFFFll || t->cap_pool < 0 || t->cap_hsps < 0 ||
-        t->cap_tb_bytes < 0 || t->cap_tb_runs < 0)
+        t->cap_tb_bytes < 0 || t->cap_tb_runs < 0 || (t->tb_lanes != 0 && t->tb_lanes != 16 && t->tb_lanes != 64))
         return BLZ_EINVAL;
     if (t->query_packing != ctx->tune.query_packing || t->pack_len != ctx->tune.pack_len) {
         // cached packs were made under the old choice

@@ -56,11 +56,19 @@ constexpr int64_t kBytesPerWorkItem = 100;       // work item, two init hits, or
 // batch's seed launches are sized for similar queries.
 int size_class(int32_t qlen) { return qlen <= 400 ? 0 : qlen <= 1000 ? 1 : qlen <= 2000 ? 2 : 3; }
 
+// Device time a batch is expected to take: its query letters x the database's residues at a rate
+// (letter-residue pairs per second) learned from the batches already run.  The prior is the
+// measured rate of one 500-letter query over 825.6 M residues in 11.7 ms.
+constexpr double kPriorPairRate = 3.5e13;
+constexpr double kDefaultOverlapMs = 5.0;
+
 struct Slot {
     blz_ctx *ctx = nullptr;
     std::thread th;
     std::vector<Request *> batch;  // assigned by the dispatcher, cleared by the slot
     bool busy = false;
+    Clock::time_point started;     // when its batch was handed out
+    double est_s = 0;              // the batch's expected device time
 };
 
 }  // namespace
@@ -81,6 +89,8 @@ struct blz_server {
     int64_t nsubj = 0;
     double work_rate = kPriorWorkRate;  // work items per query letter and subject
     int64_t work_budget = 0;            // estimated work items a batch may hold
+    uint64_t nres = 0;
+    double pair_rate = kPriorPairRate;  // query letters x residues per second of device time
 };
 
 namespace {
@@ -110,11 +120,27 @@ void prepare_loop(blz_server *s) {
     }
 }
 
+// Whether a request may share a batch: a query that packs (at most half a pack's letters, the pack
+// being 1024 - 128 - window letters) shares the seed scan with its batch-mates; a longer one scans
+// the database alone either way, so where its scan takes long (more than kBatchLongS) batching it
+// would only hold its batch-mates' results back until it is done.
+constexpr double kBatchLongS = 0.02;
+bool batchable(const blz_server *s, const Request *r) {
+    const int pack = std::max(64, 1024 - 128 - r->params.window);
+    if ((int)r->letters.size() <= pack / 2) return true;
+    return (double)r->letters.size() * (double)s->nres / s->pair_rate <= kBatchLongS;
+}
+
 // Oldest waiting request plus same-class requests behind it, within the caps: queries, query
 // letters, and the estimated device footprint (work items, each request's at most one per subject).
 std::vector<Request *> form_batch(blz_server *s) {
     std::vector<Request *> b;
     if (s->waiting.empty()) return b;
+    if (!batchable(s, s->waiting.front())) {
+        b.push_back(s->waiting.front());
+        s->waiting.pop_front();
+        return b;
+    }
     const int cls = size_class((int32_t)s->waiting.front()->letters.size());
     int64_t letters = 0;
     double work = 0;
@@ -122,7 +148,7 @@ std::vector<Request *> form_batch(blz_server *s) {
         Request *r = *it;
         const int64_t L = (int64_t)r->letters.size();
         const double w = std::min((double)s->nsubj, s->work_rate * (double)L * (double)s->nsubj);
-        if (size_class((int32_t)L) == cls &&
+        if (size_class((int32_t)L) == cls && batchable(s, r) &&
             (b.empty() || (letters + L <= s->cfg.max_batch_letters && work + w <= (double)s->work_budget))) {
             b.push_back(r);
             letters += L;
@@ -135,23 +161,49 @@ std::vector<Request *> form_batch(blz_server *s) {
     return b;
 }
 
+// Device time the batches in flight are still expected to need.
+double inflight_s(const blz_server *s, Clock::time_point now) {
+    double rem = 0;
+    for (const auto &sl : s->slots)
+        if (sl.busy) rem += std::max(0.0, sl.est_s - std::chrono::duration<double>(now - sl.started).count());
+    return rem;
+}
+
 void dispatch_loop(blz_server *s) {
     std::unique_lock<std::mutex> lk(s->mu);
+    const double overlap_s = s->cfg.overlap_ms == 0 ? kDefaultOverlapMs * 1e-3 : s->cfg.overlap_ms * 1e-3;
     for (;;) {
-        s->cv.wait(lk, [&] {
-            if (s->stopping) return true;
-            if (s->waiting.empty()) return false;
-            for (auto &sl : s->slots)
-                if (!sl.busy) return true;
-            return false;
-        });
         if (s->stopping) return;
-        for (auto &sl : s->slots) {
-            if (sl.busy || s->waiting.empty()) continue;
-            sl.batch = form_batch(s);
-            sl.busy = true;
-            s->stats.batches++;
+        Slot *free_slot = nullptr;
+        for (auto &sl : s->slots)
+            if (!sl.busy) {
+                free_slot = &sl;
+                break;
+            }
+        if (s->waiting.empty() || !free_slot) {
+            s->cv.wait(lk);
+            continue;
         }
+        const Clock::time_point now = Clock::now();
+        const double rem = inflight_s(s, now);
+        if (overlap_s >= 0 && rem > overlap_s) {  // the GPU is still taken: wait, and let requests gather
+            s->cv.wait_for(lk, std::chrono::duration<double>(rem - overlap_s));
+            continue;
+        }
+        if (s->cfg.linger_us > 0 && (int)s->waiting.size() < s->cfg.max_batch) {  // let others join
+            const Clock::time_point until = s->waiting.front()->t_ready + std::chrono::microseconds(s->cfg.linger_us);
+            if (now < until) {
+                s->cv.wait_until(lk, until);
+                continue;
+            }
+        }
+        free_slot->batch = form_batch(s);
+        int64_t letters = 0;
+        for (const Request *r : free_slot->batch) letters += (int64_t)r->letters.size();
+        free_slot->busy = true;
+        free_slot->started = now;
+        free_slot->est_s = (double)letters * (double)s->nres / s->pair_rate;
+        s->stats.batches++;
         s->cv.notify_all();
     }
 }
@@ -213,6 +265,13 @@ void slot_loop(blz_server *s, Slot *sl) {
         s->stats.residues += st.residues;
         s->stats.device_ms += st.ms_total;
         s->stats.reruns += st.reruns > 0 ? 1 : 0;
+        // the batch's letter-residue rate, from its device time (the part of a wall time shared
+        // with overlapping batches is not its own); the estimate follows it smoothly
+        if (letters > 0 && st.ms_total > 0) {
+            const double r = (double)letters * (double)s->nres / (st.ms_total * 1e-3);
+            s->pair_rate = std::min(4 * s->pair_rate, std::max(0.25 * s->pair_rate, 0.7 * s->pair_rate + 0.3 * r));
+            s->stats.pair_rate = s->pair_rate;
+        }
         // the admission's rate follows the batches: a quarter above the highest rate seen lately
         if (letters > 0 && s->nsubj > 0) {
             const double seen = (double)st.gapped_subjects / ((double)letters * (double)s->nsubj);
@@ -232,8 +291,9 @@ void blz_server_default_config(blz_server_config *c) {
     c->slots = 2;
     c->max_batch = 64;
     c->max_batch_letters = 32768;
-    c->reserved = 0;
+    c->linger_us = 0;
     c->max_batch_work = 0;
+    c->overlap_ms = 0;
 }
 
 blz_status blz_server_create(int32_t device, const blz_db *db, const blz_server_config *cfg,
@@ -247,11 +307,15 @@ blz_status blz_server_create(int32_t device, const blz_db *db, const blz_server_
         if (cfg->max_batch > 0) s->cfg.max_batch = cfg->max_batch;
         if (cfg->max_batch_letters > 0) s->cfg.max_batch_letters = cfg->max_batch_letters;
         if (cfg->max_batch_work > 0) s->cfg.max_batch_work = cfg->max_batch_work;
+        s->cfg.overlap_ms = cfg->overlap_ms;
+        s->cfg.linger_us = cfg->linger_us > 0 ? cfg->linger_us : 0;
     }
     if (s->cfg.slots > 16) s->cfg.slots = 16;
     s->db = db;
     s->nsubj = blz_db_num_subjects(db);
+    s->nres = blz_db_num_residues(db);
     s->stats.work_rate = s->work_rate;
+    s->stats.pair_rate = s->pair_rate;
     blz_status st = blz_ctx_create(device, &s->prep_ctx);
     // a batch's footprint budget: what a slot's buffers hold at start, within its share of half
     // the free device memory

@@ -41,7 +41,7 @@ struct TbState {
     size_t wruns_cap = 0;
     int2 *ring = nullptr;
     size_t ring_cap = 0;
-    int blocks_main = 0, blocks_wide = 0;  // resident workgroups per CU
+    int blocks_main[2] = {0, 0}, blocks_wide = 0;  // resident workgroups per CU (main: 16, 64 lanes)
     // last result
     std::vector<int32_t> counts;
     std::vector<blz_tb_hsp> hsps;
@@ -64,10 +64,10 @@ namespace {
 static_assert(sizeof(TbHsp) == sizeof(blz_tb_hsp), "TbHsp mirrors blz_tb_hsp");
 static_assert(sizeof(Hsp) == sizeof(blz_hsp), "Hsp mirrors blz_hsp");
 
-constexpr size_t kMainTbBytes = 2ull << 30;   // all main-pass waves' traceback rows together
-constexpr size_t kMainTbPerWave = 8u << 20;   // at most this much per wave
-constexpr size_t kMainTbMin = 256u << 10;
-constexpr int kMainRuns = 4096;               // edit-script runs per wave (both directions of one HSP)
+constexpr size_t kMainTbBytes = 4ull << 30;   // all main-pass pair slots' traceback rows together
+constexpr size_t kMainTbPerWave = 8u << 20;   // at most this much per pair slot
+constexpr size_t kMainTbMin = 64u << 10;
+constexpr int kMainRuns = 4096;               // edit-script runs per pair slot (both directions of one HSP)
 constexpr size_t kWideBytes = 4ull << 30;     // wide pass: all waves' storage together
 constexpr size_t kTbCellsMax = 0x7FFFFFFF;    // row offsets are 32-bit
 constexpr int kWideWavesMax = 1024;
@@ -157,14 +157,21 @@ blz_status blz_traceback(blz_ctx *ctx, const blz_db *db, const blz_query *const 
     S.counts.clear();
     S.hsps.clear();
     S.runs_h.clear();
-    if (!S.blocks_main) {
-        S.blocks_main = std::max(1, traceback_blocks_per_cu(false));
-        S.blocks_wide = std::max(1, traceback_blocks_per_cu(true));
+    const int lanes = ctx->tune.tb_lanes == 16 ? 16 : 64;
+    const int pairs_per_wave = 64 / lanes;  // pairs a main-pass wave runs side by side
+    if (!S.blocks_wide) {
+        S.blocks_main[0] = std::max(1, traceback_blocks_per_cu(16, false));
+        S.blocks_main[1] = std::max(1, traceback_blocks_per_cu(64, false));
+        S.blocks_wide = std::max(1, traceback_blocks_per_cu(64, true));
     }
     hipStream_t s = ctx->stream;
     const size_t np = (size_t)n_pairs;
-    // main pass shape: waves resident at once, each with bounded traceback rows
-    const size_t waves = std::max<size_t>(1, std::min<size_t>(np, (size_t)ctx->num_cus * S.blocks_main));
+    // main pass shape: the resident waves (no more than the pairs need), each pair slot with
+    // bounded traceback rows
+    const size_t blocks = std::max<size_t>(
+        1, std::min<size_t>((np + pairs_per_wave - 1) / pairs_per_wave,
+                            (size_t)ctx->num_cus * S.blocks_main[lanes == 64]));
+    const size_t waves = blocks * pairs_per_wave;  // pair slots, each with its own storage areas
     size_t tb_per_wave = std::min(kMainTbPerWave, std::max(kMainTbMin, (size_t)256 * (size_t)(max_qlen + 2)));
     tb_per_wave = std::min(tb_per_wave, std::max(kMainTbMin, kMainTbBytes / waves));
     if (ctx->tune.cap_tb_bytes > 0) tb_per_wave = (size_t)ctx->tune.cap_tb_bytes;
@@ -219,7 +226,7 @@ blz_status blz_traceback(blz_ctx *ctx, const blz_db *db, const blz_query *const 
         a.ring_mask = 0;
         if (!check(ctx, hipMemsetAsync(S.ctr, 0, sizeof(TbCounters), s), "traceback counters") ||
             !check(ctx, hipEventRecord(ctx->ev[0], s), "event") ||
-            !check(ctx, launch_traceback(a, (int)waves, false, s), "traceback main pass") ||
+            !check(ctx, launch_traceback(a, (int)blocks, lanes, false, s), "traceback main pass") ||
             !check(ctx, hipEventRecord(ctx->ev[1], s), "event") ||
             !check(ctx, hipMemcpyAsync(&hc, S.ctr, sizeof(hc), hipMemcpyDeviceToHost, s), "traceback counters") ||
             !check(ctx, hipStreamSynchronize(s), "traceback main pass"))
@@ -270,7 +277,7 @@ blz_status blz_traceback(blz_ctx *ctx, const blz_db *db, const blz_query *const 
             // next_pair and back_count restart; the run reservations and cell counts go on
             if (!check(ctx, hipMemsetAsync(&S.ctr->next_pair, 0, 2 * sizeof(unsigned int), s), "traceback counters") ||
                 !check(ctx, hipEventRecord(ctx->ev[2], s), "event") ||
-                !check(ctx, launch_traceback(a, (int)wide_waves, true, s), "traceback wide pass") ||
+                !check(ctx, launch_traceback(a, (int)wide_waves, 64, true, s), "traceback wide pass") ||
                 !check(ctx, hipEventRecord(ctx->ev[3], s), "event") ||
                 !check(ctx, hipMemcpyAsync(&hc, S.ctr, sizeof(hc), hipMemcpyDeviceToHost, s), "traceback counters") ||
                 !check(ctx, hipStreamSynchronize(s), "traceback wide pass"))

@@ -159,6 +159,10 @@ typedef struct {
     int64_t cap_tb_bytes;      /* traceback main pass: traceback bytes per wave (pairs needing
                                   more go to the wide pass) */
     int64_t cap_tb_runs;       /* traceback: edit-script runs a call starts with */
+    int32_t tb_lanes;          /* traceback main pass: lanes per pair, 64 (one pair per wave) or 16
+                                  (four pairs per wave, slower: 25.3 against 20.4 ms on the
+                                  traceback workload); 0: 64 */
+    int32_t reserved_tb;
 } blz_tuning;
 
 blz_status blz_ctx_set_tuning(blz_ctx *ctx, const blz_tuning *t);
@@ -358,20 +362,30 @@ blz_status blz_traceback_copy(const blz_ctx *ctx, int32_t *counts, int64_t n_pai
  * request and the same-size-class requests behind it run there as one batch
  * (blz_prelim_run_batch), admitted by footprint: up to max_batch queries, max_batch_letters query
  * letters and max_batch_work estimated work items, the estimate per request being its letters x
- * the database's subjects x a rate learned from the batches already run.  Batches of different
- * slots overlap on the device, and requests arriving meanwhile join the next batch.  A request's
- * lists are exactly those blz_prelim_run gives for it alone. */
+ * the database's subjects x a rate learned from the batches already run.  A slot starts its batch
+ * only while the batches in flight are estimated to need at most overlap_ms more device time (their
+ * query letters x the database's residues at a learned rate), so where one batch fills the GPU the
+ * batches run in turn and requests arriving meanwhile join the next one; where batches are short,
+ * those of different slots overlap on the device.  A query longer than a pack member (half of
+ * 1024 - 128 - window letters) whose scan is estimated above 20 ms runs alone: it would share no
+ * scan and only hold its batch-mates back.  A request's lists are exactly those blz_prelim_run
+ * gives for it alone. */
 typedef struct blz_server blz_server;
 
 typedef struct {
     int32_t slots;             /* batches in flight; 0: 2 (at most 16) */
     int32_t max_batch;         /* queries per batch; 0: 64 (1: one query per launch) */
     int64_t max_batch_letters; /* query letters per batch; 0: 32768 */
-    int32_t reserved;
+    int32_t linger_us;         /* a batch with room for more requests waits up to this long after
+                                  its oldest request became ready, for others to join; 0: none */
     int64_t max_batch_work;    /* device footprint of a batch, as estimated (query, subject) work
                                   items (each with its init hits, order entry and result: about
                                   100 bytes); 0: what a slot's buffers hold at start (48 Mi items),
                                   within its share of the free HBM.  A request alone always runs. */
+    double overlap_ms;         /* a free slot starts a batch only while the batches in flight are
+                                  estimated to need at most this much more device time, so batches
+                                  that fill the GPU alone (large databases) run one after another and
+                                  only overlap their ends; 0: 5 ms; negative: no limit */
 } blz_server_config;
 
 typedef struct {
@@ -389,6 +403,7 @@ typedef struct {
     double device_ms;          /* device time of the batches, summed over slots */
     uint64_t reruns;           /* batches run again with grown buffers (a footprint estimate fell short) */
     double work_rate;          /* work items per query letter and subject the admission assumes now */
+    double pair_rate;          /* query letters x database residues per second it assumes now */
 } blz_server_stats;
 
 void blz_server_default_config(blz_server_config *cfg);

@@ -30,7 +30,7 @@ def test_serving_alone_vs_mix():
             r, _ = eng.run(h)
             eng.free_query(h)
             alone.append(r)
-        with blz.Server(eng, slots=3, max_batch=8, max_batch_letters=4000) as srv:
+        with blz.Server(eng, slots=3, max_batch=8, max_batch_letters=4000, linger_us=20000) as srv:
             tickets = [None] * len(qs)
 
             def submitter(k):  # four client threads, staggered arrivals
@@ -87,7 +87,7 @@ def test_admission_by_footprint():
             alone.append(eng.run(h)[0])
             eng.free_query(h)
         for budget, want_single in ((1, True), (10 ** 9, False)):
-            with blz.Server(eng, slots=2, max_batch=8, max_batch_work=budget) as srv:
+            with blz.Server(eng, slots=2, max_batch=8, max_batch_work=budget, linger_us=20000) as srv:
                 tickets = [srv.submit(q) for q in qs]
                 sizes = []
                 for i, tk in enumerate(tickets):

@@ -132,19 +132,24 @@ def test_reference_queries_with_ambiguity_letters():
         got, st = eng.traceback(hs, lists, blz.tb_params(x_dropoff=stats[0].x_final))
         log(f"{len(queries)} queries: {st['pairs']} pairs, {st['hsps']} traced HSPs, {st['dp_cells']} cells, "
             f"{st['ms_main']:.2f} ms main + {st['ms_wide']:.2f} ms wide ({st['wide_pairs']} pairs)")
+        # four pairs per wave (16-lane segments) instead of one
+        eng.set_tuning(tb_lanes=16)
+        wave, wst = eng.traceback(hs, lists, blz.tb_params(x_dropoff=stats[0].x_final))
+        log(f"16 lanes per pair: {wst['ms_main']:.2f} ms main, {wst['wide_pairs']} wide pairs")
         # forced: all but the shortest pairs through the wide pass, and an edit-script buffer that
         # must grow
-        eng.set_tuning(cap_tb_bytes=4096, cap_tb_runs=64)
+        eng.set_tuning(tb_lanes=0, cap_tb_bytes=4096, cap_tb_runs=64)
         forced, fst = eng.traceback(hs, lists, blz.tb_params(x_dropoff=stats[0].x_final))
         log(f"forced: {fst['wide_pairs']} wide pairs, {fst['reruns']} reruns")
         for h in hs:
             eng.free_query(h)
     assert fst["wide_pairs"] > 0.5 * st["pairs"] and fst["reruns"] >= 1
-    for qi in range(len(queries)):
-        assert forced[qi].keys() == got[qi].keys()
-        for oid in got[qi]:
-            assert forced[qi][oid][0].tolist() == got[qi][oid][0].tolist()
-            assert forced[qi][oid][1].tolist() == got[qi][oid][1].tolist()
+    for other in (wave, forced):
+        for qi in range(len(queries)):
+            assert other[qi].keys() == got[qi].keys()
+            for oid in got[qi]:
+                assert other[qi][oid][0].tolist() == got[qi][oid][0].tolist()
+                assert other[qi][oid][1].tolist() == got[qi][oid][1].tolist()
     want = oracle_pairs(queries, res, offs, lists, stats[0].x_final, 0)
     n = compare(want, got)
     log(f"{len(want)} pairs, {n} traced HSPs equal the oracle")
