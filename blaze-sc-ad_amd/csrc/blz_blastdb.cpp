@@ -20,6 +20,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <memory>
 #include <cstring>
 #include <fstream>
 #include <numeric>
@@ -28,6 +29,7 @@
 #include <vector>
 
 #include "blz_common.h"
+#include "blz_host.h"
 #include "blz_prelim.h"
 
 namespace {
@@ -286,9 +288,15 @@ blz_status blz_db_upload_blastdb(blz_ctx *ctx, const blz_blastdb *bdb, int32_t s
     const int64_t n = bdb->first.back();
     std::vector<uint8_t> res(bdb->residues + 1);
     std::vector<uint64_t> offs((size_t)n + 1);
-    const blz_status st = blz_blastdb_gather(bdb, sort_by_length, res.data(), offs.data(), perm);
+    auto ids = std::make_shared<std::vector<int64_t>>((size_t)std::max<int64_t>(n, 1));
+    blz_status st = blz_blastdb_gather(bdb, sort_by_length, res.data(), offs.data(), ids->data());
     if (st != BLZ_OK) return st;
-    return blz_db_upload(ctx, res.data(), offs.data(), n, out);
+    st = blz_db_upload(ctx, res.data(), offs.data(), n, out);
+    if (st != BLZ_OK) return st;
+    ids->resize((size_t)n);
+    if (perm) std::copy(ids->begin(), ids->end(), perm);
+    if (sort_by_length) (*out)->oid_of = std::move(ids);  // the hit list breaks ties by these ids
+    return BLZ_OK;
 }
 
 }  // extern "C"

@@ -1465,6 +1465,7 @@ int collect(blz_ctx *ctx, int nq, const blz_db *db, blz_hsp_sink *sinks, blz_run
     ctx->last_first[nq] = r0;
     ctx->last_res.swap(res);
     ctx->last_hs.swap(hs);
+    ctx->last_oid_of = db->oid_of;
     ctx->last_ovf.clear();
     for (int64_t e : qo.left) ctx->last_ovf.push_back(e & ~(int64_t)kOvfGapped);
     blz_status rc = BLZ_OK;
@@ -1564,12 +1565,14 @@ blz_status blz_result_hitlist(const blz_ctx *ctx, int32_t qi, const blz_search_s
         e.push_back({blz_evalue(s, best), best, r.sid, i});
     }
     // NCBI's list order (s_EvalueCompareHSPLists): best e-value ascending, e-values below 1e-180
-    // counting as equal; then best score descending; then the higher ordinal id first
-    auto before = [](const Entry &a, const Entry &b) {
+    // counting as equal; then best score descending; then the higher ordinal id first (the
+    // database's own id where the upload reordered it)
+    const std::vector<int64_t> *oid_of = ctx->last_oid_of.get();
+    auto before = [oid_of](const Entry &a, const Entry &b) {
         constexpr double kTiny = 1.0e-180;
         if (!(a.ev < kTiny && b.ev < kTiny) && a.ev != b.ev) return a.ev < b.ev;
         if (a.score != b.score) return a.score > b.score;
-        return a.sid > b.sid;
+        return oid_of ? (*oid_of)[a.sid] > (*oid_of)[b.sid] : a.sid > b.sid;
     };
     const size_t k = std::min<size_t>(e.size(), (size_t)s->hitlist_size);
     std::partial_sort(e.begin(), e.begin() + k, e.end(), before);

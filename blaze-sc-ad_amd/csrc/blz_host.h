@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -145,6 +146,7 @@ struct blz_ctx {
     std::vector<size_t> last_first;      // [nq + 1] first result item of each query
     std::vector<int64_t> last_nh;        // [nq] HSPs of each query
     std::vector<int64_t> last_ovf;       // overflow pairs: query << 40 | subject
+    std::shared_ptr<const std::vector<int64_t>> last_oid_of;  // the last run's database's id map
     TbState *tb = nullptr;               // traceback stage, created by its first call
     // launch sequence of the last batch, captured; valid while the same batch comes again and
     // no other run has rewritten the device tables in between
@@ -175,6 +177,9 @@ struct blz_db {
     std::vector<uint64_t> h_offs;  // host copy of the offsets, for slicing by residues
     int64_t *long_ids = nullptr;   // device: subjects longer than the seed kernel's LDS ring takes
     int64_t n_long = 0;
+    // ordinal id of each uploaded position when the upload reordered a database (a BLAST database
+    // uploaded length-sorted); null: positions are the ids
+    std::shared_ptr<const std::vector<int64_t>> oid_of;
 };
 
 struct blz_query {

@@ -282,7 +282,8 @@ blz_status blz_search_setup(const blz_stats_options *o, const uint8_t *query, in
 double blz_evalue(const blz_search_stats *s, int32_t score);
 /* The hit list of query qi of the last run (NCBI Blast_HSPResultsFromHSPStreamWithLimitEx and
  * Blast_HSPResultsSortByEvalue): at most s->hitlist_size subjects, the best by best e-value
- * (e-values below 1e-180 equal), then best score, then higher subject id first, in that order;
+ * (e-values below 1e-180 equal), then best score, then higher ordinal id first (for a database
+ * uploaded length-sorted by blz_db_upload_blastdb, the id it has in the BLAST database), in that order;
  * each subject's HSPs keep their score order.  best_evalue[subject_cap] (optional) receives each
  * listed subject's best e-value.  BLZ_ECAPACITY (sizes reported) if the sink is too small. */
 blz_status blz_result_hitlist(const blz_ctx *ctx, int32_t qi, const blz_search_stats *s, blz_hsp_sink *sink,
