@@ -43,6 +43,7 @@ namespace {
 enum : int { kTbSub = 0, kTbGapA = 1, kTbGapB = 2, kTbOpMask = 3, kTbExtA = 4, kTbExtB = 8 };
 constexpr int kNoCell = 0xFF;  // window byte outside the stored rows
 constexpr uint32_t kRunLen = 0x0FFFFFFFu;
+constexpr int kMatRow = 32;  // LDS matrix row: 28 scores and 4 unused, so a row's address is a shift
 
 enum PairOutcome { kPairDone = 0, kPairBack = 1, kPairBad = 2 };
 enum WalkOutcome { kWalkOk = 0, kWalkFull = 1, kWalkBad = 2 };
@@ -102,9 +103,11 @@ __device__ __forceinline__ void load_awin(typename Tb<kSeg>::Lds &L, const uint8
 // the X-drop band walks B; row 0 is the gap run in A.  Row a's traceback bytes, columns
 // [rows[a].x, rows[a].x + rows[a].z), lie at tb[rows[a].y ...].  Returns the best score, its cell
 // in (*a_off, *b_off); sets *ovf when the band outgrows the ring or the rows outgrow tb_cap or
-// rows_cap (nothing usable is left then).
+// rows_cap (nothing usable is left then).  The row table is staged in registers: the lane whose
+// index is the row's index mod kSeg keeps the row, and a block of kSeg rows goes out in one
+// store when it is full and at the end.
 template <int kSeg>
-__device__ int align_ex_seg(const int16_t (*mat)[28], typename Tb<kSeg>::Lds &L, int2 *ring, int wmask,
+__device__ int align_ex_seg(const int16_t (*mat)[kMatRow], typename Tb<kSeg>::Lds &L, int2 *ring, int wmask,
                             const uint8_t *A, const uint8_t *B, int M, int N, int X, int go, int ge, bool reverse,
                             uint8_t *tb, unsigned long long tb_cap, int4 *rows, int rows_cap, int *a_off, int *b_off,
                             bool *ovf, unsigned long long *cells) {
@@ -129,8 +132,20 @@ __device__ int align_ex_seg(const int16_t (*mat)[28], typename Tb<kSeg>::Lds &L,
         ring[b & wmask] = make_int2(h, h - goe);
         tb[b] = kTbGapA;
     }
-    if (sl == 0) rows[0] = make_int4(0, 0, size, 0);
-    unsigned long long used = (unsigned long long)size;
+    int4 rreg = make_int4(0, 0, size, 0);  // row 0 (this lane's staged row while sl == 0)
+    // stages row a (first column, byte offset, length); a full block of kSeg rows goes out
+    auto stage_row = [&](int a, int rs, int ro, int len) {
+        const bool mine = sl == (a & (kSeg - 1));
+        rreg.x = mine ? rs : rreg.x;
+        rreg.y = mine ? ro : rreg.y;
+        rreg.z = mine ? len : rreg.z;
+        if ((a & (kSeg - 1)) == kSeg - 1) rows[a - (kSeg - 1) + sl] = rreg;
+    };
+    // the rows of the last, partial block [a & ~(kSeg - 1), a]
+    auto flush_rows = [&](int a) {
+        if ((a & (kSeg - 1)) != kSeg - 1 && sl <= (a & (kSeg - 1))) rows[(a & ~(kSeg - 1)) + sl] = rreg;
+    };
+    uint32_t used = (uint32_t)size;
     int bw0 = 0, aw0 = 1;
     wave_sync();
     load_bwin<kSeg>(L, B, N, reverse, bw0);
@@ -150,7 +165,8 @@ __device__ int align_ex_seg(const int16_t (*mat)[28], typename Tb<kSeg>::Lds &L,
         }
         const int16_t *mrow = mat[L.awin[a - aw0]];
         const int rs = first;
-        if (used + (unsigned long long)(size - rs) > tb_cap) {
+        used = (uint32_t)T::U((int)used);
+        if ((unsigned long long)used + (unsigned long long)(size - rs) > tb_cap) {
             *ovf = true;
             return 0;
         }
@@ -209,7 +225,8 @@ __device__ int align_ex_seg(const int16_t (*mat)[28], typename Tb<kSeg>::Lds &L,
         nc += (unsigned long long)(size - first);
         int row_end = size;
         if (fa < 0) {
-            if (sl == 0) rows[a] = make_int4(rs, (int)used, row_end - rs, 0);
+            stage_row(a, rs, (int)used, row_end - rs);
+            flush_rows(a);
             break;
         }
         if (rmax > best) {
@@ -225,7 +242,7 @@ __device__ int align_ex_seg(const int16_t (*mat)[28], typename Tb<kSeg>::Lds &L,
             if (gr >= best - X) {
                 int k = (ge == 1 ? gr - (best - X) : (gr - (best - X)) / ge) + 1;
                 if (k > N + 1 - size) k = N + 1 - size;
-                if (size + k - first > W || used + (unsigned long long)(size + k - rs) > tb_cap) {
+                if (size + k - first > W || (unsigned long long)used + (unsigned long long)(size + k - rs) > tb_cap) {
                     *ovf = true;
                     return 0;
                 }
@@ -238,14 +255,15 @@ __device__ int align_ex_seg(const int16_t (*mat)[28], typename Tb<kSeg>::Lds &L,
                 row_end = size;
             }
         }
-        if (sl == 0) rows[a] = make_int4(rs, (int)used, row_end - rs, 0);
-        used += (unsigned long long)(row_end - rs);
+        stage_row(a, rs, (int)used, row_end - rs);
+        if (a == M) flush_rows(a);
+        used += (uint32_t)(row_end - rs);
         if (size <= N) {
             if (size + 1 - first > W) {
                 *ovf = true;
                 return 0;
             }
-            if (sl == 0) ring[size & wmask] = make_int2(kMinIntDp, kMinIntDp);
+            ring[size & wmask] = make_int2(kMinIntDp, kMinIntDp);  // every lane, the same value
             size++;
         }
         wave_sync();
@@ -342,7 +360,7 @@ __device__ __forceinline__ bool contained(const Hsp &h, const Hsp &o) {
 // The traceback of one pair by one segment.  Traced HSPs are built in out[0, n) (the segment's
 // lane 0 writes), purged and sorted there; returns the outcome and the kept count in *kept.
 template <int kSeg>
-__device__ int trace_pair(const TbArgs &a, const int16_t (*mat)[28], typename Tb<kSeg>::Lds &L, int2 *ring, int wmask,
+__device__ int trace_pair(const TbArgs &a, const int16_t (*mat)[kMatRow], typename Tb<kSeg>::Lds &L, int2 *ring, int wmask,
                           uint8_t *tb, int4 *rows, uint32_t *wr, uint32_t pi, int *kept, unsigned long long *cells,
                           unsigned long long *dps) {
     using T = Tb<kSeg>;
@@ -464,13 +482,13 @@ __global__ void __launch_bounds__(64) k_traceback(TbArgs a) {
     using T = Tb<kSeg>;
     using S = Seg<kSeg>;
     constexpr int kSegs = 64 / kSeg;
-    __shared__ int16_t mat[28][28];
+    __shared__ int16_t mat[28][kMatRow];
     __shared__ typename T::Lds lds[kSegs];
     __shared__ int2 lring[kWide ? 1 : kSegs * T::kRing];
     const int lane = lane_id();
     const int seg = lane / kSeg;
     const int sl = S::lane();
-    for (int i = lane; i < 28 * 28; i += 64) mat[i / 28][i % 28] = kBlosum62Dev[i / 28][i % 28];
+    for (int i = lane; i < 28 * kMatRow; i += 64) mat[i / kMatRow][i % kMatRow] = i % kMatRow < 28 ? kBlosum62Dev[i / kMatRow][i % kMatRow] : 0;
     wave_sync();
     const size_t w = (size_t)blockIdx.x * kSegs + seg;  // this segment's storage
     uint8_t *tb = a.tb + w * a.tb_cap;
