@@ -14,8 +14,9 @@
   traceback, edit scripts back on the host) in one call per step: pairs/s, traceback GCUPS, a
   sample bit-exact against the CPU restatement.
 --workload serve: Poisson query streams (lengths log-normal, median 350, sigma 0.8, clipped
-  30-4,000) at several rates against a resident DB, through the serving loop with continuous
-  batching, and through the same loop with one query per launch: p50/p99 latency, residues/s.
+  30-4,000) at several rates against a resident DB (--subjects, 1,000,000 by default; 40,000,000
+  is the full serving workload), through the serving loop with continuous batching (--slots
+  contexts) and through the same loop with one query per launch: p50/p99 latency, residues/s.
 
     python bench.py [--workload scan|batch|stress|serve] [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
