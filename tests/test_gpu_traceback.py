@@ -191,3 +191,30 @@ def test_edge_pairs_and_bad_input():
             eng.traceback([h], [blz.Result(np.array([len(offs) - 1]), np.array([0], np.int32), top[:0])])
         eng.free_query(h)
     assert got[0][oid][0].tolist() == wt.tolist() and got[0][oid][1].tolist() == wr.tolist()
+
+
+@pytest.mark.parametrize("go,ge,x", [(10, 2, 40), (13, 1, 90), (7, 3, 30)])
+def test_other_gap_costs_and_xdrops(go, ge, x):
+    """Gap costs other than 11/1 (an extension of 2 or 3 takes the division and the multiples of
+    the extension in the band rules) and other final X-drops, prelim lists from the same costs,
+    every pair against the oracle."""
+    q = blz.synth_query(60 + go, 350)
+    spec = blz.synth_spec(seed=60 + go, n_subjects=4000, homolog_frac=0.08, ident_min=0.3, ident_max=0.8,
+                          indel_rate=0.03)
+    res, offs, _ = blz.synth_db(spec, [q])
+    p = blz.default_params(gap_open=go, gap_extend=ge)
+    with blz.Engine(0) as eng:
+        eng.upload_db(res, offs)
+        h = eng.prepare(q, p)
+        r, _ = eng.run(h)
+        got, st = eng.traceback([h], [r], blz.tb_params(x_dropoff=x, gap_open=go, gap_extend=ge))
+        eng.free_query(h)
+    n = 0
+    for oid, hs in r.as_dict().items():
+        wt, wr = oracle.traceback_subject(q, res[int(offs[oid]):int(offs[oid + 1])], hs, x_final=x, gap_open=go,
+                                          gap_extend=ge)
+        gt, gr = got[0][oid]
+        assert gt.tolist() == wt.tolist() and gr.tolist() == wr.tolist(), oid
+        n += len(wt)
+    log(f"gap {go}/{ge}, X {x}: {len(r.oids)} pairs, {n} traced HSPs equal the oracle")
+    assert n > 10
