@@ -1,10 +1,13 @@
 """Autotuner for the batched prelim stage.
 
 Picks the launch shape (waves per seed workgroup), the query-size bin edge (diagonal-ring
-length of the first bin; bins double from there), the pack length for short queries and the
-gapped-stage routing by timing batched runs of a sample batch over the resident database, one
-field at a time (coordinate descent from the built-in choices).  Times are device times of the
-whole run (blz_run_stats.ms_total), so host result handling does not enter the choice.
+length of the first bin; bins double from there), the pack length for short queries, where the
+lookup tables live (global memory, entries in LDS, or prefix and entries in LDS: the LDS tile of
+the seed kernel), whether short subjects share a wave's scan (subject groups: the engine's
+subject-length rule, in place of fixed length bins), the gapped-stage routing and its lanes per
+DP row, by timing batched runs of a sample batch over the resident database, one field at a
+time (coordinate descent from the built-in choices).  Times are device times of the whole run
+(blz_run_stats.ms_total), so host result handling does not enter the choice.
 
 No reference counterpart: the reference hard-codes 64 threads per subject and its bins
 (BLAZE_gpu/gpuBLAZE.cu:1639; BLAZE_gpu/util.hpp:124-131).
@@ -17,7 +20,10 @@ CANDIDATES: dict[str, list[int]] = {
     "seed_waves": [0, 6, 8, 12, 16],
     "bin_ring": [0, 800, 1600, 2400],
     "pack_len": [0, 600, 1400],
+    "seed_table": [0, 1, 2, 3],
+    "subject_groups": [0, 1],
     "gapped_wave_classes": [0, 8],
+    "gapped_lanes": [0, 64],
 }
 
 
@@ -49,7 +55,12 @@ def tune(engine, qhs, candidates: dict[str, list[int]] | None = None, repeats: i
             if v == best[key]:
                 continue
             trial = {**best, key: v}
-            ms = time_batch(engine, qhs, trial, repeats)
+            try:
+                ms = time_batch(engine, qhs, trial, repeats)
+            except RuntimeError as e:  # a choice this batch cannot run (e.g. tables past the LDS)
+                if log:
+                    log(f"tune: {key}={v} refused ({e})")
+                continue
             if log:
                 log(f"tune: {key}={v} {ms:.2f} ms")
             if ms < best_ms * (1.0 - margin):  # keep a change only when it is clearly faster
