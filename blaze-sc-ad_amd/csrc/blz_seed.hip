@@ -68,6 +68,7 @@ constexpr int kTaskCap = 128;
 constexpr int kDeferCap = 256;
 constexpr int kHitBuf = 128;                // most hits of one step that go through the queue
 constexpr int kHitQueue = 192;              // queued hits: fewer than 64 waiting + one step
+constexpr int kHitSlack = 8;                // entries past the queue that a step's inline stores may hit
 constexpr int kSubjBuf = 1024;              // subject letters staged in LDS per wave
 constexpr int kSeqPad = 16;                 // zero letters before and after a sequence staged in LDS
 constexpr int kMatStride = 32;              // LDS matrix row: 28 scores, 4 unused (64 bytes)
@@ -111,7 +112,7 @@ struct WaveState {
 __host__ __device__ constexpr uint32_t al16(uint32_t b) { return (b + 15u) & ~15u; }
 
 __host__ __device__ inline uint32_t wave_lds_bytes(uint32_t M, uint32_t C) {
-    return al16(2 * M + 4 * (C + 68 + 64 + 4 + kHitQueue) + sizeof(Task) * kTaskCap + sizeof(uint32_t) * kDeferCap +
+    return al16(2 * M + 4 * (C + 68 + 64 + 4 + kHitQueue + kHitSlack) + sizeof(Task) * kTaskCap + sizeof(uint32_t) * kDeferCap +
                 kSubjBuf + 2 * kSeqPad + sizeof(Group));
 }
 
@@ -267,29 +268,44 @@ __device__ __forceinline__ int two_hit_walk(const Ctx &c, SP S, int slen, int s_
     return left_score > right_score ? left_score : right_score;
 }
 
-// Two-hit state machine for one hit (q, s) whose diagonal slot holds st.
+// Lengths of the wave's task and deferred-hit lists.  Wave-uniform, so they live in scalar
+// registers; entries are placed by ballot prefix counts.
+struct Lists {
+    uint32_t nt = 0;  // tasks
+    uint32_t nd = 0;  // deferred hits
+};
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Two-hit state machine for the hits of the lanes with win set, each (q, s) on its own diagonal
+// whose slot holds st (NCBI's word finder, reference restatement gpuBLAZE.cu:1025-1049):
+//   pending diagonal: the hit waits behind the queued extension (deferred list);
+//   flag (a right extension reached barrier - 1): the hit is skipped unless it lies at or past
+//     the barrier, which clears the flag and makes it the last hit;
+//   otherwise, with diff the distance to the diagonal's last hit: diff >= window makes it the
+//     last hit, diff < word overlaps the last hit and is skipped, anything between becomes an
+//     extension task and marks the diagonal pending.
+// Computed without branches; every lane of the wave calls it.
 template <class R>
-__device__ __forceinline__ void apply_hit(const Ctx &c, WaveState<R> &w, int q, int s, uint32_t slot, uint32_t st) {
+__device__ __forceinline__ void apply_hits(const Ctx &c, WaveState<R> &w, bool win, int q, int s, uint32_t slot,
+                                           uint32_t st, Lists &l) {
     constexpr uint32_t kFlag = RingFmt<R>::kFlag, kPending = RingFmt<R>::kPending, kPosMask = RingFmt<R>::kPosMask;
     const uint32_t sabs = (uint32_t)s;
-    if (st & kPending) {
-        const uint32_t k = atomicAdd(&w.cnt[1], 1u);
-        w.defer[k] = (uint32_t)q | ((uint32_t)s << 16);
-        return;
-    }
-    if (st & kFlag) {
-        if (sabs >= (st & kPosMask) - 1u) w.ring[slot] = (R)(sabs + 1u);
-        return;
-    }
+    const bool pnd = win && (st & kPending) != 0u;
+    const bool plain = win && (st & (kPending | kFlag)) == 0u;
     const uint32_t diff = st ? sabs - (st - 1u) : 0x7FFFFFFFu;
-    if (diff >= (uint32_t)c.window) {
-        w.ring[slot] = (R)(sabs + 1u);
-        return;
-    }
-    if (diff < (uint32_t)kWord) return;
-    const uint32_t t = atomicAdd(&w.cnt[0], 1u);
-    w.task[t] = Task{(uint32_t)q | (slot << 16), (uint32_t)s | ((st - 1u + kWord) << 16)};
-    w.ring[slot] = (R)(kPending | t);
+    const bool far = diff >= (uint32_t)c.window;
+    const bool tsk = plain && !far && diff >= (uint32_t)kWord;
+    const bool rec = (plain && far) || (win && (st & kFlag) != 0u && sabs >= (st & kPosMask) - 1u);
+    const uint64_t dm = __ballot(pnd), tm = __ballot(tsk);
+    const uint32_t t = l.nt + lanes_below(tm);
+    if (pnd) w.defer[l.nd + lanes_below(dm)] = (uint32_t)q | (sabs << 16);
+    if (tsk) w.task[t] = Task{(uint32_t)q | (slot << 16), sabs | ((st - 1u + kWord) << 16)};
+    if (tsk || rec) w.ring[slot] = (R)(tsk ? (kPending | t) : sabs + 1u);
+    l.nd += (uint32_t)__popcll(dm);
+    l.nt += (uint32_t)__popcll(tm);
 }
 
 // One claim pass: of the pending lanes whose diagonals hash to the same claim entry, the
@@ -298,7 +314,7 @@ __device__ __forceinline__ void apply_hit(const Ctx &c, WaveState<R> &w, int q, 
 // winner's state is current, since no lane writes a ring slot before the claims are settled.
 template <class R>
 __device__ __forceinline__ bool claim_pass(const Ctx &c, WaveState<R> &w, bool pend, int q, int s, uint32_t slot,
-                                           uint32_t rank) {
+                                           uint32_t rank, Lists &l) {
     const uint32_t h = slot & c.claim_mask;
     uint32_t st = 0;
     if (pend) {
@@ -308,19 +324,18 @@ __device__ __forceinline__ bool claim_pass(const Ctx &c, WaveState<R> &w, bool p
     ring_sync<R>();
     const bool win = pend && w.claim[h] == rank;
     ring_sync<R>();
-    if (win) {
-        w.claim[h] = 0xFFFFFFFFu;
-        apply_hit(c, w, q, s, slot, st);
-    }
+    if (win) w.claim[h] = 0xFFFFFFFFu;
+    apply_hits(c, w, win, q, s, slot, st, l);
     ring_sync<R>();
     return pend && !win;
 }
 
 // Applies up to one hit per lane; same-diagonal hits go earliest-rank first.
 template <class R>
-__device__ __forceinline__ void resolve_round(const Ctx &c, WaveState<R> &w, bool pend, int q, int s, uint32_t rank) {
+__device__ __forceinline__ void resolve_round(const Ctx &c, WaveState<R> &w, bool pend, int q, int s, uint32_t rank,
+                                              Lists &l) {
     const uint32_t slot = ring_slot(c, q, s);
-    while (__ballot(pend)) pend = claim_pass(c, w, pend, q, s, slot, rank);
+    while (__ballot(pend)) pend = claim_pass(c, w, pend, q, s, slot, rank, l);
 }
 
 // Extends every queued pair (one lane each), writes the outcome to its diagonal, then replays
@@ -329,13 +344,13 @@ __device__ __forceinline__ void resolve_round(const Ctx &c, WaveState<R> &w, boo
 // its virtual position, its letters from the subject's place in the buffer.  Staged init hits
 // carry their subject's index in bits 24.. of len and subject-relative offsets.
 template <bool kPad, bool kGrouped, class R, class SP>
-__device__ void flush(const Ctx &c, WaveState<R> &w, SP S, int slen, int ng, uint32_t &n_ext) {
+__device__ void flush(const Ctx &c, WaveState<R> &w, SP S, int slen, int ng, uint32_t &n_ext, Lists &l) {
     constexpr uint32_t kFlag = RingFmt<R>::kFlag;
     const int lane = lane_id();
     for (;;) {
         ring_sync<R>();
-        const uint32_t nt = w.cnt[0];
-        const uint32_t nd = w.cnt[1];
+        const uint32_t nt = l.nt;
+        const uint32_t nd = l.nd;
         if (nt == 0 && nd == 0) break;
         for (uint32_t t0 = 0; t0 < nt; t0 += 64) {
             const uint32_t t = t0 + lane;
@@ -362,8 +377,7 @@ __device__ void flush(const Ctx &c, WaveState<R> &w, SP S, int slen, int ng, uin
             }
         }
         ring_sync<R>();
-        if (lane == 0) w.cnt[0] = 0;
-        ring_sync<R>();
+        l.nt = 0;
         // replay the queued hits [0, nd) in order; new deferrals land at nd and beyond
         for (uint32_t r = 0; r < nd; r += 64) {
             const uint32_t i = r + lane;
@@ -374,13 +388,13 @@ __device__ void flush(const Ctx &c, WaveState<R> &w, SP S, int slen, int ng, uin
                 q = (int)(d & 0xFFFFu);
                 s = (int)(d >> 16);
             }
-            resolve_round(c, w, pend, q, s, i);
+            resolve_round(c, w, pend, q, s, i, l);
         }
         ring_sync<R>();
-        const uint32_t nd2 = w.cnt[1];
+        const uint32_t nd2 = l.nd;
         for (uint32_t i = nd + lane; i < nd2; i += 64) w.defer[i - nd] = w.defer[i];
         ring_sync<R>();
-        if (lane == 0) w.cnt[1] = nd2 - nd;
+        l.nd = nd2 - nd;
     }
 }
 
@@ -439,6 +453,7 @@ template <int kTable, bool kPad, bool kGrouped = false, class R, class SP>
 __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP S, int slen, uint32_t &n_hits,
                              uint32_t &n_ext, int ng = 1) {
     const int lane = lane_id();
+    Lists l;                              // the task and deferred lists (empty between subjects)
     uint32_t head = 0, tail = 0;
     int gi = 0;                           // subject of the group
     int p_begin = 0, p_end = slen, voff = 0;  // its letters [p_begin, p_end), virtual shift
@@ -454,16 +469,15 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
         // a claim round over the queue's first 64 hits, after the extensions when the task or
         // deferred lists fill
         if (tail - head >= 64u || (!input && tail > head)) {
-            if (w.cnt[0] > (uint32_t)(kTaskCap - 64) || w.cnt[1] > 64u) flush<kPad, kGrouped>(c, w, S, slen, ng, n_ext);
+            if (l.nt > (uint32_t)(kTaskCap - 64) || l.nd > 64u) flush<kPad, kGrouped>(c, w, S, slen, ng, n_ext, l);
             const uint32_t n = tail - head < 64u ? tail - head : 64u;
             const bool pend = (uint32_t)lane < n;
             const uint32_t e = pend ? w.hits[head + lane] : 0u;
             const int q = (int)(e >> 16), sp = (int)(e & 0xFFFFu);
-            const bool lost = claim_pass(c, w, pend, q, sp, ring_slot(c, q, sp), (uint32_t)lane);
+            const bool lost = claim_pass(c, w, pend, q, sp, ring_slot(c, q, sp), (uint32_t)lane, l);
             const uint64_t lm = __ballot(lost);
             head += n - (uint32_t)__popcll(lm);
-            if (lost)
-                w.hits[head + __builtin_amdgcn_mbcnt_hi((uint32_t)(lm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)lm, 0u))] = e;
+            if (lost) w.hits[head + lanes_below(lm)] = e;
             wave_sync();
             continue;
         }
@@ -520,10 +534,13 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
         uint32_t pa = 0, nh = 0;
         u32x4 r = {0u, 0u, 0u, 0u};  // kTable 0: the word's record
         if (p + kWord <= p_end) {
-            // (letters in the LDS buffer are doubled)
-            const uint32_t wd = (S[p] * 784u + S[p + 1] * 28u + S[p + 2]) >> (kPad ? 1 : 0);
+            // (letters in the LDS buffer are doubled, and so is their word index: a 16-byte
+            // record's byte offset is then the index times 8)
+            const uint32_t w2 = S[p] * 784u + S[p + 1] * 28u + S[p + 2];
+            const uint32_t wd = w2 >> (kPad ? 1 : 0);
             if constexpr (kTable == 0) {
-                r = gmem(reinterpret_cast<const u32x4 *>(t.rec))[wd];
+                r = *reinterpret_cast<const __attribute__((address_space(1))) u32x4 *>(
+                    gmem(reinterpret_cast<const uint8_t *>(t.rec)) + (w2 << (kPad ? 3 : 4)));
                 nh = r.x & 0xFFu;
                 pa = r.x >> 8;
                 if (nh == 0xFFu) {  // 255 entries or more, or a first entry past 2^24
@@ -543,10 +560,13 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
         if (H <= (uint32_t)kHitBuf) {
             make_room(H);
             // append: the record's inline entries, then four independent lookup loads per pass
-            // (indices clamped to the word's last entry).  A store past the lane's nh entries goes
-            // to the lane's own word of w.excl (unused outside dense steps), so the stores are
-            // unconditional and need no branches.
-            uint32_t *const sink = w.excl + lane;
+            // (indices clamped to the word's last entry).
+            // The six inline entries are stored unconditionally, highest index first, through one
+            // base address (a lane without hits uses its own word of w.excl, idle outside dense
+            // steps).  A store past a lane's nh entries lands either past the step's H hits (in
+            // the queue's slack) or on entry j' < j of a later lane, whose own store of entry j'
+            // comes later in program order and overwrites it; two lanes with hits never share an
+            // address in one store, since their lists start at distinct offsets.
             uint32_t *const dst = w.hits + tail + ex;
             uint32_t k0 = 0;
             if constexpr (kTable == 0) {
@@ -554,8 +574,11 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
                 const uint32_t v[kWordRecInline] = {(r.y << 16) | su, (r.y & 0xFFFF0000u) | su,
                                                     (r.z << 16) | su, (r.z & 0xFFFF0000u) | su,
                                                     (r.w << 16) | su, (r.w & 0xFFFF0000u) | su};
+                // volatile (LDS): the six stores stay separate and in this order
+                typedef volatile __attribute__((address_space(3))) uint32_t lds_u32;
+                lds_u32 *const d = (lds_u32 *)(nh ? dst : w.excl + lane);
 #pragma unroll
-                for (uint32_t j = 0; j < (uint32_t)kWordRecInline; j++) *(j < nh ? dst + j : sink) = v[j];
+                for (int j = kWordRecInline - 1; j >= 0; j--) d[j] = v[j];
                 k0 = kWordRecInline;
             }
             for (uint32_t k = k0; k < nh; k += 4) {
@@ -580,7 +603,7 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
         }
         c0 += 64;
     }
-    flush<kPad, kGrouped>(c, w, S, slen, ng, n_ext);
+    flush<kPad, kGrouped>(c, w, S, slen, ng, n_ext, l);
 }
 
 // The subject's init hits (staged by this wave) go to the pool, one work item per query of
@@ -821,7 +844,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
     w.p0 = w.excl + 68;
     w.cnt = w.p0 + 64;
     w.hits = w.cnt + 4;
-    w.task = reinterpret_cast<Task *>(w.hits + kHitQueue);
+    w.task = reinterpret_cast<Task *>(w.hits + kHitQueue + kHitSlack);
     w.defer = reinterpret_cast<uint32_t *>(w.task + kTaskCap);
     w.subj = reinterpret_cast<uint8_t *>(w.defer + kDeferCap) + kSeqPad;  // kSeqPad zero letters on both sides
     w.grp = reinterpret_cast<Group *>(w.subj + kSubjBuf + kSeqPad);
@@ -1050,7 +1073,7 @@ __global__ void __launch_bounds__(256) k_seed_retry(SeedArgs a) {
     w.p0 = w.excl + 68;
     w.cnt = w.p0 + 64;
     w.hits = w.cnt + 4;
-    w.task = reinterpret_cast<Task *>(w.hits + kHitQueue);
+    w.task = reinterpret_cast<Task *>(w.hits + kHitQueue + kHitSlack);
     w.defer = reinterpret_cast<uint32_t *>(w.task + kTaskCap);
     w.subj = reinterpret_cast<uint8_t *>(w.defer + kDeferCap);
     w.grp = nullptr;
