@@ -361,7 +361,8 @@ def run_batch(args) -> int:
             t = time.perf_counter()
             got, st = eng.run_batch(qhs)
             lat.append(time.perf_counter() - t)
-            log(f"step {i}: {1000 * lat[-1]:.1f} ms ({st['ms_total']:.1f} ms device)")
+            log(f"step {i}: {1000 * lat[-1]:.1f} ms ({st['ms_total']:.1f} ms device: seed {st['ms_seed']:.1f}, "
+                f"gapped {st['ms_gapped']:.1f})")
         tuning = eng.get_tuning()
         # the plain form: one query per launch, no packing, no graph
         eng.set_tuning(**{**tuning, "query_packing": 1, "graph": 1})
