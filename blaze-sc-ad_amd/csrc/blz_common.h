@@ -13,6 +13,11 @@ constexpr int kAlpha = 28;           // NCBIstdaa letters
 constexpr int kWords = 28 * 28 * 28; // word index a*784 + b*28 + c (gpuBLAZE.cu:991)
 constexpr int kWord = 3;             // word size
 constexpr int kRelBlocks = kWords / 64 + 1;  // 64-word blocks of the compact lookup prefix
+// Longest subject a database may hold: the seed kernels' 32-bit diagonal rings keep 30-bit positions, and
+// an init hit keeps its length in 24 bits.
+constexpr uint64_t kMaxSubjectLen = (1u << 24) - 1;
+// Longest subject the first seed kernel (seed variant 1) takes: it keeps 16-bit positions.
+constexpr uint64_t kSeed1MaxSubjectLen = 65535;
 constexpr int kHspMaxWindow = 11;    // gapped-start window (gpuBLAZE.cu:64)
 constexpr int kMinIntDp = -1073741824; // INT_MIN/2, the DP's "minus infinity"
 

@@ -392,11 +392,14 @@ blz_status blz_db_upload(blz_ctx *ctx, const uint8_t *residues, const uint64_t *
         ctx->err = "offsets[0] must be 0";
         return BLZ_EINVAL;
     }
-    for (int64_t i = 0; i < nsubj; i++)
-        if (offsets[i + 1] < offsets[i] || offsets[i + 1] - offsets[i] > 65535u) {
-            ctx->err = "subject offsets must be non-decreasing and subjects shorter than 65536 letters";
+    uint64_t max_len = 0;
+    for (int64_t i = 0; i < nsubj; i++) {
+        if (offsets[i + 1] < offsets[i] || offsets[i + 1] - offsets[i] > kMaxSubjectLen) {
+            ctx->err = "subject offsets must be non-decreasing and subjects at most 16,777,215 letters long";
             return BLZ_EINVAL;
         }
+        max_len = std::max<uint64_t>(max_len, offsets[i + 1] - offsets[i]);
+    }
     const uint64_t nres = offsets[nsubj];
     (void)hipSetDevice(ctx->device);
     blz_db *db = new blz_db();
@@ -404,6 +407,7 @@ blz_status blz_db_upload(blz_ctx *ctx, const uint8_t *residues, const uint64_t *
     db->uid = g_next_uid++;
     db->nsubj = nsubj;
     db->nres = nres;
+    db->max_len = max_len;
     // 64 padding bytes so word reads near the end stay inside the allocation
     if (!check(ctx, hipMalloc((void **)&db->res, nres + 64), "db residues") ||
         !check(ctx, hipMalloc((void **)&db->offs, sizeof(uint64_t) * (nsubj + 1)), "db offsets")) {
@@ -750,6 +754,10 @@ blz_status run_queries(blz_ctx *ctx, const blz_query *const *queries, int nq, co
     for (int i = 0; i < nq; i++)
         if (!queries[i] || queries[i]->ctx->device != ctx->device) return BLZ_EINVAL;
     const int variant = ctx->tune.seed_variant ? ctx->tune.seed_variant : 2;
+    if (variant == 1 && db->max_len > kSeed1MaxSubjectLen) {
+        ctx->err = "seed variant 1 takes subjects of at most 65,535 letters";
+        return BLZ_EINVAL;
+    }
     const size_t nsubj = (size_t)db->nsubj;
     hipStream_t st = ctx->stream;
     DevCounters *ctr = ctx->ctr;  // one counter block for the run; one item counter per seed launch

@@ -174,6 +174,7 @@ struct blz_db {
     uint64_t *offs = nullptr;   // device
     int64_t nsubj = 0;
     uint64_t nres = 0;
+    uint64_t max_len = 0;          // longest subject
     std::vector<uint64_t> h_offs;  // host copy of the offsets, for slicing by residues
     int64_t *long_ids = nullptr;   // device: subjects longer than the seed kernel's LDS ring takes
     int64_t n_long = 0;

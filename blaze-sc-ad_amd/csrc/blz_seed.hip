@@ -76,9 +76,14 @@ constexpr uint32_t kMatBytes = 28 * kMatStride * 2;
 static_assert(kSubjBuf <= kNarrowMax, "staged subjects use the LDS ring");
 static_assert(kWordRecInline == 6, "scan_subject writes six inline entries");
 
+// Positions in the hit queue, the task list and the deferred list are 16-bit offsets from the
+// first position of the current window (see scan_subject); the diagonal ring holds absolute
+// positions.  A task's first-hit end is stored plus kEndBias: it may lie up to window - 1 letters
+// before the window's first position.
+constexpr uint32_t kEndBias = 64;
 struct Task {
     uint32_t q_slot;  // query offset | diagonal slot << 16
-    uint32_t s_pack;  // subject offset of the second hit | end of the first hit's word << 16
+    uint32_t s_pack;  // second hit's offset | (end of the first hit's word - window start + kEndBias) << 16
 };
 
 // Short subjects scanned together by one wave (a group).  Their letters sit in the wave's LDS
@@ -290,9 +295,10 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
 // Computed without branches; every lane of the wave calls it.
 template <class R>
 __device__ __forceinline__ void apply_hits(const Ctx &c, WaveState<R> &w, bool win, int q, int s, uint32_t slot,
-                                           uint32_t st, Lists &l) {
+                                           uint32_t st, Lists &l, uint32_t w0) {
     constexpr uint32_t kFlag = RingFmt<R>::kFlag, kPending = RingFmt<R>::kPending, kPosMask = RingFmt<R>::kPosMask;
-    const uint32_t sabs = (uint32_t)s;
+    const uint32_t sloc = (uint32_t)s;  // offset in the window starting at w0
+    const uint32_t sabs = sloc + w0;
     const bool pnd = win && (st & kPending) != 0u;
     const bool plain = win && (st & (kPending | kFlag)) == 0u;
     const uint32_t diff = st ? sabs - (st - 1u) : 0x7FFFFFFFu;
@@ -301,8 +307,8 @@ __device__ __forceinline__ void apply_hits(const Ctx &c, WaveState<R> &w, bool w
     const bool rec = (plain && far) || (win && (st & kFlag) != 0u && sabs >= (st & kPosMask) - 1u);
     const uint64_t dm = __ballot(pnd), tm = __ballot(tsk);
     const uint32_t t = l.nt + lanes_below(tm);
-    if (pnd) w.defer[l.nd + lanes_below(dm)] = (uint32_t)q | (sabs << 16);
-    if (tsk) w.task[t] = Task{(uint32_t)q | (slot << 16), sabs | ((st - 1u + kWord) << 16)};
+    if (pnd) w.defer[l.nd + lanes_below(dm)] = (uint32_t)q | (sloc << 16);
+    if (tsk) w.task[t] = Task{(uint32_t)q | (slot << 16), sloc | ((st - 1u + kWord - w0 + kEndBias) << 16)};
     if (tsk || rec) w.ring[slot] = (R)(tsk ? (kPending | t) : sabs + 1u);
     l.nd += (uint32_t)__popcll(dm);
     l.nt += (uint32_t)__popcll(tm);
@@ -314,7 +320,7 @@ __device__ __forceinline__ void apply_hits(const Ctx &c, WaveState<R> &w, bool w
 // winner's state is current, since no lane writes a ring slot before the claims are settled.
 template <class R>
 __device__ __forceinline__ bool claim_pass(const Ctx &c, WaveState<R> &w, bool pend, int q, int s, uint32_t slot,
-                                           uint32_t rank, Lists &l) {
+                                           uint32_t rank, Lists &l, uint32_t w0) {
     const uint32_t h = slot & c.claim_mask;
     uint32_t st = 0;
     if (pend) {
@@ -325,7 +331,7 @@ __device__ __forceinline__ bool claim_pass(const Ctx &c, WaveState<R> &w, bool p
     const bool win = pend && w.claim[h] == rank;
     ring_sync<R>();
     if (win) w.claim[h] = 0xFFFFFFFFu;
-    apply_hits(c, w, win, q, s, slot, st, l);
+    apply_hits(c, w, win, q, s, slot, st, l, w0);
     ring_sync<R>();
     return pend && !win;
 }
@@ -333,9 +339,9 @@ __device__ __forceinline__ bool claim_pass(const Ctx &c, WaveState<R> &w, bool p
 // Applies up to one hit per lane; same-diagonal hits go earliest-rank first.
 template <class R>
 __device__ __forceinline__ void resolve_round(const Ctx &c, WaveState<R> &w, bool pend, int q, int s, uint32_t rank,
-                                              Lists &l) {
+                                              Lists &l, uint32_t w0) {
     const uint32_t slot = ring_slot(c, q, s);
-    while (__ballot(pend)) pend = claim_pass(c, w, pend, q, s, slot, rank, l);
+    while (__ballot(pend)) pend = claim_pass(c, w, pend, q, s, slot, rank, l, w0);
 }
 
 // Extends every queued pair (one lane each), writes the outcome to its diagonal, then replays
@@ -344,7 +350,7 @@ __device__ __forceinline__ void resolve_round(const Ctx &c, WaveState<R> &w, boo
 // its virtual position, its letters from the subject's place in the buffer.  Staged init hits
 // carry their subject's index in bits 24.. of len and subject-relative offsets.
 template <bool kPad, bool kGrouped, class R, class SP>
-__device__ void flush(const Ctx &c, WaveState<R> &w, SP S, int slen, int ng, uint32_t &n_ext, Lists &l) {
+__device__ void flush(const Ctx &c, WaveState<R> &w, SP S, int slen, int ng, uint32_t &n_ext, Lists &l, uint32_t w0) {
     constexpr uint32_t kFlag = RingFmt<R>::kFlag;
     const int lane = lane_id();
     for (;;) {
@@ -358,7 +364,7 @@ __device__ void flush(const Ctx &c, WaveState<R> &w, SP S, int slen, int ng, uin
                 const Task tk = w.task[t];
                 const int q = (int)(tk.q_slot & 0xFFFFu);
                 const uint32_t slot = tk.q_slot >> 16;
-                const int ts = (int)(tk.s_pack & 0xFFFFu);
+                const int ts = (int)(tk.s_pack & 0xFFFFu) + (int)w0;  // absolute
                 int sub = 0, shift = 0, rel = 0;  // subject in the group, virtual shift, first letter
                 if constexpr (kGrouped) {
                     for (int j = 1; j < ng; j++) sub += ts >= (int)w.grp->vstart[j];
@@ -366,8 +372,9 @@ __device__ void flush(const Ctx &c, WaveState<R> &w, SP S, int slen, int ng, uin
                     rel = (int)w.grp->start[sub];
                 }
                 int hq, hs, hl, rext, slast;
-                const int score = two_hit_walk<kPad>(c, S, slen, (int)(tk.s_pack >> 16) - shift, ts - shift, q, &hq, &hs,
-                                                     &hl, &rext, &slast);
+                const int s_left = (int)(tk.s_pack >> 16) - (int)kEndBias + (int)w0;
+                const int score = two_hit_walk<kPad>(c, S, slen, s_left - shift, ts - shift, q, &hq, &hs, &hl, &rext,
+                                                     &slast);
                 n_ext++;
                 if (score >= c.cutoff) {
                     const uint32_t k = atomicAdd(&w.cnt[2], 1u);
@@ -388,7 +395,7 @@ __device__ void flush(const Ctx &c, WaveState<R> &w, SP S, int slen, int ng, uin
                 q = (int)(d & 0xFFFFu);
                 s = (int)(d >> 16);
             }
-            resolve_round(c, w, pend, q, s, i, l);
+            resolve_round(c, w, pend, q, s, i, l, w0);
         }
         ring_sync<R>();
         const uint32_t nd2 = l.nd;
@@ -449,6 +456,14 @@ __device__ __forceinline__ void word_range(const Tables &t, uint32_t wd, uint32_
 // The loop is one state machine with a single site for each of its phases (claim round with its
 // extension flush, dense expansion, word step), which keeps the inlined code and its registers
 // small.
+// A subject longer than kWindowFrom letters is scanned in windows of word positions, each a
+// multiple of the diagonal ring's length long (at most kWindowMax), so that positions in the
+// queue and the lists stay 16-bit offsets from the window's first position while a diagonal keeps
+// its ring slot; the ring holds absolute positions and carries over from window to window.  Each
+// window ends with its queue drained and a flush, which only extends pairs earlier than a later
+// flush would: the result does not depend on when flushes happen.
+constexpr int kWindowFrom = 60000;
+constexpr int kWindowMax = 32768;
 template <int kTable, bool kPad, bool kGrouped = false, class R, class SP>
 __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP S, int slen, uint32_t &n_hits,
                              uint32_t &n_ext, int ng = 1) {
@@ -461,20 +476,27 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
         p_begin = (int)w.grp->start[0];
         p_end = p_begin + (int)w.grp->len[0];
     }
+    uint32_t w0 = 0;                      // first word position of the window
+    int win = 0, w_end = 0x7FFFFFFF;      // window length (0: one window), end of the window
+    if (!kGrouped && slen > kWindowFrom) {
+        win = (int)(c.ring_len * ((uint32_t)kWindowMax / c.ring_len));
+        w_end = win;
+    }
     int c0 = p_begin;                     // next step
     uint32_t dense_h = 0, dense_r = 0;    // dense step in progress: its hits, those expanded
     int dense_c0 = 0, dense_voff = 0;
+  for (;;) {                              // windows
     bool input = true;                    // steps (or dense hits) remain
     for (;;) {
         // a claim round over the queue's first 64 hits, after the extensions when the task or
         // deferred lists fill
         if (tail - head >= 64u || (!input && tail > head)) {
-            if (l.nt > (uint32_t)(kTaskCap - 64) || l.nd > 64u) flush<kPad, kGrouped>(c, w, S, slen, ng, n_ext, l);
+            if (l.nt > (uint32_t)(kTaskCap - 64) || l.nd > 64u) flush<kPad, kGrouped>(c, w, S, slen, ng, n_ext, l, w0);
             const uint32_t n = tail - head < 64u ? tail - head : 64u;
             const bool pend = (uint32_t)lane < n;
             const uint32_t e = pend ? w.hits[head + lane] : 0u;
             const int q = (int)(e >> 16), sp = (int)(e & 0xFFFFu);
-            const bool lost = claim_pass(c, w, pend, q, sp, ring_slot(c, q, sp), (uint32_t)lane, l);
+            const bool lost = claim_pass(c, w, pend, q, sp, ring_slot(c, q, sp), (uint32_t)lane, l, w0);
             const uint64_t lm = __ballot(lost);
             head += n - (uint32_t)__popcll(lm);
             if (lost) w.hits[head + lanes_below(lm)] = e;
@@ -515,7 +537,7 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
             continue;
         }
         // the next step of 64 words
-        if (c0 + kWord > p_end) {  // this subject is done: the next one of the group, or the end
+        if (c0 + kWord > p_end || c0 >= w_end) {  // this subject (window) is done: the group's next, or the end
             if constexpr (kGrouped) {
                 if (gi + 1 < ng) {
                     gi++;
@@ -530,10 +552,10 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
             continue;
         }
         const int p = c0 + lane;
-        const int s = p + voff;  // position seen by the two-hit state
+        const int s = p + voff - (int)w0;  // position seen by the two-hit state, in the window
         uint32_t pa = 0, nh = 0;
         u32x4 r = {0u, 0u, 0u, 0u};  // kTable 0: the word's record
-        if (p + kWord <= p_end) {
+        if (p + kWord <= p_end && p < w_end) {
             // (letters in the LDS buffer are doubled, and so is their word index: a 16-byte
             // record's byte offset is then the index times 8)
             const uint32_t w2 = S[p] * 784u + S[p + 1] * 28u + S[p + 2];
@@ -599,11 +621,16 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
             dense_h = H;
             dense_r = 0;
             dense_c0 = c0;
-            dense_voff = voff;
+            dense_voff = voff - (int)w0;
         }
         c0 += 64;
     }
-    flush<kPad, kGrouped>(c, w, S, slen, ng, n_ext, l);
+    flush<kPad, kGrouped>(c, w, S, slen, ng, n_ext, l, w0);
+    if (win == 0 || w_end + kWord > p_end) break;
+    w0 = (uint32_t)w_end;  // the next window; the ring carries over
+    w_end = w_end + win < p_end ? w_end + win : p_end;
+    c0 = (int)w0;
+  }
 }
 
 // The subject's init hits (staged by this wave) go to the pool, one work item per query of

@@ -169,7 +169,8 @@ blz_status blz_ctx_set_tuning(blz_ctx *ctx, const blz_tuning *t);
 blz_status blz_ctx_get_tuning(const blz_ctx *ctx, blz_tuning *t);
 
 /* Upload a database once; it stays resident.  residues: concatenated NCBIstdaa letters;
- * offsets[nsubj+1]: subject i is residues[offsets[i] .. offsets[i+1]).  Replaces the host
+ * offsets[nsubj+1]: subject i is residues[offsets[i] .. offsets[i+1]), at most 16,777,215
+ * letters long (BLZ_EINVAL otherwise).  Replaces the host
  * gather and per-chunk H2D copies of BLAZE_GPU (gpuBLAZE.cu:1587-1618) and the DB arrays
  * BLAZE allocates in prelim_stage.cpp:1204-1215. */
 blz_status blz_db_upload(blz_ctx *ctx, const uint8_t *residues, const uint64_t *offsets,

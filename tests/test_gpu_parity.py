@@ -206,6 +206,50 @@ def test_long_subject_rings(engine):
     compare(engine, q, np.concatenate(subs), offs)
 
 
+def test_subjects_past_16_bit_positions(engine):
+    """Subjects of 65,536 letters and more (up to 150,000) are scanned in windows of word
+    positions (a multiple of the diagonal ring's length, at most 32,768), with 16-bit offsets in
+    the queue and lists and absolute positions in the ring.  Mutated query copies planted across
+    the window boundaries, at the ends and every 9,000 letters give the oracle's lists, also when
+    a tiny per-wave stage sends every long subject through the retry scan.  Seed variant 1 keeps
+    16-bit positions and refuses such a database."""
+    q = blz.synth_query(43, 700)
+    rng = np.random.default_rng(5)
+    subs = []
+    for k, L in enumerate([65536, 70001, 150000]):
+        s = blz.synth_query(700 + k, L).copy()
+        for pos in list(range(5, L - 800, 9000)) + [31900, 32100, 32300, 64300, 64600, 96500, L - 701]:
+            if pos + 700 > L:
+                continue
+            piece = q.copy()
+            mut = rng.random(700) < 0.3
+            piece[mut] = blz.synth_query(800 + pos, 700)[mut]
+            s[pos:pos + 700] = piece
+        subs.append(s)
+    subs.append(blz.synth_query(601, 300))
+    offs = np.zeros(len(subs) + 1, dtype=np.uint64)
+    offs[1:] = np.cumsum([len(s) for s in subs])
+    res = np.concatenate(subs)
+    saved = engine.get_tuning()
+    if saved["seed_variant"] == 1:
+        engine.upload_db(res, offs)
+        qh = engine.prepare(q)
+        try:
+            with pytest.raises(blz.BlzError) as e:
+                engine.run(qh)
+            assert e.value.status == 1
+        finally:
+            engine.free_query(qh)
+        return
+    stats, nsub = compare(engine, q, res, offs)
+    assert nsub == 3
+    engine.set_tuning(**{**saved, "cap_stage": 8})
+    try:
+        compare(engine, q, res, offs)
+    finally:
+        engine.set_tuning(**saved)
+
+
 def test_init_hit_cap(engine):
     """A subject with more init hits than the per-wave stage holds (6,704 > 4,096; low ungapped
     cutoff) and more HSPs than the wave pass keeps (1,799).  The batched kernel rescans it in its
@@ -252,17 +296,17 @@ def test_empty_db(engine):
 
 
 def test_upload_rejects_bad_input(engine):
-    """Letters outside NCBIstdaa (anywhere in the staged chunks) and subjects of 65,536 letters
-    or more are rejected with BLZ_EINVAL; nothing is left half-uploaded."""
+    """Letters outside NCBIstdaa (anywhere in the staged chunks) and subjects longer than
+    16,777,215 letters are rejected with BLZ_EINVAL; nothing is left half-uploaded."""
     res = np.ones(1000, np.uint8)
     res[777] = 28
     offs = np.array([0, 500, 1000], np.uint64)
     with pytest.raises(blz.BlzError) as e:
         engine.upload_db(res, offs)
     assert e.value.status == 1
-    big = np.ones(70000, np.uint8)
+    big = np.ones(1 << 24, np.uint8)
     with pytest.raises(blz.BlzError) as e:
-        engine.upload_db(big, np.array([0, 70000], np.uint64))
+        engine.upload_db(big, np.array([0, 1 << 24], np.uint64))
     assert e.value.status == 1
 
 

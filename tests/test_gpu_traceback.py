@@ -193,6 +193,41 @@ def test_edge_pairs_and_bad_input():
     assert got[0][oid][0].tolist() == wt.tolist() and got[0][oid][1].tolist() == wr.tolist()
 
 
+def test_subjects_past_16_bit_positions():
+    """Subjects of 65,536 letters and more (up to 150,000) with mutated query copies: every prelim
+    list traced, HSPs and edit scripts equal to the oracle's."""
+    q = blz.synth_query(43, 700)
+    rng = np.random.default_rng(5)
+    subs = []
+    for k, L in enumerate([65536, 70001, 150000]):
+        s = blz.synth_query(700 + k, L).copy()
+        for pos in list(range(5, L - 800, 9000)) + [31900, 32100, 32300, 64300, 64600, 96500, L - 701]:
+            if pos + 700 > L:
+                continue
+            piece = q.copy()
+            mut = rng.random(700) < 0.3
+            piece[mut] = blz.synth_query(800 + pos, 700)[mut]
+            s[pos:pos + 700] = piece
+        subs.append(s)
+    offs = np.zeros(len(subs) + 1, dtype=np.uint64)
+    offs[1:] = np.cumsum([len(s) for s in subs])
+    res = np.concatenate(subs)
+    with blz.Engine(0) as eng:
+        eng.upload_db(res, offs)
+        h = eng.prepare(q)
+        r, _ = eng.run(h)
+        got, st = eng.traceback([h], [r])
+        eng.free_query(h)
+    n = 0
+    for oid, hs in r.as_dict().items():
+        wt, wr = oracle.traceback_subject(q, res[int(offs[oid]):int(offs[oid + 1])], hs)
+        gt, gr = got[0][oid]
+        assert gt.tolist() == wt.tolist() and gr.tolist() == wr.tolist(), oid
+        n += len(wt)
+    log(f"long subjects: {len(r.oids)} pairs, {n} traced HSPs equal the oracle")
+    assert len(r.oids) == 3 and n > 10
+
+
 @pytest.mark.parametrize("go,ge,x", [(10, 2, 40), (13, 1, 90), (7, 3, 30)])
 def test_other_gap_costs_and_xdrops(go, ge, x):
     """Gap costs other than 11/1 (an extension of 2 or 3 takes the division and the multiples of
