@@ -179,8 +179,9 @@ void blz_db_free(blz_db *db);
 int64_t blz_db_num_subjects(const blz_db *db);
 uint64_t blz_db_num_residues(const blz_db *db);
 
-/* Build a query's neighbourhood lookup table on the host and upload it.  Replaces the query
- * preprocessing of prelim_stage.cpp:340-571 and copyConstantMemory (gpuBLAZE.cu:1823-1842). */
+/* Build a query's neighbourhood lookup table on the host and upload it; qlen 1..16,000 (the
+ * seed kernel keeps a query's diagonal state, qlen + window + 128 entries, in LDS).  Replaces the
+ * query preprocessing of prelim_stage.cpp:340-571 and copyConstantMemory (gpuBLAZE.cu:1823-1842). */
 blz_status blz_query_prepare(blz_ctx *ctx, const uint8_t *query, int32_t qlen,
                              const blz_params *params, blz_query **out);
 void blz_query_free(blz_query *q);
