@@ -516,17 +516,23 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
                 tail = rest;
             }
         };
-        // 64 more hits of a dense step
+        // 64 more hits of a dense step: hit g belongs to the last position lane whose range of
+        // the step's hits, [excl[i], excl[i + 1]), starts at or before g.  Each lane with hits
+        // marks the slot where its range starts in this pass's 64 (slot 0 for a range begun
+        // earlier) with an LDS max, in the queue slots the pass is about to fill, and a prefix
+        // max over the slots gives every hit its lane.
         if (dense_r < dense_h) {
             make_room(64u);
-            const uint32_t g = dense_r + lane;
+            const uint32_t g0 = dense_r, g = g0 + lane;
+            uint32_t *const mark = w.hits + tail;
+            mark[lane] = 0u;
+            wave_sync();
+            const uint32_t ex_i = w.excl[lane];
+            if (w.excl[lane + 1] > ex_i && ex_i < g0 + 64u) atomicMax(&mark[ex_i > g0 ? ex_i - g0 : 0u], (uint32_t)lane + 1u);
+            wave_sync();
+            const int lo = wave_inclusive_max((int)mark[lane] - 1);  // slot 0 always holds a mark
+            wave_sync();
             if (g < dense_h) {
-                int lo = 0, hi = 63;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (w.excl[mid] <= g) lo = mid;
-                    else hi = mid - 1;
-                }
                 const uint32_t qo = entry<kTable>(t, w.p0[lo] + (g - w.excl[lo]));
                 w.hits[tail + lane] = (qo << 16) | (uint32_t)(dense_c0 + lo + dense_voff);
             }
@@ -616,6 +622,7 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
         } else {
             // dense step: its hits enter the queue 64 at a time (above)
             w.excl[lane] = ex;
+            if (lane == 0) w.excl[64] = H;
             w.p0[lane] = pa;
             wave_sync();
             dense_h = H;
