@@ -452,7 +452,8 @@ __device__ __forceinline__ void word_range(const Tables &t, uint32_t wd, uint32_
 // front of the queue, in order, and lead the next round: per diagonal the order stays the
 // subject order, since a lost hit is behind its diagonal's winner and ahead of every later hit.
 // A step whose hits exceed kHitBuf (low-complexity letters) is expanded into the queue 64 hits
-// at a time, each hit finding its position by binary search.
+// at a time, each hit finding its position by a prefix max over the marked starts of the lanes'
+// ranges.
 // The loop is one state machine with a single site for each of its phases (claim round with its
 // extension flush, dense expansion, word step), which keeps the inlined code and its registers
 // small.
@@ -476,9 +477,12 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
         p_begin = (int)w.grp->start[0];
         p_end = p_begin + (int)w.grp->len[0];
     }
+    // only the launches with 32-bit rings in global memory (long subjects, the retry scan) take
+    // subjects past kWindowFrom; the others compile without windows
+    constexpr bool kWindows = sizeof(R) == 4 && !kGrouped;
     uint32_t w0 = 0;                      // first word position of the window
     int win = 0, w_end = 0x7FFFFFFF;      // window length (0: one window), end of the window
-    if (!kGrouped && slen > kWindowFrom) {
+    if (kWindows && slen > kWindowFrom) {
         win = (int)(c.ring_len * ((uint32_t)kWindowMax / c.ring_len));
         w_end = win;
     }
@@ -633,7 +637,7 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
         c0 += 64;
     }
     flush<kPad, kGrouped>(c, w, S, slen, ng, n_ext, l, w0);
-    if (win == 0 || w_end + kWord > p_end) break;
+    if (!kWindows || win == 0 || w_end + kWord > p_end) break;
     w0 = (uint32_t)w_end;  // the next window; the ring carries over
     w_end = w_end + win < p_end ? w_end + win : p_end;
     c0 = (int)w0;
