@@ -92,7 +92,7 @@ class Tuning(C.Structure):
                 ("cap_retry", C.c_int32), ("cap_ovf", C.c_int32), ("cap_wide_ring", C.c_int32),
                 ("cap_wide_saved", C.c_int32), ("cap_work", C.c_int64), ("cap_pool", C.c_int64),
                 ("cap_hsps", C.c_int64), ("cap_tb_bytes", C.c_int64), ("cap_tb_runs", C.c_int64),
-                ("tb_lanes", C.c_int32), ("reserved_tb", C.c_int32)]
+                ("tb_lanes", C.c_int32), ("gapped_width", C.c_int32)]
 
 
 class ServerConfig(C.Structure):

@@ -24,6 +24,7 @@ CANDIDATES: dict[str, list[int]] = {
     "subject_groups": [0, 1],
     "gapped_wave_classes": [0, 8],
     "gapped_lanes": [0, 64],
+    "gapped_width": [0, 16],
 }
 
 

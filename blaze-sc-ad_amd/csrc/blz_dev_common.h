@@ -246,4 +246,33 @@ struct Seg<16> {
     static __device__ __forceinline__ int reduce_max(int x) { return bcast(incl_max(x)); }
 };
 
+// ---------------------------------------------------------------------------------------
+// Pairs of signed 16-bit values in one 32-bit register (low half first), for the packed DP:
+// v_pk_* instructions act on both halves; adds and subtracts saturate at the int16 range.
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+constexpr uint32_t kNeg16x2 = 0x80008000u;  // (-32768, -32768): minus infinity in both halves
+__device__ __forceinline__ s16x2 as_s16x2(uint32_t x) { return __builtin_bit_cast(s16x2, x); }
+__device__ __forceinline__ uint32_t as_u32(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ s16x2 pk_splat(int x) { return s16x2{(short)x, (short)x}; }
+__device__ __forceinline__ s16x2 pk_max(s16x2 a, s16x2 b) { return __builtin_elementwise_max(a, b); }
+__device__ __forceinline__ s16x2 pk_add(s16x2 a, s16x2 b) { return __builtin_elementwise_add_sat(a, b); }
+__device__ __forceinline__ s16x2 pk_sub(s16x2 a, s16x2 b) { return __builtin_elementwise_sub_sat(a, b); }
+// bits of a where mask is set, else bits of b
+__device__ __forceinline__ uint32_t pk_select(uint32_t mask, uint32_t a, uint32_t b) { return (a & mask) | (b & ~mask); }
+// Inclusive prefix max over a 16-lane DPP row of a lane's two columns (low, high).  The lane's
+// own maximum goes to the high half, the row's prefix max of those runs on 32-bit values with
+// fused v_max_i32_dpp steps (a lane without a source keeps its own value, which a max ignores),
+// and the low half takes the maximum of the lanes before it.
+__device__ __forceinline__ s16x2 pk_incl_max16(s16x2 v) {
+    constexpr int kMin = -2147483647 - 1;
+    const int lo = v.x;
+    int x = max((int)v.y, lo);
+    x = max(x, __builtin_amdgcn_update_dpp(kMin, x, 0x111, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(kMin, x, 0x112, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(kMin, x, 0x114, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(kMin, x, 0x118, 0xf, 0xf, false));
+    const int before = __builtin_amdgcn_update_dpp(kMin, x, 0x111, 0xf, 0xf, false);  // lanes before
+    return s16x2{(short)max(lo, before), (short)x};
+}
+
 }  // namespace blz

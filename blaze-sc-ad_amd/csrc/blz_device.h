@@ -137,8 +137,9 @@ hipError_t launch_gapped_wave(const GappedArgs &a, int blocks, hipStream_t st);
 // Resident k_gapped_wave workgroups (one wave each) per CU.
 int gapped_wave_blocks_per_cu();
 // Four subjects per wave (16-lane segments); saved-HSP scratch of 4 x saved_cap per workgroup.
-hipError_t launch_gapped_seg(const GappedArgs &a, int blocks, hipStream_t st);
-int gapped_seg_blocks_per_cu();
+// packed: the 16-bit DP (two columns per lane, pairs of int16); else 32-bit, one column per lane
+hipError_t launch_gapped_seg(const GappedArgs &a, int blocks, bool packed, hipStream_t st);
+int gapped_seg_blocks_per_cu(bool packed);
 
 // ---- traceback stage ----------------------------------------------------------------------
 // Traced HSP, same layout as blz_tb_hsp in blz_prelim.h: the HSP fields, then identities,

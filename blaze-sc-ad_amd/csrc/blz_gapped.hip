@@ -20,6 +20,8 @@
 //     few subjects whose band does not fit the LDS ring.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "blz_common.h"
 #include "blz_dev_common.h"
 #include "blz_device.h"
@@ -615,13 +617,25 @@ __global__ void __launch_bounds__(64) k_gapped_wave(GappedArgs a) {
 // and its gapped start, one DP row), so a segment whose DP ends starts the next at once instead
 // of waiting at a loop exit for the other segments' rows.  Same per-subject steps, arithmetic,
 // caps and outcomes as gapped_item / wave_semi_gapped.
+//
+// kPk: packed 16-bit DP.  Each lane holds two adjacent columns (a 16-lane chunk spans 32), and
+// the row's arithmetic runs on pairs of int16 with v_pk_* instructions, adds and subtracts
+// saturating.  Scores are kept relative to the DP's start (0): a cell's score is at most 11 per
+// row, so for min(rows, columns) <= kPkMaxRows every live value lies between -(X + open + extend)
+// and 31,900, and minus infinity (-32,768) stays minus infinity under saturation; the X-drop test
+// never sees a clipped live value.  The row-gap prefix max runs on (column - row start) x extend,
+// which the band keeps below 128 x extend.  A DP past those bounds sends its subject to the wide
+// pass, as a band past the ring does.  The ring holds one 32-bit word per column (H low, E high).
+constexpr int kPkMaxRows = 2900;
+template <bool kPk>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_gapped_seg(GappedArgs a) {
     using S = Seg<16>;
+    using RingCell = typename std::conditional<kPk, uint32_t, int2>::type;
     constexpr int kLetRing = 256;   // letters of a DP's columns per segment (the band and 64 ahead)
     constexpr int kZeroLetter = 28;  // scores 0 against every letter: the gapped start's own column
     static_assert(kSegRing + 64 <= kLetRing, "a band and the 64 letters filled past it fit the letter ring");
     __shared__ int16_t mat[28][32];
-    __shared__ int2 ring_mem[4][kSegRing];
+    __shared__ RingCell ring_mem[4][kSegRing];
     __shared__ uint8_t lett_mem[4][kLetRing];
     for (int i = threadIdx.x; i < 28 * 32; i += blockDim.x) mat[i / 32][i % 32] = i % 32 < 28 ? kBlosum62Dev[i / 32][i % 32] : 0;
     for (int i = threadIdx.x; i < 4 * kLetRing; i += blockDim.x) lett_mem[i / kLetRing][i % kLetRing] = kZeroLetter;
@@ -634,7 +648,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
     for (int b = 0; b < a.heavy_classes; b++) heavy += a.ctr->bucket_count[b];
     const uint32_t navail = *a.count < a.count_cap ? *a.count : a.count_cap;
     if (heavy > navail) heavy = navail;
-    int2 *ring = ring_mem[seg];
+    RingCell *ring = ring_mem[seg];
     uint8_t *lett = lett_mem[seg];
     constexpr int wmask = kSegRing - 1, W = kSegRing;
     Hsp *saved = a.hsp_scratch + ((uint64_t)blockIdx.x * 4 + seg) * a.saved_cap;
@@ -660,6 +674,20 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
     int dps = kRun;  // the DP's status: running, empty (no rows or columns), band past the ring
     int arow = 1, first = 0, size = 0, best = 0, ba = 0, bb = 0;
     int filled = 0;  // columns [0, filled) have their letters in the letter ring
+    s16x2 lane_bge{0, 0};  // kPk: this lane's two columns' offsets in a chunk, times the extension
+    // a ring cell holding (h, e)
+    auto cell = [](int h, int e) {
+        if constexpr (kPk) {
+            const int hh = h < -32768 ? -32768 : h, ee = e < -32768 ? -32768 : e;
+            return (RingCell)((uint32_t)(uint16_t)(int16_t)hh | ((uint32_t)(uint16_t)(int16_t)ee << 16));
+        } else {
+            return (RingCell)make_int2(h, e);
+        }
+    };
+    auto neg_cell = []() {
+        if constexpr (kPk) return (RingCell)kNeg16x2;
+        else return (RingCell)make_int2(kMinIntDp, kMinIntDp);
+    };
     unsigned long long nc = 0;
 
     // Letters of the DP's columns up to the band's end, 64 columns at a time; column 0 (the
@@ -690,6 +718,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
         best = 0; ba = 0; bb = 0; nc = 0; first = 0; arow = 1;
         dps = N <= 0 || M <= 0 ? kEmpty : kRun;
         if (dps == kEmpty) return;
+        if constexpr (kPk) {  // outside the 16-bit bounds: the wide pass takes the subject
+            if ((M < N ? M : N) > kPkMaxRows || X + goe > 8192 || 128 * ge > 8192) {
+                dps = kOvf;
+                return;
+            }
+            lane_bge = s16x2{(short)(2 * sl * ge), (short)((2 * sl + 1) * ge)};
+        }
         const int k0 = (X - goe) / ge + 1;  // row 0: a run of gaps while it stays within the X-drop
         size = (N < k0 ? N : k0) + 1;
         if (size > W) {
@@ -698,7 +733,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
         }
         for (int b = sl; b < size; b += 16) {
             const int h = b == 0 ? 0 : -goe - (b - 1) * ge;
-            ring[b & wmask] = make_int2(h, h - goe);
+            ring[b & wmask] = cell(h, h - goe);
         }
         filled = 0;
         fill_letters();
@@ -797,6 +832,61 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
                 int carry_hp = kMinIntDp;  // previous row's H left of this chunk
                 int fa = 0x7fffffff, la = -1;  // this lane's first and last surviving column
                 int rmax = kMinIntDp, rmax_b = -1;
+                const int rs = first;      // kPk: the row's first column (gap offsets count from it)
+                if constexpr (kPk) {
+                    s16x2 cg = pk_splat(-32768), ch = pk_splat(best);  // carry_g, carry_h
+                    uint32_t hp_old = kNeg16x2;  // high half: previous row's H left of the chunk
+                    s16x2 bge = lane_bge;        // (column - rs) x extend, per column
+                    const s16x2 go2 = pk_splat(gopen), goe2 = pk_splat(goe), ge2 = pk_splat(ge), X2 = pk_splat(X);
+                    const s16x2 step2 = pk_splat(32 * ge);
+                    for (int c0 = first; c0 < size; c0 += 32) {
+                        const int b0 = c0 + 2 * sl;
+                        // lanes past the band read ring and letter slots they do not use
+                        const uint32_t w0 = ring[b0 & wmask], w1 = ring[(b0 + 1) & wmask];
+                        const s16x2 hprev = as_s16x2(__builtin_amdgcn_perm(w1, w0, 0x05040100u));
+                        const s16x2 eprev = as_s16x2(__builtin_amdgcn_perm(w1, w0, 0x07060302u));
+                        const int sc0 = mrow[lett[b0 & (kLetRing - 1)]], sc1 = mrow[lett[(b0 + 1) & (kLetRing - 1)]];
+                        const s16x2 sb = as_s16x2(__builtin_amdgcn_perm((uint32_t)sc1, (uint32_t)sc0, 0x05040100u));
+                        // H of the column left of each: the previous lane's high column, then our low one
+                        const uint32_t hsh = __builtin_amdgcn_update_dpp(hp_old, as_u32(hprev), 0x111, 0xf, 0xf, false);
+                        const s16x2 hpl = as_s16x2(__builtin_amdgcn_perm(as_u32(hprev), hsh, 0x05040302u));
+                        const int cnt = size - b0;  // columns of this lane inside the band: 2, 1 or none
+                        const uint32_t vm = cnt >= 2 ? 0xFFFFFFFFu : (cnt == 1 ? 0x0000FFFFu : 0u);
+                        const s16x2 de = as_s16x2(pk_select(vm, as_u32(pk_max(pk_add(hpl, sb), eprev)), kNeg16x2));
+                        const s16x2 gin = pk_incl_max16(pk_add(de, bge));
+                        const s16x2 din = pk_incl_max16(de);
+                        const uint32_t gsh = __builtin_amdgcn_update_dpp(kNeg16x2, as_u32(gin), 0x111, 0xf, 0xf, false);
+                        const s16x2 gex = pk_max(as_s16x2(__builtin_amdgcn_perm(as_u32(gin), gsh, 0x05040302u)), cg);
+                        const s16x2 h = pk_max(de, pk_sub(gex, pk_add(bge, go2)));
+                        // alive: X - ((best before the cell) - H) >= 0, both halves at once
+                        const s16x2 dead = pk_sub(X2, pk_sub(pk_max(din, ch), h)) >> 15;
+                        const uint32_t alive = ~as_u32(dead) & vm;
+                        const bool al0 = (alive & 0xFFFFu) != 0, al1 = (alive >> 16) != 0;
+                        fa = min(fa, al0 ? b0 : (al1 ? b0 + 1 : 0x7fffffff));
+                        la = al1 ? b0 + 1 : (al0 ? b0 : la);
+                        const s16x2 cmv = as_s16x2(S::bcast((int)as_u32(din)));  // high half: the chunk's max
+                        const int cm = cmv.y;
+                        if (cm > ch.y) {  // a new best in this chunk: its first cell (H equals D/E there)
+                            rmax = cm;
+                            const uint64_t m0 = S::ballot(de.x == cm), m1 = S::ballot(de.y == cm);
+                            const int c_lo = m0 ? 2 * (int)__builtin_ctzll(m0) : 0x7fffffff;
+                            const int c_hi = m1 ? 2 * (int)__builtin_ctzll(m1) + 1 : 0x7fffffff;
+                            rmax_b = c0 + min(c_lo, c_hi);
+                        }
+                        // new (H, E) for the band's columns (minus infinity where the X-drop fails);
+                        // a column past the band gets its own old word back
+                        const uint32_t hs = pk_select(alive, as_u32(h), kNeg16x2);
+                        const uint32_t es = pk_select(alive, as_u32(pk_max(pk_sub(h, goe2), pk_sub(eprev, ge2))), kNeg16x2);
+                        const uint32_t ho = pk_select(vm, hs, as_u32(hprev)), eo = pk_select(vm, es, as_u32(eprev));
+                        ring[b0 & wmask] = __builtin_amdgcn_perm(eo, ho, 0x05040100u);
+                        ring[(b0 + 1) & wmask] = __builtin_amdgcn_perm(eo, ho, 0x07060302u);
+                        cg = pk_max(cg, as_s16x2(S::bcast((int)as_u32(gin))).yy);
+                        ch = pk_max(ch, cmv.yy);
+                        hp_old = (uint32_t)S::bcast((int)as_u32(hprev));
+                        bge = pk_add(bge, step2);
+                    }
+                    carry_g = cg.x;
+                } else
                 for (int c0 = first; c0 < size; c0 += 16) {
                     const int b = c0 + sl;
                     const bool valid = b < size;
@@ -844,7 +934,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
                     if (la < size - 1) {
                         size = la + 1;
                     } else {
-                        const int gr = carry_g - goe - (size - 1) * ge;  // row gap one past the last cell
+                        // row gap one past the last cell (kPk: carry_g counts columns from rs)
+                        const int gr = carry_g - goe - (size - 1 - (kPk ? rs : 0)) * ge;
                         if (gr >= best - X) {
                             // cells of the run (a division only for a gap extension other than 1)
                             int kk = (ge == 1 ? gr - (best - X) : (gr - (best - X)) / ge) + 1;
@@ -854,7 +945,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
                             } else {
                                 for (int i = sl; i < kk; i += 16) {
                                     const int hv = gr - i * ge;
-                                    ring[(size + i) & wmask] = make_int2(hv, hv - goe);
+                                    ring[(size + i) & wmask] = cell(hv, hv - goe);
                                 }
                                 size += kk;
                             }
@@ -864,7 +955,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
                         if (size + 1 - first > W) {
                             dps = kOvf;
                         } else {
-                            if (sl == 0) ring[size & wmask] = make_int2(kMinIntDp, kMinIntDp);
+                            if (sl == 0) ring[size & wmask] = neg_cell();
                             size++;
                         }
                     }
@@ -915,17 +1006,25 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
     }
 }
 
-int gapped_seg_blocks_per_cu() {
-    static const int n = [] {
+int gapped_seg_blocks_per_cu(bool packed) {
+    static const int n32 = [] {
         int k = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, k_gapped_seg, 64, 0) != hipSuccess || k <= 0) k = 8;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, k_gapped_seg<false>, 64, 0) != hipSuccess || k <= 0) k = 8;
         return k;
     }();
-    return n;
+    static const int n16 = [] {
+        int k = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, k_gapped_seg<true>, 64, 0) != hipSuccess || k <= 0) k = 8;
+        return k;
+    }();
+    return packed ? n16 : n32;
 }
 
-hipError_t launch_gapped_seg(const GappedArgs &a, int blocks, hipStream_t st) {
-    hipLaunchKernelGGL(k_gapped_seg, dim3(blocks), dim3(64), 0, st, a);
+hipError_t launch_gapped_seg(const GappedArgs &a, int blocks, bool packed, hipStream_t st) {
+    if (packed)
+        hipLaunchKernelGGL(k_gapped_seg<true>, dim3(blocks), dim3(64), 0, st, a);
+    else
+        hipLaunchKernelGGL(k_gapped_seg<false>, dim3(blocks), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 

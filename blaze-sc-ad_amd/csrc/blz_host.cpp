@@ -362,7 +362,8 @@ blz_status blz_ctx_set_tuning(blz_ctx *ctx, const blz_tuning *t) {
         t->subject_groups > 1 || t->overflow_mode < 0 || t->overflow_mode > 1 || t->cap_stage < 0 ||
         t->cap_retry < 0 || t->cap_ovf < 0 || t->cap_wide_ring < 0 || t->cap_wide_saved < 0 || t->cap_work < 0 ||
         t->cap_work > 0xFFFFFFFFll || t->cap_pool < 0 || t->cap_hsps < 0 ||
-        t->cap_tb_bytes < 0 || t->cap_tb_runs < 0 || (t->tb_lanes != 0 && t->tb_lanes != 16 && t->tb_lanes != 64))
+        t->cap_tb_bytes < 0 || t->cap_tb_runs < 0 || (t->tb_lanes != 0 && t->tb_lanes != 16 && t->tb_lanes != 64) ||
+        (t->gapped_width != 0 && t->gapped_width != 16 && t->gapped_width != 32))
         return BLZ_EINVAL;
     if (t->query_packing != ctx->tune.query_packing || t->pack_len != ctx->tune.pack_len) {
         // cached packs were made under the old choice
@@ -967,7 +968,8 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
     }
     const size_t lds_lanes = (size_t)(ctx->tune.gapped_blocks ? ctx->tune.gapped_blocks : ctx->num_cus * 4) * 64;
     const bool seg16 = ctx->tune.gapped_lanes != 64;  // four subjects per wave
-    const int heavy_blocks = ctx->num_cus * (seg16 ? gapped_seg_blocks_per_cu() : gapped_wave_blocks_per_cu());
+    const bool packed = ctx->tune.gapped_width == 16;  // 16-bit DP scores, two columns per lane
+    const int heavy_blocks = ctx->num_cus * (seg16 ? gapped_seg_blocks_per_cu(packed) : gapped_wave_blocks_per_cu());
     const size_t saved_records =
         std::max(lds_lanes * kSavedCap, (size_t)heavy_blocks * (seg16 ? 4 : 1) * kSavedCapWave);
     const size_t wide_lanes = (size_t)kGappedBlocks * 64;
@@ -1136,7 +1138,7 @@ blz_status plan_and_launch(blz_ctx *ctx, const blz_query *const *queries, int nq
         return (nsubj == 0 || check(ctx, launch_order(ctx->work, ctr, ctx->order, (uint32_t)work_cap, 512, st),
                                     "k_order launch")) &&
                (nsubj == 0 || ga.heavy_classes == 0 ||
-                check(ctx, seg16 ? launch_gapped_seg(gh, heavy_blocks, st) : launch_gapped_wave(gh, heavy_blocks, st),
+                check(ctx, seg16 ? launch_gapped_seg(gh, heavy_blocks, packed, st) : launch_gapped_wave(gh, heavy_blocks, st),
                       "k_gapped_wave launch")) &&
                (nsubj == 0 || ga.heavy_classes >= kBuckets ||
                 check(ctx, launch_gapped_lds(ga, lds_blocks, st), "k_gapped_lds launch")) &&

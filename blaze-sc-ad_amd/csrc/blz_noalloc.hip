@@ -225,7 +225,7 @@ extern "C" blz_status blz_prelim_run_noalloc(blz_ctx *ctx, const blz_query *q, c
     if (seed_lds_bytes(ring, waves) > ctx->lds_per_block)
         return fail(ctx, BLZ_EINVAL, "query too long for the diagonal ring in LDS");
     const int blocks = std::max(1, std::min(ctx->num_cus * 8 / waves, kNaSeedWaves / waves));
-    const int heavy_blocks = std::min(kNaMaxHeavyBlocks, ctx->num_cus * gapped_seg_blocks_per_cu());
+    const int heavy_blocks = std::min(kNaMaxHeavyBlocks, ctx->num_cus * gapped_seg_blocks_per_cu(ctx->tune.gapped_width == 16));
     Layout L;
     if (heavy_blocks < 1 || !carve(base, (size_t)blocks * waves, (size_t)heavy_blocks, L))
         return fail(ctx, BLZ_EINVAL, "launch shape does not fit the no-allocation scratch array");
@@ -303,7 +303,7 @@ extern "C" blz_status blz_prelim_run_noalloc(blz_ctx *ctx, const blz_query *q, c
     if (e == hipSuccess && any) e = launch_seed(sa, blocks, waves, st);
     if (e == hipSuccess) e = hipEventRecord(ctx->ev[1], st);
     if (e == hipSuccess && any) e = launch_order(L.work, L.ctr, L.order, (uint32_t)kNaMaxSubjects, 128, st);
-    if (e == hipSuccess && any) e = launch_gapped_seg(g, heavy_blocks, st);
+    if (e == hipSuccess && any) e = launch_gapped_seg(g, heavy_blocks, ctx->tune.gapped_width == 16, st);
     if (e == hipSuccess && any) e = launch_gapped_wide(w, 1, kNaWideLanes, st);
     if (e == hipSuccess) e = hipEventRecord(ctx->ev[2], st);
     if (e == hipSuccess && any) {

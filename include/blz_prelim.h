@@ -162,7 +162,10 @@ typedef struct {
     int32_t tb_lanes;          /* traceback main pass: lanes per pair, 64 (one pair per wave) or 16
                                   (four pairs per wave, slower: 25.3 against 20.4 ms on the
                                   traceback workload); 0: 64 */
-    int32_t reserved_tb;
+    int32_t gapped_width;      /* bits per DP score in the 16-lane gapped pass: 32 (one column per
+                                  lane) or 16 (two columns per lane, packed int16 arithmetic; DPs
+                                  past its bounds go to the wide pass; slower: 3.12 against 2.77 ms
+                                  on the scan workload); 0: 32 */
 } blz_tuning;
 
 blz_status blz_ctx_set_tuning(blz_ctx *ctx, const blz_tuning *t);

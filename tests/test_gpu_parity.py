@@ -16,13 +16,13 @@ NTHREADS = min(16, os.cpu_count() or 1)
 # seed kernels: 1 = global tables with in-place extension, 2 = the batched kernel (short
 # subjects of a single-query scan share a wave's scan), 3 = the batched kernel scanning one
 # subject at a time; the first also runs the gapped DP with whole waves per subject, the others
-# with four subjects per wave
+# with four subjects per wave, the third with the packed 16-bit DP (two columns per lane)
 @pytest.fixture(scope="module", params=[1, 2, 3], ids=["seed_v1", "seed_v2", "seed_v2_single"])
 def engine(request):
     print(f"\n[engine] seed variant {request.param}", flush=True)
     with blz.Engine(0) as eng:
         eng.set_tuning(seed_variant=min(request.param, 2), gapped_lanes=64 if request.param == 1 else 16,
-                       subject_groups=1 if request.param == 3 else 0)
+                       subject_groups=1 if request.param == 3 else 0, gapped_width=16 if request.param == 3 else 0)
         yield eng
 
 

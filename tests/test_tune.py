@@ -42,7 +42,7 @@ def test_keeps_only_clear_wins_and_leaves_engine_set():
 
 def test_candidates_cover_launch_shape_bins_tiles_and_routing():
     for k in ("seed_waves", "bin_ring", "pack_len", "seed_table", "subject_groups", "gapped_wave_classes",
-              "gapped_lanes"):
+              "gapped_lanes", "gapped_width"):
         assert k in blz_tune.CANDIDATES and blz_tune.CANDIDATES[k][0] == 0  # built-in first
 
 
