@@ -315,12 +315,22 @@ __device__ __forceinline__ void apply_hits(const Ctx &c, WaveState<R> &w, bool w
 }
 
 // One claim pass: of the pending lanes whose diagonals hash to the same claim entry, the
-// earliest rank wins and applies its hit; returns true for the lanes that lost (their hit is
-// still to be applied).  The claim and the diagonal's state are read in the same pass: a
-// winner's state is current, since no lane writes a ring slot before the claims are settled.
+// earliest rank wins and applies its hit.  The claim and the diagonal's state are read in the
+// same pass: a winner's state is current, since no lane writes a ring slot before the claims
+// are settled.
+//
+// The lanes that lost then read their diagonal's state as the winners left it, and settle their
+// hit at once where it is skipped or deferred.  This is exact for every losing hit, whether an
+// earlier hit of its diagonal lost too or not: against one state, the skipped hits of a
+// diagonal are its earliest (a plain state skips diff < min(word, window), and the distance
+// grows with the position; a flag skips positions before the barrier), a skipped hit leaves the state as it
+// was, and on a pending diagonal every hit is deferred, in rank order.  The first hit of a
+// diagonal that would change its state is left, with every later one, for the next pass.
+// Returns true for the lanes whose hit is still to be applied.
 template <class R>
 __device__ __forceinline__ bool claim_pass(const Ctx &c, WaveState<R> &w, bool pend, int q, int s, uint32_t slot,
                                            uint32_t rank, Lists &l, uint32_t w0) {
+    constexpr uint32_t kFlag = RingFmt<R>::kFlag, kPending = RingFmt<R>::kPending, kPosMask = RingFmt<R>::kPosMask;
     const uint32_t h = slot & c.claim_mask;
     uint32_t st = 0;
     if (pend) {
@@ -333,7 +343,19 @@ __device__ __forceinline__ bool claim_pass(const Ctx &c, WaveState<R> &w, bool p
     if (win) w.claim[h] = 0xFFFFFFFFu;
     apply_hits(c, w, win, q, s, slot, st, l, w0);
     ring_sync<R>();
-    return pend && !win;
+    // (every lane's slot lies in the ring, so all of them read it; bitwise tests, no branches)
+    const bool lost = pend & !win;
+    const uint32_t st2 = w.ring[slot];
+    const uint32_t sabs = (uint32_t)s + w0;
+    const uint32_t kind = st2 & (kPending | kFlag);
+    const bool pnd = lost & (kind == kPending);
+    const bool skip = lost & (st2 != 0u) &
+                      (((kind == 0u) & (sabs - (st2 - 1u) < (uint32_t)min(kWord, c.window))) |
+                       ((kind == kFlag) & (sabs < (st2 & kPosMask) - 1u)));
+    const uint64_t dm = __ballot(pnd);
+    if (pnd) w.defer[l.nd + lanes_below(dm)] = (uint32_t)q | ((uint32_t)s << 16);
+    l.nd += (uint32_t)__popcll(dm);
+    return lost && !pnd && !skip;
 }
 
 // Applies up to one hit per lane; same-diagonal hits go earliest-rank first.
