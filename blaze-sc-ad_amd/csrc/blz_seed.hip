@@ -331,16 +331,16 @@ template <class R>
 __device__ __forceinline__ bool claim_pass(const Ctx &c, WaveState<R> &w, bool pend, int q, int s, uint32_t slot,
                                            uint32_t rank, Lists &l, uint32_t w0) {
     constexpr uint32_t kFlag = RingFmt<R>::kFlag, kPending = RingFmt<R>::kPending, kPosMask = RingFmt<R>::kPosMask;
-    const uint32_t h = slot & c.claim_mask;
-    uint32_t st = 0;
-    if (pend) {
-        atomicMin(&w.claim[h], rank);
-        st = w.ring[slot];
-    }
+    // every lane takes part, without branches: an idle lane claims an entry of its own with the
+    // largest rank (no effect on the minimum), reads a ring slot that lies in the ring, and all
+    // lanes free their entries once the claims are read, so the table is free again after the pass
+    const uint32_t h = (pend ? slot : (uint32_t)lane_id()) & c.claim_mask;
+    atomicMin(&w.claim[h], pend ? rank : 0xFFFFFFFFu);
+    const uint32_t st = w.ring[slot];
     ring_sync<R>();
-    const bool win = pend && w.claim[h] == rank;
+    const bool win = pend & (w.claim[h] == rank);
     ring_sync<R>();
-    if (win) w.claim[h] = 0xFFFFFFFFu;
+    w.claim[h] = 0xFFFFFFFFu;
     apply_hits(c, w, win, q, s, slot, st, l, w0);
     ring_sync<R>();
     // (every lane's slot lies in the ring, so all of them read it; bitwise tests, no branches)
@@ -496,8 +496,8 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
     int gi = 0;                           // subject of the group
     int p_begin = 0, p_end = slen, voff = 0;  // its letters [p_begin, p_end), virtual shift
     if constexpr (kGrouped) {
-        p_begin = (int)w.grp->start[0];
-        p_end = p_begin + (int)w.grp->len[0];
+        p_begin = __builtin_amdgcn_readfirstlane((int)w.grp->start[0]);
+        p_end = p_begin + __builtin_amdgcn_readfirstlane((int)w.grp->len[0]);
     }
     // only the launches with 32-bit rings in global memory (long subjects, the retry scan) take
     // subjects past kWindowFrom; the others compile without windows
@@ -573,8 +573,8 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
             if constexpr (kGrouped) {
                 if (gi + 1 < ng) {
                     gi++;
-                    p_begin = (int)w.grp->start[gi];
-                    p_end = p_begin + (int)w.grp->len[gi];
+                    p_begin = __builtin_amdgcn_readfirstlane((int)w.grp->start[gi]);
+                    p_end = p_begin + __builtin_amdgcn_readfirstlane((int)w.grp->len[gi]);
                     voff = gi * c.gap;
                     c0 = p_begin;
                     continue;
@@ -639,9 +639,12 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
                 uint32_t e[4];
 #pragma unroll
                 for (int j = 0; j < 4; j++) e[j] = entry<kTable>(t, pa + min(k + j, nh - 1));
+                // stores past the lane's entries go to its own idle word of w.excl (no branches,
+                // so the four loads stay ahead of the stores)
+                typedef volatile __attribute__((address_space(3))) uint32_t lds_u32x;
 #pragma unroll
                 for (uint32_t j = 0; j < 4; j++)
-                    if (k + j < nh) dst[k + j] = (e[j] << 16) | (uint32_t)s;
+                    *(lds_u32x *)(k + j < nh ? dst + k + j : w.excl + lane) = (e[j] << 16) | (uint32_t)s;
             }
             wave_sync();
             tail += H;
