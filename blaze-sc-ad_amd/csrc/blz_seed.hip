@@ -520,12 +520,16 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
             if (l.nt > (uint32_t)(kTaskCap - 64) || l.nd > 64u) flush<kPad, kGrouped>(c, w, S, slen, ng, n_ext, l, w0);
             const uint32_t n = tail - head < 64u ? tail - head : 64u;
             const bool pend = (uint32_t)lane < n;
-            const uint32_t e = pend ? w.hits[head + lane] : 0u;
+            const uint32_t e0 = w.hits[head + lane];  // (past the queue: the task list, in the wave's LDS)
+            const uint32_t e = pend ? e0 : 0u;
             const int q = (int)(e >> 16), sp = (int)(e & 0xFFFFu);
             const bool lost = claim_pass(c, w, pend, q, sp, ring_slot(c, q, sp), (uint32_t)lane, l, w0);
             const uint64_t lm = __ballot(lost);
             head += n - (uint32_t)__popcll(lm);
-            if (lost) w.hits[head + lanes_below(lm)] = e;
+            // (the others store to the deferred list's last 64 words, unused here: at most 128
+            // deferred hits after a round)
+            *(volatile __attribute__((address_space(3))) uint32_t *)(lost ? w.hits + head + lanes_below(lm)
+                                                                           : w.defer + kDeferCap - 64 + lane) = e;
             wave_sync();
             continue;
         }
@@ -587,7 +591,22 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
         const int s = p + voff - (int)w0;  // position seen by the two-hit state, in the window
         uint32_t pa = 0, nh = 0;
         u32x4 r = {0u, 0u, 0u, 0u};  // kTable 0: the word's record
-        if (p + kWord <= p_end && p < w_end) {
+        if constexpr (kTable == 0) {
+            // every lane loads a record, lanes past the last word that of the last word (clamped
+            // position); their count is then taken as 0
+            const int p_last = min(p_end - kWord, w_end - 1);
+            const int pc = min(p, p_last);
+            const uint32_t w2 = S[pc] * 784u + S[pc + 1] * 28u + S[pc + 2];
+            const uint32_t wd = w2 >> (kPad ? 1 : 0);
+            r = *reinterpret_cast<const __attribute__((address_space(1))) u32x4 *>(
+                gmem(reinterpret_cast<const uint8_t *>(t.rec)) + (w2 << (kPad ? 3 : 4)));
+            nh = p <= p_last ? r.x & 0xFFu : 0u;
+            pa = r.x >> 8;
+            if (nh == 0xFFu) {  // 255 entries or more, or a first entry past 2^24
+                pa = gmem(t.gprefix)[wd];
+                nh = gmem(t.gprefix)[wd + 1] - pa;
+            }
+        } else if (p + kWord <= p_end && p < w_end) {
             // (letters in the LDS buffer are doubled, and so is their word index: a 16-byte
             // record's byte offset is then the index times 8)
             const uint32_t w2 = S[p] * 784u + S[p + 1] * 28u + S[p + 2];
