@@ -117,6 +117,18 @@ def test_random_params(engine, case):
     compare(engine, q, res, offs, blz.default_params(**kw), oracle.params(**kw))
 
 
+@pytest.mark.parametrize("window", [1, 2, 3, 4])
+def test_small_windows(engine, window):
+    """Two-hit windows at and below the word size.  With a window of 3 or less no pair is ever
+    extended (a second hit is either an overlap or its diagonal's new last hit), and hits that lose
+    a claim and are settled against their diagonal's new state follow the same rule; every list
+    and counter matches the oracle."""
+    kw = dict(window=window, ungapped_cutoff=20)
+    print(f"[case] small window {window}", flush=True)
+    q, res, offs, _ = make(4500 + window, 400, 20000, homolog_frac=0.05, ident_min=0.4, ident_max=0.9)
+    compare(engine, q, res, offs, blz.default_params(**kw), oracle.params(**kw))
+
+
 def test_edge_subjects(engine):
     """Subjects of length 1, 2, 3, the query itself, a reversed query and empty-range neighbours."""
     q = blz.synth_query(3, 200)
