@@ -106,7 +106,7 @@ struct WaveState {
     uint32_t *claim;
     uint32_t *excl;
     uint32_t *p0;
-    uint32_t *cnt;  // [0] tasks, [1] deferred, [2] init hits
+    uint32_t *cnt;  // [2] init hits staged ([0], [1], [3] unused)
     uint32_t *hits;
     Task *task;
     uint32_t *defer;  // query offset | subject offset << 16
@@ -323,9 +323,10 @@ __device__ __forceinline__ void apply_hits(const Ctx &c, WaveState<R> &w, bool w
 // hit at once where it is skipped or deferred.  This is exact for every losing hit, whether an
 // earlier hit of its diagonal lost too or not: against one state, the skipped hits of a
 // diagonal are its earliest (a plain state skips diff < min(word, window), and the distance
-// grows with the position; a flag skips positions before the barrier), a skipped hit leaves the state as it
-// was, and on a pending diagonal every hit is deferred, in rank order.  The first hit of a
-// diagonal that would change its state is left, with every later one, for the next pass.
+// grows with the position; a flag skips positions before the barrier), a skipped hit leaves
+// the state as it was, and on a pending diagonal every hit is deferred, in rank order.  The
+// first hit of a diagonal that would change its state is left, with every later one, for the
+// next pass.
 // Returns true for the lanes whose hit is still to be applied.
 template <class R>
 __device__ __forceinline__ bool claim_pass(const Ctx &c, WaveState<R> &w, bool pend, int q, int s, uint32_t slot,
