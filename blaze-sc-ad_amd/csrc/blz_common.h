@@ -41,14 +41,18 @@ struct WorkItem {
 };
 
 constexpr int kBuckets = 8;
-// Expected gapped-stage cost class of a subject, 0 = heaviest, from its init hit count and best
-// ungapped score.  Each init hit outside the HSPs found before it is a DP of its own, and a
-// chance hit's DP runs about as long as a homolog's (X-drop 38 in a random walk), so the count
-// comes first: 5 or more, 4, 3, 2, then single hits by score (150, 80, 55).
-__host__ __device__ inline int cost_bucket(int best_ungapped, int n_init) {
+// Expected gapped-stage cost class of a subject, 0 = heaviest, from its init hit count, best
+// ungapped score and length.  Each init hit outside the HSPs found before it is a DP of its own,
+// and a chance hit's DP runs about as long as a homolog's (X-drop 38 in a random walk), so the
+// count comes first: 5 or more, 4, 3, 2.  Single hits: a score of 150 or more first (a long
+// homologous alignment), then by subject length (400, 200 letters), which bounds the DP's
+// length: a low-scoring hit on a long homolog runs as long as the alignment, and ordered by
+// score it was taken among the last.
+__host__ __device__ inline int cost_bucket(int best_ungapped, int n_init, int slen) {
     if (n_init >= 5) return 0;
     if (n_init >= 2) return 5 - n_init;
-    return best_ungapped >= 150 ? 4 : best_ungapped >= 80 ? 5 : best_ungapped >= 55 ? 6 : 7;
+    if (best_ungapped >= 150) return 4;
+    return slen >= 400 ? 5 : slen >= 200 ? 6 : 7;
 }
 
 // HSP record, same layout as blz_hsp in blz_prelim.h.

@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(256) k_seed(SeedArgs a) {
                     if (lane == 0) {
                         const uint32_t wi = atomicAdd(&a.ctr->work_count, 1u);
                         if (wi < a.work_cap) {
-                            const int bucket = cost_bucket(best, (int)nih);
+                            const int bucket = cost_bucket(best, (int)nih, slen);
                             a.work[wi] = WorkItem{sid, (uint64_t)base, (int32_t)nih, 0, bucket, 0};
                             atomicAdd(&a.ctr->bucket_count[bucket], 1u);
                             atomicAdd(&a.ctr->init_hits, (unsigned long long)nih);

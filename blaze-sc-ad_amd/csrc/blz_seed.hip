@@ -693,7 +693,7 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
 // the item's pack that has any.  Query offsets become offsets in that query.
 template <class R>
 __device__ void emit_init_hits(const SeedArgs &a, const QueryDev &qd, WaveState<R> &w, const Ctx &c, int64_t sid,
-                               uint32_t nih, bool count_init = true) {
+                               int slen, uint32_t nih, bool count_init = true) {
     const int lane = lane_id();
     const int nm = qd.n_members;
     auto overflow = [&](int qi) {
@@ -733,7 +733,7 @@ __device__ void emit_init_hits(const SeedArgs &a, const QueryDev &qd, WaveState<
             const int qi = (int)qd.member_idx[0];
             const uint32_t wi = atomicAdd(&a.ctr->work_count, 1u);
             if (wi < a.work_cap) {
-                const int bucket = cost_bucket(best, (int)nih);
+                const int bucket = cost_bucket(best, (int)nih, slen);
                 a.work[wi] = WorkItem{sid, (uint64_t)base, (int32_t)nih, qi, bucket, 0};
                 atomicAdd(&a.ctr->bucket_count[bucket], 1u);
                 if (count_init) atomicAdd(&a.ctr->init_hits, (unsigned long long)nih);
@@ -785,7 +785,7 @@ __device__ void emit_init_hits(const SeedArgs &a, const QueryDev &qd, WaveState<
         const int qi = (int)qd.member_idx[lane];
         const uint32_t wi = atomicAdd(&a.ctr->work_count, 1u);
         if (wi < a.work_cap) {
-            const int bucket = cost_bucket((int)bestm[lane], (int)mine);
+            const int bucket = cost_bucket((int)bestm[lane], (int)mine, slen);
             a.work[wi] = WorkItem{sid, (uint64_t)(base + off), (int32_t)mine, qi, bucket, 0};
             atomicAdd(&a.ctr->bucket_count[bucket], 1u);
             atomicAdd(&a.ctr->init_hits, (unsigned long long)mine);
@@ -868,7 +868,7 @@ __device__ void emit_group(const SeedArgs &a, const QueryDev &qd, WaveState<R> &
         const int64_t sid = w.grp->sid[bin / nm];
         const uint32_t wi = atomicAdd(&a.ctr->work_count, 1u);
         if (wi < a.work_cap) {
-            const int bucket = cost_bucket((int)bestm[bin], (int)n);
+            const int bucket = cost_bucket((int)bestm[bin], (int)n, (int)w.grp->len[bin / nm]);
             a.work[wi] = WorkItem{sid, (uint64_t)(base + offm[bin]), (int32_t)n, qi, bucket, 0};
             atomicAdd(&a.ctr->bucket_count[bucket], 1u);
             atomicAdd(&a.ctr->init_hits, (unsigned long long)n);
@@ -1024,7 +1024,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
             wave_sync();
             if (nih > 0) {
                 if (ng == 1 && qd.n_members == 1)
-                    emit_init_hits(a, qd, w, c, w.grp->sid[0], nih);  // long lists leave sorted
+                    emit_init_hits(a, qd, w, c, w.grp->sid[0], (int)w.grp->len[0], nih);  // long lists leave sorted
                 else
                     emit_group(a, qd, w, c, ng, nih);
                 wave_sync();
@@ -1094,7 +1094,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
             const uint32_t nih = w.cnt[2];
             wave_sync();
             if (nih > 0) {
-                emit_init_hits(a, qd, w, c, sid, nih);
+                emit_init_hits(a, qd, w, c, sid, slen, nih);
                 wave_sync();
                 if (lane == 0) w.cnt[2] = 0;
                 wave_sync();
@@ -1205,7 +1205,7 @@ __global__ void __launch_bounds__(256) k_seed_retry(SeedArgs a) {
         const uint32_t nih = w.cnt[2];
         wave_sync();
         if (nih > 0) {
-            emit_init_hits(a, qd, w, c, sid, nih, count_init);
+            emit_init_hits(a, qd, w, c, sid, slen, nih, count_init);
             wave_sync();
             if (lane == 0) w.cnt[2] = 0;
             wave_sync();
