@@ -368,19 +368,23 @@ __device__ __forceinline__ void resolve_round(const Ctx &c, WaveState<R> &w, boo
 }
 
 // Extends every queued pair (one lane each), writes the outcome to its diagonal, then replays
-// the hits queued behind pending diagonals; repeats until nothing is pending.
+// the hits queued behind pending diagonals.  With drain (the end of a scan or window) it repeats
+// until nothing is pending; otherwise only until the lists are back to 64 entries or fewer: the
+// tasks a replay creates then wait for the next flush and fill its walk passes (a pending task
+// with hits deferred behind it is the scan's normal state).
 // kGrouped: the tasks belong to the subjects of w.grp (ng of them); a task's subject follows from
 // its virtual position, its letters from the subject's place in the buffer.  Staged init hits
 // carry their subject's index in bits 24.. of len and subject-relative offsets.
 template <bool kPad, bool kGrouped, class R, class SP>
-__device__ void flush(const Ctx &c, WaveState<R> &w, SP S, int slen, int ng, uint32_t &n_ext, Lists &l, uint32_t w0) {
+__device__ void flush(const Ctx &c, WaveState<R> &w, SP S, int slen, int ng, uint32_t &n_ext, Lists &l, uint32_t w0,
+                      bool drain) {
     constexpr uint32_t kFlag = RingFmt<R>::kFlag;
     const int lane = lane_id();
     for (;;) {
         ring_sync<R>();
         const uint32_t nt = l.nt;
         const uint32_t nd = l.nd;
-        if (nt == 0 && nd == 0) break;
+        if (drain ? nt == 0 && nd == 0 : nt <= 64u && nd <= 64u) break;
         for (uint32_t t0 = 0; t0 < nt; t0 += 64) {
             const uint32_t t = t0 + lane;
             if (t < nt) {
@@ -518,7 +522,7 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
         // a claim round over the queue's first 64 hits, after the extensions when the task or
         // deferred lists fill
         if (tail - head >= 64u || (!input && tail > head)) {
-            if (l.nt > (uint32_t)(kTaskCap - 64) || l.nd > 64u) flush<kPad, kGrouped>(c, w, S, slen, ng, n_ext, l, w0);
+            if (l.nt > (uint32_t)(kTaskCap - 64) || l.nd > 64u) flush<kPad, kGrouped>(c, w, S, slen, ng, n_ext, l, w0, false);
             const uint32_t n = tail - head < 64u ? tail - head : 64u;
             const bool pend = (uint32_t)lane < n;
             const uint32_t e0 = w.hits[head + lane];  // (past the queue: the task list, in the wave's LDS)
@@ -681,7 +685,7 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
         }
         c0 += 64;
     }
-    flush<kPad, kGrouped>(c, w, S, slen, ng, n_ext, l, w0);
+    flush<kPad, kGrouped>(c, w, S, slen, ng, n_ext, l, w0, true);
     if (!kWindows || win == 0 || w_end + kWord > p_end) break;
     w0 = (uint32_t)w_end;  // the next window; the ring carries over
     w_end = w_end + win < p_end ? w_end + win : p_end;
