@@ -369,9 +369,10 @@ __device__ __forceinline__ void resolve_round(const Ctx &c, WaveState<R> &w, boo
 
 // Extends every queued pair (one lane each), writes the outcome to its diagonal, then replays
 // the hits queued behind pending diagonals.  With drain (the end of a scan or window) it repeats
-// until nothing is pending; otherwise only until the lists are back to 64 entries or fewer: the
-// tasks a replay creates then wait for the next flush and fill its walk passes (a pending task
-// with hits deferred behind it is the scan's normal state).
+// until nothing is pending.  Otherwise it runs full walk passes only (the oldest 64 tasks) and
+// stops once both lists are back to 64 entries or fewer: the other tasks, and those a replay
+// creates, wait for the next flush and fill its walk pass (a pending task with hits deferred
+// behind it is the scan's normal state).
 // kGrouped: the tasks belong to the subjects of w.grp (ng of them); a task's subject follows from
 // its virtual position, its letters from the subject's place in the buffer.  Staged init hits
 // carry their subject's index in bits 24.. of len and subject-relative offsets.
@@ -385,9 +386,12 @@ __device__ void flush(const Ctx &c, WaveState<R> &w, SP S, int slen, int ng, uin
         const uint32_t nt = l.nt;
         const uint32_t nd = l.nd;
         if (drain ? nt == 0 && nd == 0 : nt <= 64u && nd <= 64u) break;
-        for (uint32_t t0 = 0; t0 < nt; t0 += 64) {
+        // in the middle of a scan one full walk pass of the oldest 64 tasks, while the replay
+        // below cannot overfill the task list (it adds at most one task per replayed hit)
+        const uint32_t ne = !drain && nt > 64u && nd <= 64u ? 64u : nt;
+        for (uint32_t t0 = 0; t0 < ne; t0 += 64) {
             const uint32_t t = t0 + lane;
-            if (t < nt) {
+            if (t < ne) {
                 const Task tk = w.task[t];
                 const int q = (int)(tk.q_slot & 0xFFFFu);
                 const uint32_t slot = tk.q_slot >> 16;
@@ -411,7 +415,14 @@ __device__ void flush(const Ctx &c, WaveState<R> &w, SP S, int slen, int ng, uin
             }
         }
         ring_sync<R>();
-        l.nt = 0;
+        if (ne < nt) {  // the other tasks stay queued, moved to the front (a ring entry only says "pending")
+            const bool mv = (uint32_t)lane < nt - ne;
+            const Task tk = mv ? w.task[ne + lane] : Task{0u, 0u};
+            wave_sync();
+            if (mv) w.task[lane] = tk;
+            wave_sync();
+        }
+        l.nt = nt - ne;
         // replay the queued hits [0, nd) in order; new deferrals land at nd and beyond
         for (uint32_t r = 0; r < nd; r += 64) {
             const uint32_t i = r + lane;
