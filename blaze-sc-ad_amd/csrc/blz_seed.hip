@@ -527,6 +527,8 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
     int c0 = p_begin;                     // next step
     uint32_t dense_h = 0, dense_r = 0;    // dense step in progress: its hits, those expanded
     int dense_c0 = 0, dense_voff = 0;
+    bool dense_pf = false;                // the next pass's owner lanes and entries are loaded
+    uint32_t pf_q = 0, pf_lo = 0;         // (this lane's hit of that pass: entry, owner lane)
   for (;;) {                              // windows
     bool input = true;                    // steps (or dense hits) remain
     for (;;) {
@@ -569,20 +571,34 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
         // max over the slots gives every hit its lane.
         if (dense_r < dense_h) {
             make_room(64u);
-            const uint32_t g0 = dense_r, g = g0 + lane;
-            uint32_t *const mark = w.hits + tail;
-            mark[lane] = 0u;
-            wave_sync();
-            const uint32_t ex_i = w.excl[lane];
-            if (w.excl[lane + 1] > ex_i && ex_i < g0 + 64u) atomicMax(&mark[ex_i > g0 ? ex_i - g0 : 0u], (uint32_t)lane + 1u);
-            wave_sync();
-            const int lo = wave_inclusive_max((int)mark[lane] - 1);  // slot 0 always holds a mark
-            wave_sync();
-            if (g < dense_h) {
-                const uint32_t qo = entry<kTable>(t, w.p0[lo] + (g - w.excl[lo]));
-                w.hits[tail + lane] = (qo << 16) | (uint32_t)(dense_c0 + lo + dense_voff);
-            }
+            // owner lanes of the pass starting at hit g0, marked in mark[0, 64)
+            auto owners = [&](uint32_t g0, uint32_t *mark) {
+                mark[lane] = 0u;
+                wave_sync();
+                const uint32_t ex_i = w.excl[lane];
+                if (w.excl[lane + 1] > ex_i && ex_i < g0 + 64u) atomicMax(&mark[ex_i > g0 ? ex_i - g0 : 0u], (uint32_t)lane + 1u);
+                wave_sync();
+                const int lo = wave_inclusive_max((int)mark[lane] - 1);  // slot 0 always holds a mark
+                wave_sync();
+                return lo;
+            };
+            const uint32_t g = dense_r + lane;
             const uint32_t n = dense_h - dense_r < 64u ? dense_h - dense_r : 64u;
+            uint32_t qo = pf_q, lo = pf_lo;
+            if (!dense_pf) {
+                lo = (uint32_t)owners(dense_r, w.hits + tail);
+                qo = g < dense_h ? entry<kTable>(t, w.p0[lo] + (g - w.excl[lo])) : 0u;
+            }
+            if (g < dense_h) w.hits[tail + lane] = (qo << 16) | (uint32_t)(dense_c0 + (int)lo + dense_voff);
+            // the next pass's owners (marked in the claim table's first 64 entries, free between
+            // claim passes, then freed again) and its entry loads, in flight while rounds run
+            const uint32_t g1 = dense_r + n;
+            dense_pf = g1 < dense_h;
+            if (dense_pf) {
+                pf_lo = (uint32_t)owners(g1, w.claim);
+                w.claim[lane] = 0xFFFFFFFFu;
+                pf_q = g1 + lane < dense_h ? entry<kTable>(t, w.p0[pf_lo] + (g1 + lane - w.excl[pf_lo])) : 0u;
+            }
             wave_sync();
             tail += n;
             dense_r += n;
@@ -691,6 +707,7 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
             wave_sync();
             dense_h = H;
             dense_r = 0;
+            dense_pf = false;
             dense_c0 = c0;
             dense_voff = voff - (int)w0;
         }
