@@ -99,6 +99,15 @@ struct Group {
     uint32_t len[kGroupMax];
     uint32_t vstart[kGroupMax];  // start + i * gap
 };
+// Subjects too long for the buffer (up to kNarrowMax letters) are grouped the same way while
+// their letters stay in global memory: subject i's positions are shifted by vstart[i] (the
+// previous subjects' lengths plus gap each), up to kNarrowMax virtual positions in all (the
+// 16-bit ring's position field).  The table lies in the (then unused) letter buffer, laid out as
+// a Group (start unused) followed by the subjects' DB offsets.
+struct GlobalGroup {
+    Group g;
+    uint64_t o0[kGroupMax];
+};
 
 template <class R>
 struct WaveState {
@@ -376,7 +385,7 @@ __device__ __forceinline__ void resolve_round(const Ctx &c, WaveState<R> &w, boo
 // kGrouped: the tasks belong to the subjects of w.grp (ng of them); a task's subject follows from
 // its virtual position, its letters from the subject's place in the buffer.  Staged init hits
 // carry their subject's index in bits 24.. of len and subject-relative offsets.
-template <bool kPad, bool kGrouped, class R, class SP>
+template <bool kPad, bool kGrouped, bool kGlobal, class R, class SP>
 __device__ void flush(const Ctx &c, WaveState<R> &w, SP S, int slen, int ng, uint32_t &n_ext, Lists &l, uint32_t w0,
                       bool drain) {
     constexpr uint32_t kFlag = RingFmt<R>::kFlag;
@@ -397,14 +406,22 @@ __device__ void flush(const Ctx &c, WaveState<R> &w, SP S, int slen, int ng, uin
                 const uint32_t slot = tk.q_slot >> 16;
                 const int ts = (int)(tk.s_pack & 0xFFFFu) + (int)w0;  // absolute
                 int sub = 0, shift = 0, rel = 0;  // subject in the group, virtual shift, first letter
+                SP St = S;                        // its letters
+                int slen_t = slen;
                 if constexpr (kGrouped) {
                     for (int j = 1; j < ng; j++) sub += ts >= (int)w.grp->vstart[j];
-                    shift = sub * c.gap;
-                    rel = (int)w.grp->start[sub];
+                    if constexpr (kGlobal) {  // letters in global memory, positions from the subject's start
+                        shift = (int)w.grp->vstart[sub];
+                        St = S + reinterpret_cast<const GlobalGroup *>(w.grp)->o0[sub];
+                        slen_t = (int)w.grp->len[sub];
+                    } else {
+                        shift = sub * c.gap;
+                        rel = (int)w.grp->start[sub];
+                    }
                 }
                 int hq, hs, hl, rext, slast;
                 const int s_left = (int)(tk.s_pack >> 16) - (int)kEndBias + (int)w0;
-                const int score = two_hit_walk<kPad>(c, S, slen, s_left - shift, ts - shift, q, &hq, &hs, &hl, &rext,
+                const int score = two_hit_walk<kPad>(c, St, slen_t, s_left - shift, ts - shift, q, &hq, &hs, &hl, &rext,
                                                      &slast);
                 n_ext++;
                 if (score >= c.cutoff) {
@@ -503,7 +520,7 @@ __device__ __forceinline__ void word_range(const Tables &t, uint32_t wd, uint32_
 // flush would: the result does not depend on when flushes happen.
 constexpr int kWindowFrom = 60000;
 constexpr int kWindowMax = 32768;
-template <int kTable, bool kPad, bool kGrouped = false, class R, class SP>
+template <int kTable, bool kPad, bool kGrouped = false, bool kGlobal = false, class R, class SP>
 __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP S, int slen, uint32_t &n_hits,
                              uint32_t &n_ext, int ng = 1) {
     const int lane = lane_id();
@@ -511,10 +528,20 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
     uint32_t head = 0, tail = 0;
     int gi = 0;                           // subject of the group
     int p_begin = 0, p_end = slen, voff = 0;  // its letters [p_begin, p_end), virtual shift
-    if constexpr (kGrouped) {
-        p_begin = __builtin_amdgcn_readfirstlane((int)w.grp->start[0]);
-        p_end = p_begin + __builtin_amdgcn_readfirstlane((int)w.grp->len[0]);
-    }
+    SP Sc = S;                            // kGlobal: the current subject's letters
+    auto subject = [&](int i) {           // subject i of the group
+        if constexpr (kGlobal) {
+            p_begin = 0;
+            p_end = __builtin_amdgcn_readfirstlane((int)w.grp->len[i]);
+            voff = __builtin_amdgcn_readfirstlane((int)w.grp->vstart[i]);
+            Sc = S + first_lane64(reinterpret_cast<const GlobalGroup *>(w.grp)->o0[i]);
+        } else {
+            p_begin = __builtin_amdgcn_readfirstlane((int)w.grp->start[i]);
+            p_end = p_begin + __builtin_amdgcn_readfirstlane((int)w.grp->len[i]);
+            voff = i * c.gap;
+        }
+    };
+    if constexpr (kGrouped) subject(0);
     // only the launches with 32-bit rings in global memory (long subjects, the retry scan) take
     // subjects past kWindowFrom; the others compile without windows
     constexpr bool kWindows = sizeof(R) == 4 && !kGrouped;
@@ -535,7 +562,7 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
         // a claim round over the queue's first 64 hits, after the extensions when the task or
         // deferred lists fill
         if (tail - head >= 64u || (!input && tail > head)) {
-            if (l.nt > (uint32_t)(kTaskCap - 64) || l.nd > 64u) flush<kPad, kGrouped>(c, w, S, slen, ng, n_ext, l, w0, false);
+            if (l.nt > (uint32_t)(kTaskCap - 64) || l.nd > 64u) flush<kPad, kGrouped, kGlobal>(c, w, S, slen, ng, n_ext, l, w0, false);
             const uint32_t n = tail - head < 64u ? tail - head : 64u;
             const bool pend = (uint32_t)lane < n;
             const uint32_t e0 = w.hits[head + lane];  // (past the queue: the task list, in the wave's LDS)
@@ -609,9 +636,7 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
             if constexpr (kGrouped) {
                 if (gi + 1 < ng) {
                     gi++;
-                    p_begin = __builtin_amdgcn_readfirstlane((int)w.grp->start[gi]);
-                    p_end = p_begin + __builtin_amdgcn_readfirstlane((int)w.grp->len[gi]);
-                    voff = gi * c.gap;
+                    subject(gi);
                     c0 = p_begin;
                     continue;
                 }
@@ -628,7 +653,7 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
             // position); their count is then taken as 0
             const int p_last = min(p_end - kWord, w_end - 1);
             const int pc = min(p, p_last);
-            const uint32_t w2 = S[pc] * 784u + S[pc + 1] * 28u + S[pc + 2];
+            const uint32_t w2 = Sc[pc] * 784u + Sc[pc + 1] * 28u + Sc[pc + 2];
             const uint32_t wd = w2 >> (kPad ? 1 : 0);
             r = *reinterpret_cast<const __attribute__((address_space(1))) u32x4 *>(
                 gmem(reinterpret_cast<const uint8_t *>(t.rec)) + (w2 << (kPad ? 3 : 4)));
@@ -641,7 +666,7 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
         } else if (p + kWord <= p_end && p < w_end) {
             // (letters in the LDS buffer are doubled, and so is their word index: a 16-byte
             // record's byte offset is then the index times 8)
-            const uint32_t w2 = S[p] * 784u + S[p + 1] * 28u + S[p + 2];
+            const uint32_t w2 = Sc[p] * 784u + Sc[p + 1] * 28u + Sc[p + 2];
             const uint32_t wd = w2 >> (kPad ? 1 : 0);
             if constexpr (kTable == 0) {
                 r = *reinterpret_cast<const __attribute__((address_space(1))) u32x4 *>(
@@ -713,7 +738,7 @@ __device__ void scan_subject(const Ctx &c, WaveState<R> &w, const Tables &t, SP 
         }
         c0 += 64;
     }
-    flush<kPad, kGrouped>(c, w, S, slen, ng, n_ext, l, w0, true);
+    flush<kPad, kGrouped, kGlobal>(c, w, S, slen, ng, n_ext, l, w0, true);
     if (!kWindows || win == 0 || w_end + kWord > p_end) break;
     w0 = (uint32_t)w_end;  // the next window; the ring carries over
     w_end = w_end + win < p_end ? w_end + win : p_end;
@@ -1066,6 +1091,33 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
             ng = 0;
             used = 0;
         };
+        // subjects longer than the buffer (to kNarrowMax letters) form groups of their own, their
+        // table in the buffer (run before short subjects need it, and the other way round)
+        int ngg = 0;
+        uint32_t gv = 0;  // virtual positions the global group spans
+        GlobalGroup *const gg = reinterpret_cast<GlobalGroup *>(w.subj);
+        auto run_global_group = [&]() {
+            uint4 *r4 = reinterpret_cast<uint4 *>(w.ring);
+            for (uint32_t i = lane; i < c.ring_len / 8; i += 64) r4[i] = uint4{0, 0, 0, 0};
+            wave_sync();
+            WaveState<uint16_t> wg = w;
+            wg.grp = &gg->g;
+            scan_subject<kTable, false, true, true>(c, wg, t, a.res, 0, n_hits, n_ext, ngg);
+            wave_sync();
+            const uint32_t nih = w.cnt[2];
+            wave_sync();
+            if (nih > 0) {
+                if (ngg == 1 && qd.n_members == 1)
+                    emit_init_hits(a, qd, wg, c, gg->g.sid[0], (int)gg->g.len[0], nih);  // long lists leave sorted
+                else
+                    emit_group(a, qd, wg, c, ngg, nih);
+                wave_sync();
+                if (lane == 0) w.cnt[2] = 0;
+                wave_sync();
+            }
+            ngg = 0;
+            gv = 0;
+        };
         for (;;) {
             int64_t k = 0;
             if (lane == 0) k = sub0 + (int64_t)atomicAdd(reinterpret_cast<uint32_t *>(&blk[1]), 1u);
@@ -1077,6 +1129,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
             const uint8_t *Sg = a.res + o0;
             const uint32_t Mq = c.ring_len;
             if (!kLong && slen <= kSubjBuf) {  // staged in LDS, alone when groups are off (gap 0)
+                if (ngg > 0) run_global_group();  // the buffer holds its table
                 if (ng == group_cap || used + (uint32_t)slen > (uint32_t)kSubjBuf) run_group();
                 // letters at [used, used + slen) by 16-byte loads, then zeros up to the next
                 // 64-aligned start (at least 4) or the buffer's end pad
@@ -1108,17 +1161,28 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
                 used = next;
                 continue;
             }
-            if (ng > 0) run_group();  // the single-subject paths below reuse the buffer
+            if (ng > 0) run_group();  // the paths below reuse the buffer
+            if constexpr (!kLong) {
+                if (slen > kNarrowMax) continue;  // the long-subject launch takes it
+                const uint32_t v0 = ngg > 0 ? gv + (uint32_t)c.gap : 0u;
+                if (ngg > 0 && (ngg == group_cap || v0 + (uint32_t)slen > (uint32_t)kNarrowMax)) run_global_group();
+                const uint32_t vs = ngg > 0 ? gv + (uint32_t)c.gap : 0u;
+                if (lane == 0) {
+                    gg->g.sid[ngg] = sid;
+                    gg->g.start[ngg] = 0;
+                    gg->g.len[ngg] = (uint32_t)slen;
+                    gg->g.vstart[ngg] = vs;
+                    gg->o0[ngg] = o0;
+                }
+                wave_sync();
+                ngg++;
+                gv = vs + (uint32_t)slen;
+                continue;
+            }
             if constexpr (kLong) {
                 for (uint32_t i = lane; i < Mq; i += 64) wl.ring[i] = 0;
                 ring_sync<uint32_t>();
                 scan_subject<kTable, false>(c, wl, t, Sg, slen, n_hits, n_ext);
-            } else {
-                if (slen > kNarrowMax) continue;  // the long-subject launch takes it
-                uint4 *r4 = reinterpret_cast<uint4 *>(w.ring);
-                for (uint32_t i = lane; i < Mq / 8; i += 64) r4[i] = uint4{0, 0, 0, 0};
-                wave_sync();
-                scan_subject<kTable, false>(c, w, t, Sg, slen, n_hits, n_ext);  // read from global memory
             }
 
             // hand the subject's init hits to the gapped stage
@@ -1133,6 +1197,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
             }
         }
         if (ng > 0) run_group();
+        if (ngg > 0) run_global_group();
     }
     unsigned long long ext_tot = n_ext, hit_tot = n_hits;
 #pragma unroll
