@@ -169,8 +169,8 @@ def main() -> int:
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: every GPU box here has one, and the path shards by database "
-              "subsets with no collective in the hot loop; run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU (the path shards by database subsets, one per GPU, with no "
+              "collective in the hot loop, so one GPU's share is the figure); run with --gpus 1", file=sys.stderr)
         return 2
     if args.workload == "batch":
         return run_batch(args)
@@ -272,7 +272,7 @@ def run_scan(args) -> int:
         # the same search as a caller with a new query each time sees it: table build, upload,
         # planning, capture of a new launch sequence (no replay of an earlier one), lists back on
         # the host; three untimed new queries first (the host side of this path settles over the
-        # first few: 14.2, 13.0, 12.6, 12.0, then 11.7 ms on one box)
+        # first few: 14.2, 13.0, 12.6, 12.0, then 11.7 ms on one MI355X)
         fresh_lat = []
         for i in range(3 + 5):
             t = time.perf_counter()

@@ -10,4 +10,4 @@ for p in (ROOT / "blaze-sc-ad_amd", ROOT / "oracle", ROOT):
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs a gfx950 GPU (runs on the MI355X box)")
+    config.addinivalue_line("markers", "gpu: needs a gfx950 GPU (MI355X)")
