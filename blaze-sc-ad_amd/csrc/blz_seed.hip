@@ -1131,25 +1131,23 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)
             if (!kLong && slen <= kSubjBuf) {  // staged in LDS, alone when groups are off (gap 0)
                 if (ngg > 0) run_global_group();  // the buffer holds its table
                 if (ng == group_cap || used + (uint32_t)slen > (uint32_t)kSubjBuf) run_group();
-                // letters at [used, used + slen) by 16-byte loads, then zeros up to the next
-                // 64-aligned start (at least 4) or the buffer's end pad
-                const uint64_t a0 = o0 & ~uint64_t(15);
-                const int lead = (int)(o0 - a0);
-                const int nvec = (lead + slen + 15) >> 4;
-                const uint4 *src = reinterpret_cast<const uint4 *>(a.res + a0);
-                uint8_t *dst = w.subj + used;
-                for (int v = lane; v < nvec; v += 64) {
-                    const uint4 x = src[v];
-                    const uint32_t xw[4] = {x.x, x.y, x.z, x.w};
-#pragma unroll
-                    for (int kk = 0; kk < 16; kk++) {
-                        const int p = v * 16 + kk - lead;
-                        if (p >= 0 && p < slen) dst[p] = (uint8_t)((xw[kk >> 2] >> (8 * (kk & 3))) << 1);  // doubled
-                    }
-                }
+                // letters at [used, used + slen), doubled, then zeros up to the next 64-aligned
+                // start (at least 4 zeros) or the buffer's end pad: one dword per lane, shifted
+                // into place from two aligned loads (the DB keeps 64 zero bytes past its end, and
+                // the source offset is clamped to the subject's last letter, so every load stays
+                // inside the DB)
                 const uint32_t next = (used + (uint32_t)slen + 4u + 63u) & ~63u;
                 const uint32_t zend = next < (uint32_t)(kSubjBuf + kSeqPad) ? next : (uint32_t)(kSubjBuf + kSeqPad);
-                for (uint32_t i = used + (uint32_t)slen + lane; i < zend; i += 64) w.subj[i] = 0;
+                const auto *R32 = gmem(reinterpret_cast<const uint32_t *>(a.res));
+                uint32_t *dst = reinterpret_cast<uint32_t *>(w.subj + used);
+                for (uint32_t j = lane; j < (zend - used) >> 2; j += 64) {
+                    const int rem = slen - 4 * (int)j;  // letters from this dword on
+                    const uint64_t src = o0 + (uint64_t)max(min(4 * (int)j, slen - 1), 0);
+                    const uint32_t lo = R32[src >> 2], hi = R32[(src >> 2) + 1];
+                    const uint32_t x = (uint32_t)((((uint64_t)hi << 32) | lo) >> (8u * ((uint32_t)src & 3u)));
+                    const uint32_t keep = rem >= 4 ? 0xFFFFFFFFu : rem <= 0 ? 0u : (1u << (8 * rem)) - 1u;
+                    dst[j] = (x & keep) << 1;  // doubled (letters are below 128)
+                }
                 if (lane == 0) {
                     w.grp->sid[ng] = sid;
                     w.grp->start[ng] = used;
